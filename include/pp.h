@@ -1,0 +1,170 @@
+/*
+ * pp.h -- C ABI of libpp_hip.so, the MI355X (gfx950) implementation of the
+ * PatchyParticles rototranslation Monte Carlo hot path: the Kern-Frenkel pair
+ * test, the per-particle energy with cell-list neighbour search and the cell
+ * rebuild, driven as checkerboard sweeps on the GPU.
+ *
+ * Plain C types only: pointers, sizes, doubles.  Host arrays are always in
+ * particle-index order, positions as r[3*i+k], orientation matrices by rows
+ * as O[9*i+3*row+col] (row k of the reference's PatchyParticle::orientation,
+ * defs.h:47-56).
+ *
+ * Every call is synchronous with respect to the host and returns a pp_status;
+ * pp_last_error() gives a human-readable message for the last failure on the
+ * calling thread.  A context is not thread-safe: one thread at a time.
+ *
+ * Which reference interface each entry point replaces (file:line in the
+ * reference, PatchyParticles):
+ *   pp_create / pp_create_batch   system_init + cells_init/cells_fill + MC_init
+ *                                 (system.c:58-251, cells.c:17-63, MC.c:85-131)
+ *   pp_sweeps                     syst.do_ensemble called once per step
+ *                                 (main.c:119 -> do_NVT MC.c:19-29 /
+ *                                 do_NPT MC.c:60-68, MC_move_rototranslate
+ *                                 MC.c:311-340, MC_change_volume MC.c:450-513)
+ *   pp_energy                     sum over particles of MC_energy
+ *                                 (MC.c:272-309; main.c:54-61)
+ *   pp_check_energy               MC_check_energy + cells_check
+ *                                 (MC.c:766-784, cells.c:65-76; main.c:86-87)
+ *   pp_download / pp_upload       the System particle arrays read by
+ *                                 output_save (output.c:302-317) / written by
+ *                                 the .rrr loader (system.c:213-245)
+ *   pp_get_counters(_all)         syst->tries/accepted/energy/V/box read by
+ *                                 output_print (output.c:119-196)
+ *   pp_reset_counters             utils_reset_acceptance_counters (utils.c:276-282)
+ *   pp_energy_snapshot            MC_energy over every particle of a given
+ *                                 snapshot, without a context (MC.c:272-309)
+ *   pp_generate                   the generator program's random insertion
+ *                                 (generator.c:10-124)
+ */
+#ifndef PP_H
+#define PP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum pp_status {
+	PP_OK = 0,
+	PP_ERR_ARG = 1,         /* invalid argument */
+	PP_ERR_HIP = 2,         /* HIP runtime error (no device, launch failure...) */
+	PP_ERR_OVERLAP = 3,     /* configuration contains an overlap (main.c:59, MC.c:779) */
+	PP_ERR_BOX = 4,         /* box too small for the cell grid (< 4 (1+delta) per side) */
+	PP_ERR_CELL_CAP = 5,    /* a cell holds more particles than the sweep kernel stages */
+	PP_ERR_ENERGY = 6,      /* tracked energy disagrees with the recomputed one (MC.c:780-782) */
+	PP_ERR_NOMEM = 7,       /* host or device allocation failed */
+	PP_ERR_UNSUPPORTED = 8, /* ensemble other than 0 (NVT) / 5 (NPT), or Dynamics != 0 */
+	PP_ERR_CELLS = 9        /* cell population != N (cells.c:74) */
+} pp_status;
+
+/* Ensembles (defs.h:21-25). */
+#define PP_NVT 0
+#define PP_NPT 5
+
+/* Physical and move parameters of one system, the reference's input keys. */
+typedef struct pp_params {
+	int ensemble;              /* Ensemble: PP_NVT or PP_NPT */
+	double temperature;        /* Temperature */
+	double disp_max;           /* Disp_max: displacement uniform in +-Disp_max/2 per axis */
+	double theta_max;          /* Theta_max: rotation angle uniform in [0, Theta_max) */
+	double kf_delta;           /* KF_delta: bond range 1 + delta */
+	double kf_cosmax;          /* KF_cosmax: patch half-angle cosine */
+	double pressure;           /* P (NPT) */
+	double rescale_factor_max; /* rescale_factor_max (NPT): ln V step */
+} pp_params;
+
+/* Options of the GPU engine (no reference counterpart). */
+typedef struct pp_options {
+	uint64_t seed;           /* Philox key; the whole trajectory is a function of it */
+	int threads_per_cell;    /* lanes that scan one cell's neighbourhood: 8/16/32/64 (0: 64) */
+	int block_size;          /* threads per workgroup: 64/128/256 (0: 256) */
+	double cell_occupancy;   /* target particles per sweep cell; 0: smallest cells (side >= 1+delta) */
+	int device;              /* HIP device ordinal */
+} pp_options;
+
+typedef struct pp_counters {
+	long long tries[2];    /* [0] rototranslations, [1] volume moves, since the last reset */
+	long long accepted[2];
+	long long bonds;       /* tracked number of bonded pairs: energy = -bonds */
+	long long sweeps;      /* sweeps done since creation */
+	double box[3];
+	double volume;         /* V as the reference tracks it (MC.c:500) */
+} pp_counters;
+
+typedef struct pp_ctx pp_ctx;
+
+/* One system.  r: 3n doubles, O: 9n doubles (rows), box: 3 doubles.
+ * Fails with PP_ERR_OVERLAP if the configuration overlaps (main.c:57-60). */
+pp_status pp_create(const pp_params *params, const pp_options *opts, int n, const double *r,
+                    const double *O, const double box[3], pp_ctx **out);
+
+/* n_rep independent systems in one context; all kernels run on all of them
+ * at once.  params[n_rep], n[n_rep]; r, O concatenated in replica order;
+ * box[3*n_rep]. */
+pp_status pp_create_batch(int n_rep, const pp_params *params, const pp_options *opts, const int *n,
+                          const double *r, const double *O, const double *box, pp_ctx **out);
+
+/* Advance every replica by n sweeps.  A sweep is N single-particle
+ * rototranslation attempts (MC.c:21) done as 8 checkerboard colour passes,
+ * plus, under NPT, one volume-move attempt (expected once per sweep in
+ * do_NPT, MC.c:62-63). */
+pp_status pp_sweeps(pp_ctx *ctx, long long n);
+
+/* Recompute the energy of replica rep from scratch: bonds[i] = number of
+ * bonded neighbours of particle i, overlap[i] = 1 if i overlaps a neighbour
+ * (then bonds[i] = 0, as MC_energy returns 0).  Either array may be NULL.
+ * total_bonds = number of bonded pairs, any_overlap = OR of overlap[]. */
+pp_status pp_energy(pp_ctx *ctx, int rep, int32_t *bonds, uint8_t *overlap, long long *total_bonds,
+                    int *any_overlap);
+
+/* MC_check_energy for every replica: recompute, compare with the tracked
+ * energy (relative tolerance 1e-5), re-orthonormalise every orientation by
+ * Gram-Schmidt, count the cell population.  energy_per_rep may be NULL. */
+pp_status pp_check_energy(pp_ctx *ctx, double *energy_per_rep);
+
+pp_status pp_download(pp_ctx *ctx, int rep, double *r, double *O, double box[3]);
+pp_status pp_upload(pp_ctx *ctx, int rep, const double *r, const double *O, const double box[3]);
+
+pp_status pp_get_counters(pp_ctx *ctx, int rep, pp_counters *out);
+/* pp_get_counters for every replica at once: out[pp_num_replicas(ctx)]. */
+pp_status pp_get_counters_all(pp_ctx *ctx, pp_counters *out);
+pp_status pp_reset_counters(pp_ctx *ctx);
+int pp_num_replicas(const pp_ctx *ctx);
+int pp_num_particles(const pp_ctx *ctx, int rep);
+
+/* Per-particle energy of a snapshot without a context (GPU, same kernels). */
+pp_status pp_energy_snapshot(int n, const double *r, const double *O, const double box[3], double kf_delta,
+                             double kf_cosmax, int32_t *bonds, uint8_t *overlap, long long *total_bonds,
+                             int *any_overlap);
+
+/* Random sequential insertion of n non-overlapping unit spheres in a cubic
+ * box of side (n/rho)^(1/3), each with a uniformly random orientation
+ * (generator.c:43-84, the reference's generator program; host code, no GPU).
+ * seed < 0 reproduces the reference generator's stream; rho <= 0.7.
+ * r: 3n, O: 9n, box: 3 doubles. */
+pp_status pp_generate(int n, double rho, long seed, double *r, double *O, double box[3]);
+
+/* Energy trace: with capacity > 0, the bonded-pair count of every replica is
+ * recorded after each of the next `capacity` sweeps (the energy.dat series
+ * of output_print, output.c:124, at every sweep); capacity 0 turns it off.
+ * pp_get_trace copies up to `capacity` entries, oldest first. */
+pp_status pp_set_trace(pp_ctx *ctx, int capacity);
+pp_status pp_get_trace(pp_ctx *ctx, int rep, long long *bonds, int capacity, int *count);
+
+/* Kernel timing with HIP events on the context's stream (off by default).
+ * name: "pass", "rebuild", "energy", "volume" or NULL for the sum. */
+pp_status pp_set_timing(pp_ctx *ctx, int on);
+pp_status pp_get_timing(pp_ctx *ctx, const char *name, long long *launches, double *total_ms);
+
+/* Grid of replica rep: cells per side and cell side lengths. */
+pp_status pp_get_grid(pp_ctx *ctx, int rep, int n_side[3], double side[3]);
+
+void pp_destroy(pp_ctx *ctx);
+const char *pp_strerror(pp_status s);
+const char *pp_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,11 @@
+"""MI355X (gfx950) implementation of the PatchyParticles rototranslation Monte
+Carlo hot path: Kern-Frenkel pair test, per-particle energy with cell lists,
+cell rebuild, as checkerboard sweeps in hand-written HIP kernels behind the C
+ABI of include/pp.h (libpp_hip.so).
+"""
+from ._lib import EXPORTS, LIB_PATH, PPError, energy_snapshot, generate, lib
+from .rrr import read_rrr, write_rrr
+from .system import NPT, NVT, System
+
+__all__ = ["System", "NVT", "NPT", "energy_snapshot", "generate", "read_rrr", "write_rrr", "PPError", "lib", "LIB_PATH",
+           "EXPORTS"]

@@ -1,0 +1,91 @@
+"""Build the gfx950 library ``libpp_hip.so`` (and the C host programs) in-tree.
+
+    python -m patchyparticles_amd.build        # or __graft_entry__.build()
+
+hipcc flags:
+  --offload-arch=gfx950          MI355X only, one code path
+  -ffp-contract=off              no fused multiply-add: the pair test must round
+                                 where the reference's C source rounds
+  -Xclang -target-feature -Xclang -packed-fp32-ops
+                                 keep the compiler from emitting packed FP32 ops
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import sys
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(PKG)
+CSRC = os.path.join(PKG, "csrc")
+HOST = os.path.join(CSRC, "host")
+LIB = os.path.join(PKG, "libpp_hip.so")
+BIN = os.path.join(PKG, "bin")
+
+HIP_SOURCES = ["pp_kernels.hip", "pp_api.cpp", "pp_host.cpp"]
+HIPCC_FLAGS = [
+    "--offload-arch=gfx950",
+    "-O3",
+    "-std=c++17",
+    "-fPIC",
+    "-ffp-contract=off",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wall",
+    "-Wno-unused-result",
+]
+
+
+def hipcc():
+    for cand in (os.environ.get("HIPCC"), shutil.which("hipcc"), "/opt/rocm/bin/hipcc"):
+        if cand and os.path.exists(cand):
+            return cand
+    raise FileNotFoundError("hipcc not found")
+
+
+def _run(cmd, **kw):
+    print("+", " ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True, **kw)
+
+
+def build_lib(verbose=True):
+    objs = []
+    obj_dir = os.path.join(PKG, "_build")
+    os.makedirs(obj_dir, exist_ok=True)
+    for src in HIP_SOURCES:
+        obj = os.path.join(obj_dir, os.path.splitext(src)[0] + ".o")
+        lang = ["-x", "hip"] if src.endswith(".cpp") else []
+        _run([hipcc(), *HIPCC_FLAGS, *lang, "-c", os.path.join(CSRC, src), "-o", obj])
+        objs.append(obj)
+    tmp = LIB + ".tmp"
+    _run([hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", tmp])
+    os.replace(tmp, LIB)
+    return LIB
+
+
+def build_host():
+    """C host programs (input file / .rrr / output driver, generator)."""
+    srcs = [f for f in ("pp_main.c", "pp_input.c", "pp_rrr.c", "pp_output.c") if os.path.exists(os.path.join(HOST, f))]
+    if not srcs:
+        return None
+    os.makedirs(BIN, exist_ok=True)
+    inc = ["-I", os.path.join(ROOT, "include"), "-I", HOST]
+    common = [os.path.join(HOST, f) for f in ("pp_input.c", "pp_rrr.c", "pp_output.c") if os.path.exists(os.path.join(HOST, f))]
+    out = os.path.join(BIN, "PatchyParticles_hip")
+    _run(["gcc", "-O2", "-std=gnu99", "-Wall", *inc, os.path.join(HOST, "pp_main.c"), *common,
+          "-L", PKG, "-lpp_hip", "-Wl,-rpath,$ORIGIN/..", "-lm", "-o", out])
+    gen = os.path.join(HOST, "pp_generator.c")
+    if os.path.exists(gen):
+        _run(["gcc", "-O2", "-std=gnu99", "-Wall", *inc, gen, os.path.join(HOST, "pp_rrr.c"), "-lm",
+              "-o", os.path.join(BIN, "generator_hip")])
+    return out
+
+
+def build_all():
+    build_lib()
+    build_host()
+
+
+if __name__ == "__main__":
+    build_all()
+    sys.exit(0)

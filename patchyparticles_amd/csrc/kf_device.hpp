@@ -1,0 +1,236 @@
+// kf_device.hpp -- device-side geometry of the Kern-Frenkel hot path for gfx950.
+//
+// Floating point: this header is compiled with -ffp-contract=off (and the
+// pragma below), so every product and sum rounds where it is written.  The
+// pair test reproduces the reference's operation order (MC.c:232-266,
+// utils.h:14-21) so that bond and overlap classifications are bit-identical
+// to the reference on the same FP64 snapshot.
+#pragma once
+#pragma clang fp contract(off)
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace ppd {
+
+// Interaction classes (defs.h:31-33).
+constexpr int KF_OVERLAP = -1;
+constexpr int KF_NO_BOND = 0;
+constexpr int KF_BOND = 1;
+
+// Tetrahedral base patches, uploaded bit-for-bit from the host (they are the
+// host's normalize() of (+-h,+-h,+-h), h = 0.5/sqrt(3), system.c:20-31).
+struct Base {
+	double b[4][3];
+};
+
+// patch_i = O * b_i over the rows of O, left-to-right sums (utils.h:17-21).
+__device__ __forceinline__ void kf_patches(const double O[9], const Base &B, double P[4][3]) {
+#pragma unroll
+	for(int i = 0; i < 4; i++)
+#pragma unroll
+		for(int k = 0; k < 3; k++) P[i][k] = O[3 * k + 0] * B.b[i][0] + O[3 * k + 1] * B.b[i][1] + O[3 * k + 2] * B.b[i][2];
+}
+
+// Minimum-image separation d = r_q - r_p, component by component.  The image
+// index comes from rint(d * (1/L)) instead of rint(d / L): the two agree
+// whenever the component can matter (|d - kL| < 1+delta <= L/3 keeps d/L at
+// least 1/6 away from a rounding boundary), and when they disagree the
+// component is ~L/2 in both forms, so d^2 exceeds (1+delta)^2 either way.
+// The subtraction d - L*k is then the reference's exact expression.
+__device__ __forceinline__ double kf_min_image(double d, double L, double invL) {
+	double k = __builtin_rint(d * invL);
+	double t = L * k;
+	return d - t;
+}
+
+__device__ __forceinline__ double kf_dot(double a0, double a1, double a2, double b0, double b1, double b2) {
+	return a0 * b0 + a1 * b1 + a2 * b2;
+}
+
+// Angular part of MC_would_interact (MC.c:242-262): d = (dx,dy,dz) is the
+// minimum-image r_q - r_p with rc2 > d2 >= 1.  Returns true on a bond.
+__device__ __forceinline__ bool kf_bond(double dx, double dy, double dz, double d2, const double P[4][3],
+                                        const double Q[4][3], double cosmax) {
+	double len = __builtin_sqrt(d2);
+	dx = dx / len;
+	dy = dy / len;
+	dz = dz / len;
+	bool bond = false;
+#pragma unroll
+	for(int a = 0; a < 4; a++) {
+		double cp = kf_dot(dx, dy, dz, P[a][0], P[a][1], P[a][2]);
+		if(cp > cosmax) {
+#pragma unroll
+			for(int b = 0; b < 4; b++) {
+				double cq = -kf_dot(dx, dy, dz, Q[b][0], Q[b][1], Q[b][2]);
+				bond |= (cq > cosmax);
+			}
+		}
+	}
+	return bond;
+}
+
+// The same test split in its two halves.  MC.c:250-262 returns a bond as
+// soon as some patch a of p has d^.p_a > cosmax and some patch b of q has
+// -d^.q_b > cosmax (the inner condition does not depend on a), i.e.
+// bond <=> any_p && any_q.  Evaluating any_q only when any_p holds lets the
+// caller skip loading q's orientation for most pairs.
+//
+// d^ = d / |d|: one sqrt, three divisions (MC.c:244-248).
+__device__ __forceinline__ void kf_unit(double &dx, double &dy, double &dz, double d2) {
+	double len = __builtin_sqrt(d2);
+	dx = dx / len;
+	dy = dy / len;
+	dz = dz / len;
+}
+
+// any patch of p (12 doubles, patch-major) with d^.p_a > cosmax
+__device__ __forceinline__ bool kf_any_p(double ux, double uy, double uz, const double *P, double cosmax) {
+	bool any = false;
+#pragma unroll
+	for(int a = 0; a < 4; a++) any |= kf_dot(ux, uy, uz, P[3 * a], P[3 * a + 1], P[3 * a + 2]) > cosmax;
+	return any;
+}
+
+// any patch q_b = Oq * b_b of q with -(d^.q_b) > cosmax
+__device__ __forceinline__ bool kf_any_q(double ux, double uy, double uz, const double Oq[9], const Base &B,
+                                         double cosmax) {
+	bool any = false;
+#pragma unroll
+	for(int b = 0; b < 4; b++) {
+		double q0 = Oq[0] * B.b[b][0] + Oq[1] * B.b[b][1] + Oq[2] * B.b[b][2];
+		double q1 = Oq[3] * B.b[b][0] + Oq[4] * B.b[b][1] + Oq[5] * B.b[b][2];
+		double q2 = Oq[6] * B.b[b][0] + Oq[7] * B.b[b][1] + Oq[8] * B.b[b][2];
+		any |= -kf_dot(ux, uy, uz, q0, q1, q2) > cosmax;
+	}
+	return any;
+}
+
+// Cell coordinate of x on a grid of n cells over [s, s+L) (periodic).
+// Deterministic: the rebuild and the in-cell test of a trial move call the
+// same function on the same doubles.
+__device__ __forceinline__ int cell_coord(double x, double s, double invL, int n) {
+	double t = (x - s) * invL;
+	t = t - __builtin_floor(t);
+	int i = (int) (t * (double) n);
+	return i < n ? i : n - 1;
+}
+
+// ---------------------------------------------------------------------
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11): 10 rounds, multipliers
+// 0xD2511F53 / 0xCD9E8D57, Weyl key increments 0x9E3779B9 / 0xBB67AE85.
+struct u32x4 {
+	uint32_t x, y, z, w;
+};
+
+__device__ __forceinline__ u32x4 philox(u32x4 c, uint32_t k0, uint32_t k1) {
+#pragma unroll
+	for(int i = 0; i < 10; i++) {
+		uint32_t hi0 = __umulhi(0xD2511F53u, c.x), lo0 = 0xD2511F53u * c.x;
+		uint32_t hi1 = __umulhi(0xCD9E8D57u, c.z), lo1 = 0xCD9E8D57u * c.z;
+		c = u32x4{hi1 ^ c.y ^ k0, lo1, hi0 ^ c.w ^ k1, lo0};
+		k0 += 0x9E3779B9u;
+		k1 += 0xBB67AE85u;
+	}
+	return c;
+}
+
+// Uniform double in [0, 1) with 32 random bits.
+__device__ __forceinline__ double u01(uint32_t v) { return (double) v * 2.3283064365386962890625e-10; }
+
+// Counter layout (the key is the 64-bit seed):
+//   x: cell index within the replica (< 2^24; or 0 for per-replica draws)
+//   y: stream tag (8 bits) | colour (3 bits) << 8 | trial index (16 bits) << 16
+//   z: sweep index, low 32 bits
+//   w: replica (20 bits) | sweep index bits 32..43 << 20
+enum : uint32_t {
+	TAG_TRIAL_A = 0,
+	TAG_TRIAL_B = 1,
+	TAG_LEARN_A = 2,
+	TAG_LEARN_B = 3,
+	TAG_AXIS = 4,  // rejection block index in the high byte of the cell word
+	TAG_AXIS_FALLBACK = 5,
+	TAG_SHIFT = 8,
+	TAG_COLOUR = 9,
+	TAG_VOLUME = 10
+};
+
+__device__ __forceinline__ u32x4 rng(uint64_t seed, int rep, uint64_t sweep, uint32_t cell, uint32_t tag,
+                                     uint32_t colour, uint32_t trial) {
+	u32x4 c{cell, tag | (colour << 8) | (trial << 16), (uint32_t) sweep,
+	        ((uint32_t) rep & 0xFFFFFu) | ((uint32_t) (sweep >> 32) << 20)};
+	return philox(c, (uint32_t) seed, (uint32_t) (seed >> 32));
+}
+
+// Random colour order of a sweep: Fisher-Yates over 8 colours, the
+// permutation kept as eight 4-bit fields of one word; 16 random bits per
+// step from one Philox block.  Returns the colour of pass k.
+__host__ __device__ __forceinline__ uint32_t colour_perm_from(const uint32_t w4[4]) {
+	uint32_t perm = 0x76543210u;
+	for(int i = 7; i > 0; i--) {
+		const int s = 7 - i;
+		uint32_t w = (w4[s & 3] >> ((s >> 2) * 16)) & 0xFFFFu;
+		uint32_t j = (w * (uint32_t) (i + 1)) >> 16;
+		uint32_t pi = (perm >> (4 * i)) & 0xFu, pj = (perm >> (4 * j)) & 0xFu;
+		perm &= ~((0xFu << (4 * i)) | (0xFu << (4 * j)));
+		perm |= (pj << (4 * i)) | (pi << (4 * j));
+	}
+	return perm;
+}
+
+__device__ __forceinline__ int colour_of_pass(uint64_t seed, int rep, uint64_t sweep, int k) {
+	u32x4 v = rng(seed, rep, sweep, 0, TAG_COLOUR, 0, 0);
+	const uint32_t w4[4] = {v.x, v.y, v.z, v.w};
+	return (int) ((colour_perm_from(w4) >> (4 * k)) & 0xFu);
+}
+
+// Rodrigues rotation about a unit axis by t (get_rotation_matrix,
+// utils.c:186-202), then O' = O R (matrix_matrix_multiplication,
+// utils.c:172-184; utils_rotate_matrix utils.c:204-209).
+__device__ __forceinline__ void rotate_body(const double O[9], const double a[3], double st, double ct,
+                                            double On[9]) {
+	double uct = 1. - ct;
+	double R[9];
+	R[0] = a[0] * a[0] + (1. - a[0] * a[0]) * ct;
+	R[1] = a[0] * a[1] * uct - a[2] * st;
+	R[2] = a[0] * a[2] * uct + a[1] * st;
+	R[3] = a[0] * a[1] * uct + a[2] * st;
+	R[4] = a[1] * a[1] + (1. - a[1] * a[1]) * ct;
+	R[5] = a[1] * a[2] * uct - a[0] * st;
+	R[6] = a[0] * a[2] * uct - a[1] * st;
+	R[7] = a[1] * a[2] * uct + a[0] * st;
+	R[8] = a[2] * a[2] + (1. - a[2] * a[2]) * ct;
+#pragma unroll
+	for(int i = 0; i < 3; i++)
+#pragma unroll
+		for(int j = 0; j < 3; j++) On[3 * i + j] = O[3 * i + 0] * R[j] + O[3 * i + 1] * R[3 + j] + O[3 * i + 2] * R[6 + j];
+}
+
+// Gram-Schmidt of the rows, then each normalised (utils.c:91-113, 31-36).
+__device__ __forceinline__ void gram_schmidt_rows(double O[9]) {
+	double *v1 = O, *v2 = O + 3, *v3 = O + 6;
+	double n1 = kf_dot(v1[0], v1[1], v1[2], v1[0], v1[1], v1[2]);
+	double c21 = kf_dot(v2[0], v2[1], v2[2], v1[0], v1[1], v1[2]) / n1;
+	v2[0] -= c21 * v1[0];
+	v2[1] -= c21 * v1[1];
+	v2[2] -= c21 * v1[2];
+	double s31 = kf_dot(v3[0], v3[1], v3[2], v1[0], v1[1], v1[2]);
+	double s32 = kf_dot(v3[0], v3[1], v3[2], v2[0], v2[1], v2[2]);
+	double n2 = kf_dot(v2[0], v2[1], v2[2], v2[0], v2[1], v2[2]);
+	double c31 = s31 / n1;
+	double c32 = s32 / n2;
+	v3[0] -= c31 * v1[0] + c32 * v2[0];
+	v3[1] -= c31 * v1[1] + c32 * v2[1];
+	v3[2] -= c31 * v1[2] + c32 * v2[2];
+#pragma unroll
+	for(int r = 0; r < 3; r++) {
+		double *v = O + 3 * r;
+		double len = __builtin_sqrt(kf_dot(v[0], v[1], v[2], v[0], v[1], v[2]));
+		v[0] = v[0] / len;
+		v[1] = v[1] / len;
+		v[2] = v[2] / len;
+	}
+}
+
+}  // namespace ppd

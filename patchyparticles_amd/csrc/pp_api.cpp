@@ -1,0 +1,741 @@
+// pp_api.cpp -- the C ABI of libpp_hip.so (declared in include/pp.h).
+//
+// A context owns one HIP stream and the device state of one or more
+// replicas; every call is synchronous for the caller.  The device copy of the
+// replica table is authoritative (kernels update buffer selectors, grids,
+// volumes and tracked energies); host code reads it back before changing it.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/pp.h"
+#include "pp_internal.h"
+
+using namespace ppd;
+
+namespace {
+
+thread_local std::string g_err;
+
+pp_status fail(pp_status s, const char *fmt, ...) {
+	char buf[512];
+	va_list ap;
+	va_start(ap, fmt);
+	vsnprintf(buf, sizeof buf, fmt, ap);
+	va_end(ap);
+	g_err = buf;
+	return s;
+}
+
+#define HIPCHECK(call)                                                                                    \
+	do {                                                                                                  \
+		hipError_t e_ = (call);                                                                           \
+		if(e_ != hipSuccess) return fail(PP_ERR_HIP, "%s failed: %s", #call, hipGetErrorString(e_));      \
+	} while(0)
+
+// normalize() of the tetrahedral base vectors (system.c:20-31, utils.c:31-36)
+Base host_base_patches() {
+	Base B;
+	const double h = 0.5 / std::sqrt(3.);
+	const int sgn[4][3] = {{-1, -1, 1}, {1, -1, -1}, {1, 1, 1}, {-1, 1, -1}};
+	for(int i = 0; i < 4; i++) {
+		double v[3];
+		for(int k = 0; k < 3; k++) v[k] = sgn[i][k] > 0 ? h : -h;
+		double len = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+		for(int k = 0; k < 3; k++) B.b[i][k] = v[k] / len;
+	}
+	return B;
+}
+
+}  // namespace
+
+struct pp_ctx {
+	int device = 0;
+	hipStream_t stream = nullptr;
+	int R = 0;
+	std::vector<Rep> reps;  // host copy (refreshed from the device when needed)
+	std::vector<int> np, p_off, act_off;
+	DevState d{};
+	int T = 32, block = 256;
+	long long sweep = 0;
+	bool any_npt = false;
+	long long trace_len = -1;  // entries written to d.trace; -1 = tracing off
+	long long trace_from = 0;  // sweep index at which tracing began
+	// timing
+	bool timing = false;
+	std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> pending;  // (kind, events)
+	long long t_launch[4] = {0, 0, 0, 0};
+	double t_ms[4] = {0, 0, 0, 0};
+	std::vector<void *> allocs;
+
+	~pp_ctx() {
+		for(auto &p : pending) {
+			hipEventDestroy(p.second.first);
+			hipEventDestroy(p.second.second);
+		}
+		for(void *p : allocs) hipFree(p);
+		if(stream) hipStreamDestroy(stream);
+	}
+};
+
+namespace {
+
+enum { TK_PASS = 0, TK_REBUILD = 1, TK_ENERGY = 2, TK_VOLUME = 3 };
+
+template <typename Tp>
+pp_status dev_alloc(pp_ctx *c, Tp **p, size_t n) {
+	void *q = nullptr;
+	hipError_t e = hipMalloc(&q, std::max<size_t>(n, 1) * sizeof(Tp));
+	if(e != hipSuccess) return fail(PP_ERR_NOMEM, "hipMalloc(%zu bytes): %s", n * sizeof(Tp), hipGetErrorString(e));
+	c->allocs.push_back(q);
+	*p = (Tp *) q;
+	return PP_OK;
+}
+
+#define TRY(x)                          \
+	do {                                \
+		pp_status s_ = (x);             \
+		if(s_ != PP_OK) return s_;      \
+	} while(0)
+
+pp_status pull_reps(pp_ctx *c) {
+	HIPCHECK(hipMemcpyAsync(c->reps.data(), c->d.reps, sizeof(Rep) * c->R, hipMemcpyDeviceToHost, c->stream));
+	HIPCHECK(hipStreamSynchronize(c->stream));
+	return PP_OK;
+}
+
+pp_status push_reps(pp_ctx *c) {
+	HIPCHECK(hipMemcpyAsync(c->d.reps, c->reps.data(), sizeof(Rep) * c->R, hipMemcpyHostToDevice, c->stream));
+	HIPCHECK(hipStreamSynchronize(c->stream));
+	return PP_OK;
+}
+
+pp_status pull_ctr(pp_ctx *c, std::vector<unsigned long long> &h) {
+	h.resize((size_t) c->R * NCTR * NSLOT);
+	HIPCHECK(hipMemcpyAsync(h.data(), c->d.ctr, h.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+	                        c->stream));
+	HIPCHECK(hipStreamSynchronize(c->stream));
+	return PP_OK;
+}
+
+long long ctr_sum(const std::vector<unsigned long long> &h, int r, int row) {
+	unsigned long long s = 0;
+	for(int k = 0; k < NSLOT; k++) s += h[((size_t) r * NCTR + row) * NSLOT + k];
+	return (long long) s;
+}
+
+pp_status check_err(pp_ctx *c) {
+	int e = 0;
+	HIPCHECK(hipMemcpyAsync(&e, c->d.err, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+	HIPCHECK(hipStreamSynchronize(c->stream));
+	HIPCHECK(hipGetLastError());
+	if(e & ERR_CELL_CAP) return fail(PP_ERR_CELL_CAP, "a sweep cell holds more than %d particles", CELL_CAP);
+	if(e & ERR_BOX) return fail(PP_ERR_BOX, "box too small for the cell grid");
+	return PP_OK;
+}
+
+void tic(pp_ctx *c, int kind, hipEvent_t *a) {
+	if(!c->timing) return;
+	hipEventCreate(a);
+	hipEventRecord(*a, c->stream);
+	(void) kind;
+}
+
+void toc(pp_ctx *c, int kind, hipEvent_t a) {
+	if(!c->timing) return;
+	hipEvent_t b;
+	hipEventCreate(&b);
+	hipEventRecord(b, c->stream);
+	c->pending.push_back({kind, {a, b}});
+}
+
+pp_status collect_timing(pp_ctx *c) {
+	if(c->pending.empty()) return PP_OK;
+	HIPCHECK(hipStreamSynchronize(c->stream));
+	for(auto &p : c->pending) {
+		float ms = 0.f;
+		hipEventElapsedTime(&ms, p.second.first, p.second.second);
+		c->t_ms[p.first] += ms;
+		c->t_launch[p.first] += 1;
+		hipEventDestroy(p.second.first);
+		hipEventDestroy(p.second.second);
+	}
+	c->pending.clear();
+	return PP_OK;
+}
+
+// rebuild every replica on its grid g (zero-cost for the trajectory: the
+// in-cell order after a rebuild depends only on particle indices)
+void rebuild_now(pp_ctx *c) {
+	hipEvent_t a{};
+	tic(c, TK_REBUILD, &a);
+	launch_rebuild(c->d, 0, c->stream);
+	launch_flip(c->d, c->stream);
+	toc(c, TK_REBUILD, a);
+}
+
+pp_status energy_now(pp_ctx *c, int per_particle) {
+	rebuild_now(c);
+	hipEvent_t a{};
+	tic(c, TK_ENERGY, &a);
+	launch_energy(c->d, 0, per_particle, c->T, c->stream);
+	toc(c, TK_ENERGY, a);
+	HIPCHECK(hipStreamSynchronize(c->stream));
+	return PP_OK;
+}
+
+pp_status validate_params(const pp_params &p, int r) {
+	if(p.ensemble != PP_NVT && p.ensemble != PP_NPT)
+		return fail(PP_ERR_UNSUPPORTED, "replica %d: Ensemble %d not supported (0 = NVT, 5 = NPT)", r, p.ensemble);
+	if(!(p.temperature > 0.)) return fail(PP_ERR_ARG, "replica %d: Temperature must be > 0", r);
+	if(!(p.kf_delta > 0.)) return fail(PP_ERR_ARG, "replica %d: KF_delta must be > 0", r);
+	if(!(p.disp_max >= 0.) || !(p.theta_max >= 0.)) return fail(PP_ERR_ARG, "replica %d: negative move size", r);
+	if(p.ensemble == PP_NPT && !(p.rescale_factor_max >= 0.))
+		return fail(PP_ERR_ARG, "replica %d: rescale_factor_max must be >= 0", r);
+	return PP_OK;
+}
+
+// common constructor: energy_only selects grids of >= 3 cells of any parity
+pp_status create_impl(int R, const pp_params *prm, const pp_options *opts, const int *n, const double *r,
+                      const double *O, const double *box, bool energy_only, pp_ctx **out) {
+	if(!out) return fail(PP_ERR_ARG, "out is NULL");
+	*out = nullptr;
+	if(R <= 0 || !prm || !n || !box) return fail(PP_ERR_ARG, "bad replica arguments");
+	pp_options o{};
+	if(opts) o = *opts;
+	if(o.threads_per_cell == 0) o.threads_per_cell = 64;
+	if(o.block_size == 0) o.block_size = 256;
+	const int Tv = o.threads_per_cell;
+	if(Tv != 8 && Tv != 16 && Tv != 32 && Tv != 64) return fail(PP_ERR_ARG, "threads_per_cell must be 8/16/32/64");
+	if(o.block_size % 64 || o.block_size < 64 || o.block_size > 256)
+		return fail(PP_ERR_ARG, "block_size must be 64, 128, 192 or 256");
+	if(pass_lds_bytes(Tv, o.block_size) > MAX_LDS_BYTES)
+		return fail(PP_ERR_ARG, "threads_per_cell %d with block_size %d needs %zu bytes of LDS (max %zu)", Tv,
+		            o.block_size, pass_lds_bytes(Tv, o.block_size), MAX_LDS_BYTES);
+
+	long long Ntot = 0;
+	for(int k = 0; k < R; k++) {
+		if(n[k] < 0) return fail(PP_ERR_ARG, "negative particle count");
+		TRY(validate_params(prm[k], k));
+		Ntot += n[k];
+	}
+	if(Ntot > (1LL << 30)) return fail(PP_ERR_ARG, "too many particles");
+	if(Ntot > 0 && (!r || !O)) return fail(PP_ERR_ARG, "positions/orientations are NULL");
+
+	int ndev = 0;
+	if(hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return fail(PP_ERR_HIP, "no HIP device available");
+	if(o.device < 0 || o.device >= ndev) return fail(PP_ERR_ARG, "device %d out of range", o.device);
+
+	pp_ctx *c = new pp_ctx();
+	c->device = o.device;
+	c->R = R;
+	c->T = Tv;
+	c->block = o.block_size;
+	c->reps.resize(R);
+	c->np.assign(n, n + R);
+	c->p_off.resize(R);
+	c->act_off.resize(R + 1);
+	auto bail = [&](pp_status s) {
+		delete c;
+		return s;
+	};
+	if(hipSetDevice(o.device) != hipSuccess) return bail(fail(PP_ERR_HIP, "hipSetDevice failed"));
+	if(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
+		return bail(fail(PP_ERR_HIP, "hipStreamCreate failed"));
+
+	long long poff = 0, coff = 0, aoff = 0;
+	for(int k = 0; k < R; k++) {
+		Rep &rp = c->reps[k];
+		memset(&rp, 0, sizeof(Rep));
+		const pp_params &p = prm[k];
+		const double *b = box + 3 * k;
+		for(int a = 0; a < 3; a++)
+			if(!(b[a] > 0.)) return bail(fail(PP_ERR_ARG, "replica %d: box side %d is not positive", k, a));
+		const double rcut = 1. + p.kf_delta;
+		rp.rc2 = rcut * rcut;  // SQR(1. + delta) (system.c:86)
+		rp.cosmax = p.kf_cosmax;
+		rp.T = p.temperature;
+		rp.disp_max = p.disp_max;
+		rp.theta_max = p.theta_max;
+		rp.P = p.pressure;
+		rp.rfm = p.rescale_factor_max;
+		rp.ensemble = p.ensemble;
+		rp.min_n = energy_only ? 3 : 4;
+		rp.even = energy_only ? 0 : 1;
+		// a small margin keeps cells strictly wider than the bond range
+		// against the rounding of the cell-coordinate arithmetic
+		double side = rcut * (1. + 1e-9);
+		if(!energy_only && o.cell_occupancy > 0. && n[k] > 0) {
+			double V = b[0] * b[1] * b[2];
+			side = std::max(side, std::cbrt(o.cell_occupancy * V / n[k]));
+		}
+		rp.side_min = side;
+		int gn[3];
+		if(!make_grid(b, side, rp.min_n, rp.even, nullptr, gn))
+			return bail(fail(PP_ERR_BOX,
+			                 "replica %d: box (%g, %g, %g) holds fewer than %d cells of side %g per axis", k, b[0],
+			                 b[1], b[2], rp.min_n, side));
+		for(int a = 0; a < 3; a++) {
+			int cap = gn[a];
+			if(p.ensemble == PP_NPT) cap = (gn[a] + std::max(2, gn[a] / 8) + 1) & ~1;
+			rp.cap_n[a] = cap;
+			rp.g.n[a] = gn[a];
+			rp.g.shift[a] = 0.;
+		}
+		set_grid_box(rp.g, b);
+		rp.tg = rp.g;
+		rp.V = b[0] * b[1] * b[2];
+		rp.p_off = (int) poff;
+		rp.np = n[k];
+		rp.cell_off = (int) coff;
+		rp.cell_cap = rp.cap_n[0] * rp.cap_n[1] * rp.cap_n[2];
+		if(rp.cell_cap >= (1 << 24))
+			return bail(fail(PP_ERR_BOX, "replica %d: %d cells exceed the 2^24 cells one replica may hold", k,
+			                 rp.cell_cap));
+		rp.sel = 0;
+		c->p_off[k] = (int) poff;
+		c->act_off[k] = (int) aoff;
+		poff += n[k];
+		coff += rp.cell_cap;
+		aoff += (long long) (rp.cap_n[0] / 2) * (rp.cap_n[1] / 2) * (rp.cap_n[2] / 2);
+		if(p.ensemble == PP_NPT) c->any_npt = true;
+		if(coff > (1LL << 30)) return bail(fail(PP_ERR_NOMEM, "cell grid too large"));
+	}
+	c->act_off[R] = (int) aoff;
+
+	DevState &d = c->d;
+	d.Ntot = (int) Ntot;
+	d.stride = (int) std::max<long long>(Ntot, 1);
+	d.R = R;
+	d.C_total = (int) coff;
+	d.ntiles = (d.C_total + SCAN_TILE - 1) / SCAN_TILE;
+	d.total_act = (int) aoff;
+	d.base = host_base_patches();
+	d.seed = o.seed;
+	pp_status st;
+	if((st = dev_alloc(c, &d.f[0], (size_t) NCOMP * d.stride)) || (st = dev_alloc(c, &d.f[1], (size_t) NCOMP * d.stride)) ||
+	   (st = dev_alloc(c, &d.pid[0], d.stride)) || (st = dev_alloc(c, &d.pid[1], d.stride)) ||
+	   (st = dev_alloc(c, &d.reps, R)) || (st = dev_alloc(c, (int **) &d.prep, d.stride)) ||
+	   (st = dev_alloc(c, (int **) &d.act_off, R + 1)) || (st = dev_alloc(c, &d.count, d.C_total + 1)) ||
+	   (st = dev_alloc(c, &d.loc, d.C_total + 1)) || (st = dev_alloc(c, &d.start, d.C_total + 1)) ||
+	   (st = dev_alloc(c, &d.slot, d.stride)) || (st = dev_alloc(c, &d.cellid, d.stride)) ||
+	   (st = dev_alloc(c, &d.idx, d.stride)) || (st = dev_alloc(c, &d.tile_sum, std::max(d.ntiles, 1))) ||
+	   (st = dev_alloc(c, &d.ctr, (size_t) R * NCTR * NSLOT)) || (st = dev_alloc(c, &d.err, 1)) ||
+	   (st = dev_alloc(c, &d.out_bonds, d.stride)) || (st = dev_alloc(c, &d.out_ovl, d.stride)))
+		return bail(st);
+
+	// host staging: SoA in particle-index order, slot j = particle j
+	{
+		std::vector<double> h((size_t) NCOMP * d.stride, 0.);
+		std::vector<int> hp(d.stride, 0), hr(d.stride, 0);
+		for(int k = 0; k < R; k++)
+			for(int i = 0; i < n[k]; i++) {
+				long long j = c->p_off[k] + i;
+				for(int a = 0; a < 3; a++) h[(size_t) a * d.stride + j] = r[3 * j + a];
+				for(int a = 0; a < 9; a++) h[(size_t) (3 + a) * d.stride + j] = O[9 * j + a];
+				hp[j] = i;
+				hr[j] = k;
+			}
+		if(hipMemcpy(d.f[0], h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+		   hipMemcpy(d.pid[0], hp.data(), hp.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
+		   hipMemcpy((void *) d.prep, hr.data(), hr.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
+		   hipMemcpy((void *) d.act_off, c->act_off.data(), (R + 1) * sizeof(int), hipMemcpyHostToDevice) !=
+		       hipSuccess ||
+		   hipMemcpy(d.reps, c->reps.data(), R * sizeof(Rep), hipMemcpyHostToDevice) != hipSuccess)
+			return bail(fail(PP_ERR_HIP, "initial upload failed"));
+	}
+	launch_fill_i32(d.count, 0, d.C_total + 1, c->stream);
+	launch_fill_i32(d.err, 0, 1, c->stream);
+	launch_fill_u64(d.ctr, 0ull, (long long) R * NCTR * NSLOT, c->stream);
+
+	// initial energy (main.c:54-61): any overlap is an error
+	if((st = energy_now(c, 0)) != PP_OK) return bail(st);
+	if((st = check_err(c)) != PP_OK) return bail(st);
+	std::vector<unsigned long long> h;
+	if((st = pull_ctr(c, h)) != PP_OK) return bail(st);
+	if((st = pull_reps(c)) != PP_OK) return bail(st);
+	for(int k = 0; k < R; k++) {
+		// an energy-only context takes any snapshot; a sampler refuses overlaps
+		if(!energy_only && ctr_sum(h, k, C_OVL) > 0)
+			return bail(fail(PP_ERR_OVERLAP, "replica %d: initial configuration contains an overlap", k));
+		c->reps[k].bonds_base = ctr_sum(h, k, C_EBONDS) / 2;
+	}
+	if((st = push_reps(c)) != PP_OK) return bail(st);
+	*out = c;
+	return PP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+pp_status pp_create(const pp_params *params, const pp_options *opts, int n, const double *r, const double *O,
+                    const double box[3], pp_ctx **out) {
+	return create_impl(1, params, opts, &n, r, O, box, false, out);
+}
+
+pp_status pp_create_batch(int n_rep, const pp_params *params, const pp_options *opts, const int *n, const double *r,
+                          const double *O, const double *box, pp_ctx **out) {
+	return create_impl(n_rep, params, opts, n, r, O, box, false, out);
+}
+
+pp_status pp_sweeps(pp_ctx *c, long long nsw) {
+	if(!c) return fail(PP_ERR_ARG, "ctx is NULL");
+	if(nsw < 0) return fail(PP_ERR_ARG, "negative sweep count");
+	HIPCHECK(hipSetDevice(c->device));
+	for(long long s = 0; s < nsw; s++) {
+		const uint64_t sw = (uint64_t) c->sweep;
+		hipEvent_t a{};
+		tic(c, TK_REBUILD, &a);
+		long long tp = -1;
+		if(c->trace_len >= 0 && c->trace_len < c->d.trace_cap) tp = c->trace_len++;
+		launch_prep(c->d, sw, tp, c->stream);
+		launch_rebuild(c->d, 0, c->stream);
+		launch_flip(c->d, c->stream);
+		toc(c, TK_REBUILD, a);
+		for(int k = 0; k < 8; k++) {
+			hipEvent_t b{};
+			tic(c, TK_PASS, &b);
+			launch_pass(c->d, sw, k, c->T, c->block, c->stream);
+			toc(c, TK_PASS, b);
+		}
+		if(c->any_npt) {
+			hipEvent_t v{};
+			tic(c, TK_VOLUME, &v);
+			launch_vol_propose(c->d, sw, c->stream);
+			launch_rebuild(c->d, 1, c->stream);
+			launch_energy(c->d, 1, 0, c->T, c->stream);
+			launch_vol_decide(c->d, sw, c->stream);
+			toc(c, TK_VOLUME, v);
+		}
+		c->sweep++;
+		if(c->timing && c->pending.size() > 4096) TRY(collect_timing(c));
+	}
+	TRY(collect_timing(c));
+	return check_err(c);
+}
+
+pp_status pp_energy(pp_ctx *c, int rep, int32_t *bonds, uint8_t *overlap, long long *total_bonds, int *any_overlap) {
+	if(!c) return fail(PP_ERR_ARG, "ctx is NULL");
+	if(rep < 0 || rep >= c->R) return fail(PP_ERR_ARG, "replica %d out of range", rep);
+	HIPCHECK(hipSetDevice(c->device));
+	TRY(energy_now(c, 1));
+	TRY(collect_timing(c));
+	TRY(check_err(c));
+	const int n = c->np[rep], off = c->p_off[rep];
+	if(bonds && n) HIPCHECK(hipMemcpy(bonds, c->d.out_bonds + off, n * sizeof(int32_t), hipMemcpyDeviceToHost));
+	if(overlap && n) HIPCHECK(hipMemcpy(overlap, c->d.out_ovl + off, n * sizeof(uint8_t), hipMemcpyDeviceToHost));
+	std::vector<unsigned long long> h;
+	TRY(pull_ctr(c, h));
+	if(total_bonds) *total_bonds = ctr_sum(h, rep, C_EBONDS) / 2;
+	if(any_overlap) *any_overlap = ctr_sum(h, rep, C_OVL) > 0;
+	return PP_OK;
+}
+
+pp_status pp_check_energy(pp_ctx *c, double *energy_per_rep) {
+	if(!c) return fail(PP_ERR_ARG, "ctx is NULL");
+	HIPCHECK(hipSetDevice(c->device));
+	TRY(energy_now(c, 0));
+	TRY(collect_timing(c));
+	TRY(check_err(c));
+	std::vector<unsigned long long> h;
+	TRY(pull_ctr(c, h));
+	TRY(pull_reps(c));
+	// cells_check (cells.c:65-76): every particle sits in exactly one cell
+	std::vector<int> st(2);
+	for(int k = 0; k < c->R; k++) {
+		const Rep &rp = c->reps[k];
+		HIPCHECK(hipMemcpy(&st[0], c->d.start + rp.cell_off, sizeof(int), hipMemcpyDeviceToHost));
+		HIPCHECK(hipMemcpy(&st[1], c->d.start + rp.cell_off + rp.cell_cap, sizeof(int), hipMemcpyDeviceToHost));
+		if(st[1] - st[0] != rp.np)
+			return fail(PP_ERR_CELLS, "replica %d: there are %d particles in cells, there should be %d", k,
+			            st[1] - st[0], rp.np);
+	}
+	pp_status result = PP_OK;
+	for(int k = 0; k < c->R; k++) {
+		Rep &rp = c->reps[k];
+		const long long db = ctr_sum(h, k, C_DBONDS);
+		const double E_old = -(double) (rp.bonds_base + db);
+		const long long B = ctr_sum(h, k, C_EBONDS) / 2;
+		const double E = -(double) B;
+		if(ctr_sum(h, k, C_OVL) > 0) {
+			result = fail(PP_ERR_OVERLAP, "replica %d: computing energy from scratch resulted in an overlap", k);
+			continue;
+		}
+		// MC.c:780-782
+		if(std::fabs(E_old) > 1e-5 && std::fabs((E - E_old) / E_old) > 1e-5 && result == PP_OK)
+			result = fail(PP_ERR_ENERGY, "replica %d: energy check failed, old energy = %lf, new energy = %lf", k, E_old,
+			              E);
+		rp.bonds_base = B - db;
+		if(energy_per_rep) energy_per_rep[k] = E;
+	}
+	TRY(push_reps(c));
+	launch_gram_schmidt(c->d, c->stream);
+	HIPCHECK(hipStreamSynchronize(c->stream));
+	return result;
+}
+
+pp_status pp_download(pp_ctx *c, int rep, double *r, double *O, double box[3]) {
+	if(!c) return fail(PP_ERR_ARG, "ctx is NULL");
+	if(rep < 0 || rep >= c->R) return fail(PP_ERR_ARG, "replica %d out of range", rep);
+	HIPCHECK(hipSetDevice(c->device));
+	TRY(pull_reps(c));
+	const Rep &rp = c->reps[rep];
+	const int n = rp.np, off = rp.p_off, S = c->d.stride;
+	std::vector<double> h((size_t) NCOMP * std::max(n, 1));
+	std::vector<int> hp(std::max(n, 1));
+	for(int a = 0; a < NCOMP && n; a++)
+		HIPCHECK(hipMemcpy(h.data() + (size_t) a * n, c->d.f[rp.sel] + (size_t) a * S + off, n * sizeof(double),
+		                   hipMemcpyDeviceToHost));
+	if(n) HIPCHECK(hipMemcpy(hp.data(), c->d.pid[rp.sel] + off, n * sizeof(int), hipMemcpyDeviceToHost));
+	for(int j = 0; j < n; j++) {
+		int i = hp[j];
+		if(i < 0 || i >= n) return fail(PP_ERR_HIP, "corrupt particle index %d", i);
+		if(r)
+			for(int a = 0; a < 3; a++) r[3 * i + a] = h[(size_t) a * n + j];
+		if(O)
+			for(int a = 0; a < 9; a++) O[9 * i + a] = h[(size_t) (3 + a) * n + j];
+	}
+	if(box)
+		for(int a = 0; a < 3; a++) box[a] = rp.g.box[a];
+	return PP_OK;
+}
+
+pp_status pp_upload(pp_ctx *c, int rep, const double *r, const double *O, const double box[3]) {
+	if(!c) return fail(PP_ERR_ARG, "ctx is NULL");
+	if(rep < 0 || rep >= c->R) return fail(PP_ERR_ARG, "replica %d out of range", rep);
+	HIPCHECK(hipSetDevice(c->device));
+	TRY(pull_reps(c));
+	Rep &rp = c->reps[rep];
+	const int n = rp.np, off = rp.p_off, S = c->d.stride;
+	if(n && (!r || !O)) return fail(PP_ERR_ARG, "positions/orientations are NULL");
+	if(box) {
+		int gn[3];
+		for(int a = 0; a < 3; a++)
+			if(!(box[a] > 0.)) return fail(PP_ERR_ARG, "box side %d is not positive", a);
+		if(!make_grid(box, rp.side_min, rp.min_n, rp.even, rp.cap_n, gn))
+			return fail(PP_ERR_BOX, "box too small for the cell grid");
+		set_grid_box(rp.g, box);
+		for(int a = 0; a < 3; a++) {
+			rp.g.n[a] = gn[a];
+			rp.g.shift[a] = 0.;
+		}
+		rp.V = box[0] * box[1] * box[2];
+	}
+	std::vector<double> h((size_t) NCOMP * std::max(n, 1));
+	std::vector<int> hp(std::max(n, 1));
+	for(int i = 0; i < n; i++) {
+		for(int a = 0; a < 3; a++) h[(size_t) a * n + i] = r[3 * i + a];
+		for(int a = 0; a < 9; a++) h[(size_t) (3 + a) * n + i] = O[9 * i + a];
+		hp[i] = i;
+	}
+	for(int a = 0; a < NCOMP && n; a++)
+		HIPCHECK(hipMemcpy(c->d.f[rp.sel] + (size_t) a * S + off, h.data() + (size_t) a * n, n * sizeof(double),
+		                   hipMemcpyHostToDevice));
+	if(n) HIPCHECK(hipMemcpy(c->d.pid[rp.sel] + off, hp.data(), n * sizeof(int), hipMemcpyHostToDevice));
+	TRY(push_reps(c));
+	// new tracked energy for this replica
+	TRY(energy_now(c, 0));
+	TRY(check_err(c));
+	std::vector<unsigned long long> hc;
+	TRY(pull_ctr(c, hc));
+	TRY(pull_reps(c));
+	if(ctr_sum(hc, rep, C_OVL) > 0) return fail(PP_ERR_OVERLAP, "uploaded configuration contains an overlap");
+	c->reps[rep].bonds_base = ctr_sum(hc, rep, C_EBONDS) / 2 - ctr_sum(hc, rep, C_DBONDS);
+	return push_reps(c);
+}
+
+pp_status pp_get_counters(pp_ctx *c, int rep, pp_counters *out) {
+	if(!c || !out) return fail(PP_ERR_ARG, "NULL argument");
+	if(rep < 0 || rep >= c->R) return fail(PP_ERR_ARG, "replica %d out of range", rep);
+	HIPCHECK(hipSetDevice(c->device));
+	std::vector<unsigned long long> h;
+	TRY(pull_ctr(c, h));
+	TRY(pull_reps(c));
+	const Rep &rp = c->reps[rep];
+	out->tries[0] = ctr_sum(h, rep, C_TRY_ROTO);
+	out->accepted[0] = ctr_sum(h, rep, C_ACC_ROTO);
+	out->tries[1] = ctr_sum(h, rep, C_TRY_VOL);
+	out->accepted[1] = ctr_sum(h, rep, C_ACC_VOL);
+	out->bonds = rp.bonds_base + ctr_sum(h, rep, C_DBONDS);
+	out->sweeps = c->sweep;
+	for(int a = 0; a < 3; a++) out->box[a] = rp.g.box[a];
+	out->volume = rp.V;
+	return PP_OK;
+}
+
+pp_status pp_get_counters_all(pp_ctx *c, pp_counters *out) {
+	if(!c || !out) return fail(PP_ERR_ARG, "NULL argument");
+	HIPCHECK(hipSetDevice(c->device));
+	std::vector<unsigned long long> h;
+	TRY(pull_ctr(c, h));
+	TRY(pull_reps(c));
+	for(int rep = 0; rep < c->R; rep++) {
+		const Rep &rp = c->reps[rep];
+		pp_counters &o = out[rep];
+		o.tries[0] = ctr_sum(h, rep, C_TRY_ROTO);
+		o.accepted[0] = ctr_sum(h, rep, C_ACC_ROTO);
+		o.tries[1] = ctr_sum(h, rep, C_TRY_VOL);
+		o.accepted[1] = ctr_sum(h, rep, C_ACC_VOL);
+		o.bonds = rp.bonds_base + ctr_sum(h, rep, C_DBONDS);
+		o.sweeps = c->sweep;
+		for(int a = 0; a < 3; a++) o.box[a] = rp.g.box[a];
+		o.volume = rp.V;
+	}
+	return PP_OK;
+}
+
+pp_status pp_reset_counters(pp_ctx *c) {
+	if(!c) return fail(PP_ERR_ARG, "ctx is NULL");
+	HIPCHECK(hipSetDevice(c->device));
+	std::vector<unsigned long long> h;
+	TRY(pull_ctr(c, h));
+	for(int k = 0; k < c->R; k++)
+		for(int row : {(int) C_TRY_ROTO, (int) C_ACC_ROTO, (int) C_TRY_VOL, (int) C_ACC_VOL})
+			for(int s = 0; s < NSLOT; s++) h[((size_t) k * NCTR + row) * NSLOT + s] = 0;
+	HIPCHECK(hipMemcpy(c->d.ctr, h.data(), h.size() * sizeof(unsigned long long), hipMemcpyHostToDevice));
+	return PP_OK;
+}
+
+int pp_num_replicas(const pp_ctx *c) { return c ? c->R : 0; }
+
+int pp_num_particles(const pp_ctx *c, int rep) { return (c && rep >= 0 && rep < c->R) ? c->np[rep] : -1; }
+
+pp_status pp_energy_snapshot(int n, const double *r, const double *O, const double box[3], double kf_delta,
+                             double kf_cosmax, int32_t *bonds, uint8_t *overlap, long long *total_bonds,
+                             int *any_overlap) {
+	pp_params p{};
+	p.ensemble = PP_NVT;
+	p.temperature = 1.;
+	p.kf_delta = kf_delta;
+	p.kf_cosmax = kf_cosmax;
+	pp_ctx *c = nullptr;
+	pp_status s = create_impl(1, &p, nullptr, &n, r, O, box, true, &c);
+	if(s != PP_OK) return s;
+	s = pp_energy(c, 0, bonds, overlap, total_bonds, any_overlap);
+	pp_destroy(c);
+	return s;
+}
+
+pp_status pp_set_trace(pp_ctx *c, int capacity) {
+	if(!c) return fail(PP_ERR_ARG, "ctx is NULL");
+	if(capacity < 0) return fail(PP_ERR_ARG, "negative capacity");
+	HIPCHECK(hipSetDevice(c->device));
+	HIPCHECK(hipStreamSynchronize(c->stream));
+	if(c->d.trace) {
+		HIPCHECK(hipFree(c->d.trace));
+		c->allocs.erase(std::find(c->allocs.begin(), c->allocs.end(), (void *) c->d.trace));
+		c->d.trace = nullptr;
+	}
+	c->d.trace_cap = 0;
+	c->trace_len = -1;
+	if(capacity > 0) {
+		TRY(dev_alloc(c, &c->d.trace, (size_t) c->R * capacity));
+		c->d.trace_cap = capacity;
+		c->trace_len = 0;
+		c->trace_from = c->sweep;
+	}
+	return PP_OK;
+}
+
+pp_status pp_get_trace(pp_ctx *c, int rep, long long *bonds, int capacity, int *count) {
+	if(!c) return fail(PP_ERR_ARG, "ctx is NULL");
+	if(rep < 0 || rep >= c->R) return fail(PP_ERR_ARG, "replica %d out of range", rep);
+	HIPCHECK(hipSetDevice(c->device));
+	HIPCHECK(hipStreamSynchronize(c->stream));
+	// entry k holds the energy after sweep k (written at the start of sweep k+1);
+	// the state after the last sweep is appended from the counters
+	int m = (int) std::max<long long>(c->trace_len, 0);
+	int have = m > 0 ? m - 1 : 0;
+	int nout = std::min(have, capacity);
+	if(nout > 0 && bonds)
+		HIPCHECK(hipMemcpy(bonds, c->d.trace + (size_t) rep * c->d.trace_cap + 1, nout * sizeof(long long),
+		                   hipMemcpyDeviceToHost));
+	// the energy after the last traced sweep is still in the counters only if
+	// no sweep has run past the end of the trace
+	if(bonds && nout < capacity && m > 0 && c->sweep - c->trace_from == m) {
+		pp_counters k;
+		TRY(pp_get_counters(c, rep, &k));
+		bonds[nout++] = k.bonds;
+	}
+	if(count) *count = nout;
+	return PP_OK;
+}
+
+pp_status pp_set_timing(pp_ctx *c, int on) {
+	if(!c) return fail(PP_ERR_ARG, "ctx is NULL");
+	c->timing = on != 0;
+	for(int k = 0; k < 4; k++) {
+		c->t_ms[k] = 0.;
+		c->t_launch[k] = 0;
+	}
+	return PP_OK;
+}
+
+pp_status pp_get_timing(pp_ctx *c, const char *name, long long *launches, double *total_ms) {
+	if(!c) return fail(PP_ERR_ARG, "ctx is NULL");
+	TRY(collect_timing(c));
+	int lo = 0, hi = 4;
+	if(name) {
+		static const char *names[4] = {"pass", "rebuild", "energy", "volume"};
+		lo = -1;
+		for(int k = 0; k < 4; k++)
+			if(!strcmp(name, names[k])) lo = k;
+		if(lo < 0) return fail(PP_ERR_ARG, "unknown kernel group '%s'", name);
+		hi = lo + 1;
+	}
+	long long nl = 0;
+	double ms = 0.;
+	for(int k = lo; k < hi; k++) {
+		nl += c->t_launch[k];
+		ms += c->t_ms[k];
+	}
+	if(launches) *launches = nl;
+	if(total_ms) *total_ms = ms;
+	return PP_OK;
+}
+
+pp_status pp_get_grid(pp_ctx *c, int rep, int n_side[3], double side[3]) {
+	if(!c) return fail(PP_ERR_ARG, "ctx is NULL");
+	if(rep < 0 || rep >= c->R) return fail(PP_ERR_ARG, "replica %d out of range", rep);
+	TRY(pull_reps(c));
+	const Rep &rp = c->reps[rep];
+	for(int a = 0; a < 3; a++) {
+		if(n_side) n_side[a] = rp.g.n[a];
+		if(side) side[a] = rp.g.box[a] / rp.g.n[a];
+	}
+	return PP_OK;
+}
+
+void pp_destroy(pp_ctx *c) {
+	if(!c) return;
+	hipSetDevice(c->device);
+	if(c->stream) hipStreamSynchronize(c->stream);
+	delete c;
+}
+
+const char *pp_strerror(pp_status s) {
+	switch(s) {
+	case PP_OK: return "success";
+	case PP_ERR_ARG: return "invalid argument";
+	case PP_ERR_HIP: return "HIP runtime error";
+	case PP_ERR_OVERLAP: return "configuration contains an overlap";
+	case PP_ERR_BOX: return "box too small for the cell grid";
+	case PP_ERR_CELL_CAP: return "too many particles in one cell";
+	case PP_ERR_ENERGY: return "energy check failed";
+	case PP_ERR_NOMEM: return "out of memory";
+	case PP_ERR_UNSUPPORTED: return "unsupported ensemble or dynamics";
+	case PP_ERR_CELLS: return "cell population differs from N";
+	}
+	return "unknown status";
+}
+
+const char *pp_last_error(void) { return g_err.c_str(); }
+
+}  // extern "C"

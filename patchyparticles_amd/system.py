@@ -1,0 +1,162 @@
+"""Python face of the GPU hot path: a ``System`` holds one or more replicas in a
+device context and advances them with ``do_ensemble`` (the reference's
+``syst.do_ensemble`` seam, main.c:119).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from ._lib import Counters, Options, Params, _arr, _ptr, check, lib
+
+NVT, NPT = 0, 5  # defs.h:21-25
+
+
+class System:
+    """Replicas of hard spheres with 4 Kern-Frenkel patches, on one GPU.
+
+    Positions ``r`` (N, 3) and orientation matrices ``O`` (N, 3, 3) by rows, as
+    the reference's PatchyParticle (defs.h:47-56).  For several replicas pass
+    lists of per-replica arrays and parameters (see :meth:`batch`).
+    """
+
+    def __init__(self, r, O, box, *, T, disp_max=0.05, theta_max=0.1, kf_delta=0.12, kf_cosmax=0.92,
+                 ensemble=NVT, P=0.0, rescale_factor_max=0.0, seed=1, threads_per_cell=0, block_size=0,
+                 cell_occupancy=0.0, device=0):
+        self._h = None
+        self._make([r], [O], [box], [dict(T=T, disp_max=disp_max, theta_max=theta_max, kf_delta=kf_delta,
+                                          kf_cosmax=kf_cosmax, ensemble=ensemble, P=P,
+                                          rescale_factor_max=rescale_factor_max)],
+                   seed, threads_per_cell, block_size, cell_occupancy, device)
+
+    @classmethod
+    def batch(cls, rs, Os, boxes, params, *, seed=1, threads_per_cell=0, block_size=0, cell_occupancy=0.0,
+              device=0):
+        """Many independent replicas in one context; ``params`` is a list of
+        dicts with the keyword arguments of the constructor (T, P, ...)."""
+        self = cls.__new__(cls)
+        self._h = None
+        self._make(rs, Os, boxes, params, seed, threads_per_cell, block_size, cell_occupancy, device)
+        return self
+
+    def _make(self, rs, Os, boxes, params, seed, tpc, block, occ, device):
+        R = len(rs)
+        self.n = [int(np.asarray(r).size // 3) for r in rs]
+        P = (Params * R)()
+        for k, p in enumerate(params):
+            P[k].ensemble = int(p.get("ensemble", NVT))
+            P[k].temperature = float(p["T"])
+            P[k].disp_max = float(p.get("disp_max", 0.05))
+            P[k].theta_max = float(p.get("theta_max", 0.1))
+            P[k].kf_delta = float(p.get("kf_delta", 0.12))
+            P[k].kf_cosmax = float(p.get("kf_cosmax", 0.92))
+            P[k].pressure = float(p.get("P", 0.0))
+            P[k].rescale_factor_max = float(p.get("rescale_factor_max", 0.0))
+        o = Options(int(seed) & 0xFFFFFFFFFFFFFFFF, int(tpc), int(block), float(occ), int(device))
+        r_all = np.concatenate([_arr(r) for r in rs]) if R else np.zeros(0)
+        O_all = np.concatenate([_arr(x) for x in Os]) if R else np.zeros(0)
+        b_all = np.concatenate([_arr(b) for b in boxes])
+        h = C.c_void_p()
+        check(lib().pp_create_batch(R, P, C.byref(o), np.asarray(self.n, np.int32), np.ascontiguousarray(r_all),
+                                    np.ascontiguousarray(O_all), np.ascontiguousarray(b_all), C.byref(h)))
+        self._h = h
+        self.R = R
+
+    def close(self):
+        if self._h is not None:
+            lib().pp_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # -- the hot path -------------------------------------------------------
+    def do_ensemble(self, sweeps=1):
+        """Advance every replica by ``sweeps`` sweeps (N attempts each)."""
+        check(lib().pp_sweeps(self._h, int(sweeps)))
+
+    def energy(self, rep=0):
+        """(bonds[N], overlap[N], bonded pairs, any overlap) recomputed from
+        scratch, as MC_energy over every particle (MC.c:272-309)."""
+        n = self.n[rep]
+        bonds = np.zeros(max(n, 1), np.int32)
+        ov = np.zeros(max(n, 1), np.uint8)
+        tot = C.c_longlong(0)
+        anyo = C.c_int(0)
+        check(lib().pp_energy(self._h, rep, _ptr(bonds), _ptr(ov), C.byref(tot), C.byref(anyo)))
+        return bonds[:n], ov[:n].astype(bool), tot.value, bool(anyo.value)
+
+    def check_energy(self):
+        """MC_check_energy (MC.c:766-784) on every replica; returns energies."""
+        e = np.zeros(self.R)
+        check(lib().pp_check_energy(self._h, _ptr(e)))
+        return e
+
+    def state(self, rep=0):
+        n = self.n[rep]
+        r = np.zeros(3 * max(n, 1))
+        O = np.zeros(9 * max(n, 1))
+        box = np.zeros(3)
+        check(lib().pp_download(self._h, rep, _ptr(r), _ptr(O), _ptr(box)))
+        return r[:3 * n].reshape(n, 3), O[:9 * n].reshape(n, 3, 3), box
+
+    def upload(self, rep, r, O, box=None):
+        r = _arr(r)
+        O = _arr(O)
+        b = None if box is None else _arr(box)
+        check(lib().pp_upload(self._h, rep, _ptr(r), _ptr(O), _ptr(b)))
+
+    def counters(self, rep=0):
+        c = Counters()
+        check(lib().pp_get_counters(self._h, rep, C.byref(c)))
+        return dict(tries=list(c.tries), accepted=list(c.accepted), bonds=c.bonds, sweeps=c.sweeps,
+                    box=list(c.box), volume=c.volume)
+
+    def counters_all(self):
+        """Counters of every replica: dict of numpy arrays (one entry per replica)."""
+        cs = (Counters * self.R)()
+        check(lib().pp_get_counters_all(self._h, cs))
+        return dict(tries=np.array([list(c.tries) for c in cs]), accepted=np.array([list(c.accepted) for c in cs]),
+                    bonds=np.array([c.bonds for c in cs]), box=np.array([list(c.box) for c in cs]),
+                    volume=np.array([c.volume for c in cs]))
+
+    def reset_counters(self):
+        check(lib().pp_reset_counters(self._h))
+
+    def grid(self, rep=0):
+        n = (C.c_int * 3)()
+        s = (C.c_double * 3)()
+        check(lib().pp_get_grid(self._h, rep, n, s))
+        return list(n), list(s)
+
+    def set_trace(self, capacity):
+        """Record the bonded-pair count after each of the next `capacity` sweeps."""
+        self._trace_cap = int(capacity)
+        check(lib().pp_set_trace(self._h, int(capacity)))
+
+    def trace(self, rep=0):
+        """Energy E = -bonds after each traced sweep (numpy int64, oldest first)."""
+        cap = getattr(self, "_trace_cap", 0)
+        out = np.zeros(max(cap, 1), np.int64)
+        cnt = C.c_int(0)
+        check(lib().pp_get_trace(self._h, rep, _ptr(out), cap, C.byref(cnt)))
+        return -out[:cnt.value]
+
+    def set_timing(self, on=True):
+        check(lib().pp_set_timing(self._h, 1 if on else 0))
+
+    def timing(self, name=None):
+        nl = C.c_longlong(0)
+        ms = C.c_double(0)
+        check(lib().pp_get_timing(self._h, None if name is None else name.encode(), C.byref(nl), C.byref(ms)))
+        return nl.value, ms.value

@@ -1,0 +1,237 @@
+"""Write the golden vectors of tests/golden/ from the reference itself.
+
+Runs only where the reference sources exist (/root/reference): builds
+oracle/_ref (oracle/Makefile, straight from the reference sources) and calls
+the reference's own functions through oracle/_ref/libppref.so and its
+executables.  Outputs are data only (inputs and expected outputs):
+
+  pair_kat.npz      MC_would_interact on hand-built pairs at the strict
+                    thresholds and across periodic faces/corners
+  snap_*.npz        per-particle MC_energy (bonds, overlap) on snapshots
+  traj_*.npz        positions/orientations/box after k steps of the
+                    reference main loop (NVT and NPT), from a given start
+  generator.json    sha256 of the reference generator's generated.rrr
+
+    python tests/golden/make_golden.py
+"""
+from __future__ import annotations
+
+import hashlib
+import json
+import os
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+import pyoracle as po  # noqa: E402
+
+REF_SRC = "/root/reference"
+
+
+def build_ref():
+    subprocess.run(["make", "-C", os.path.join(ROOT, "oracle"), "all", "ref", f"REF={REF_SRC}"], check=True,
+                   stdout=subprocess.DEVNULL)
+
+
+def rand_rot(rng):
+    q = rng.normal(size=4)
+    q /= np.linalg.norm(q)
+    a, b, c, d = q
+    return np.array([[a * a + b * b - c * c - d * d, 2 * (b * c - a * d), 2 * (b * d + a * c)],
+                     [2 * (b * c + a * d), a * a - b * b + c * c - d * d, 2 * (c * d - a * b)],
+                     [2 * (b * d - a * c), 2 * (c * d + a * b), a * a - b * b - c * c + d * d]])
+
+
+def align_patch(rng, target, patch_index, base):
+    """A rotation whose patch `patch_index` points along unit vector target."""
+    b = base[patch_index]
+    v = np.cross(b, target)
+    s = np.linalg.norm(v)
+    c = float(np.dot(b, target))
+    if s < 1e-12:
+        R = np.eye(3) if c > 0 else -np.eye(3)
+    else:
+        k = v / s
+        K = np.array([[0, -k[2], k[1]], [k[2], 0, -k[0]], [-k[1], k[0], 0]])
+        R = np.eye(3) + s * K + (1 - c) * K @ K
+    # a random spin about the target keeps the patch where it is
+    t = rng.uniform(0, 2 * np.pi)
+    K = np.array([[0, -target[2], target[1]], [target[2], 0, -target[0]], [-target[1], target[0], 0]])
+    S = np.eye(3) + np.sin(t) * K + (1 - np.cos(t)) * K @ K
+    return S @ R
+
+
+def pair_cases(rng, base, delta=0.12, cosmax=0.92):
+    rc = 1 + delta
+    box = np.array([7.3, 8.1, 9.7])
+    cases = []
+
+    def add(rp, Op, rq, Oq, bx=box):
+        cases.append((np.array(rp, float), np.array(Op, float), np.array(rq, float), np.array(Oq, float),
+                      np.array(bx, float)))
+
+    # radial thresholds along random directions, with aligned patches
+    for target_d2 in (1.0, rc * rc):
+        for _ in range(40):
+            u = rng.normal(size=3)
+            u /= np.linalg.norm(u)
+            dist = np.sqrt(target_d2)
+            dist = dist * (1 + rng.choice([-1, 0, 1]) * rng.choice([1e-16, 2e-16, 1e-15, 1e-12]))
+            rp = rng.uniform(0, box)
+            rq = rp + u * dist
+            Op = align_patch(rng, u, rng.integers(4), base)
+            Oq = align_patch(rng, -u, rng.integers(4), base)
+            add(rp, Op, rq, Oq)
+    # angular threshold: patch at an angle whose cosine is ~cosmax
+    for _ in range(60):
+        u = rng.normal(size=3)
+        u /= np.linalg.norm(u)
+        dist = rng.uniform(1.0, rc)
+        rp = rng.uniform(0, box)
+        rq = rp + u * dist
+        th = np.arccos(cosmax) * (1 + rng.choice([-1, 1]) * rng.choice([1e-15, 1e-13, 1e-10, 1e-6]))
+        w = np.cross(u, rng.normal(size=3))
+        w /= np.linalg.norm(w)
+        tp = np.cos(th) * u + np.sin(th) * w
+        Op = align_patch(rng, tp, rng.integers(4), base)
+        Oq = align_patch(rng, -u, rng.integers(4), base) if rng.random() < 0.5 else rand_rot(rng)
+        add(rp, Op, rq, Oq)
+    # minimum image across faces, edges and corners, coordinates far outside the box
+    for _ in range(120):
+        u = rng.normal(size=3)
+        u /= np.linalg.norm(u)
+        dist = rng.uniform(0.9, 1.3)
+        corner = rng.integers(0, 2, size=3) * box
+        rp = corner + rng.uniform(-0.3, 0.3, size=3) + rng.integers(-3, 4, size=3) * box
+        rq = rp + u * dist + rng.integers(-3, 4, size=3) * box
+        Op = align_patch(rng, u, rng.integers(4), base) if rng.random() < 0.7 else rand_rot(rng)
+        Oq = align_patch(rng, -u, rng.integers(4), base) if rng.random() < 0.7 else rand_rot(rng)
+        add(rp, Op, rq, Oq)
+    # random pairs
+    for _ in range(80):
+        rp = rng.uniform(0, box)
+        rq = rp + rng.uniform(-1.4, 1.4, size=3)
+        add(rp, rand_rot(rng), rq, rand_rot(rng))
+    return cases
+
+
+def write_conf(path, r, O, box):
+    po.rrr_save(path, 0, box, r, O)
+
+
+def main():
+    build_ref()
+    tmp = tempfile.mkdtemp(prefix="golden_")
+    rng = np.random.default_rng(20261202)
+    base = po.base_patches()
+
+    # --- reference run-time state: load a small configuration -------------
+    r0, O0, box0 = po.generate(64, 0.1, seed=3)
+    conf0 = os.path.join(tmp, "init0.rrr")
+    write_conf(conf0, r0, O0, box0)
+    ref = po.Ref(conf0, T=0.1, seed=1)
+
+    # --- pair KATs ----------------------------------------------------------
+    cases = pair_cases(rng, base)
+    res = np.array([ref.pair(*c) for c in cases], np.int32)
+    np.savez_compressed(os.path.join(HERE, "pair_kat.npz"),
+                        rp=np.array([c[0] for c in cases]), Op=np.array([c[1] for c in cases]),
+                        rq=np.array([c[2] for c in cases]), Oq=np.array([c[3] for c in cases]),
+                        box=np.array([c[4] for c in cases]), expected=res, delta=0.12, cosmax=0.92)
+    print("pair_kat:", np.bincount(res + 1))
+
+    # --- snapshot energies ----------------------------------------------------
+    def snapshot(name, conf, T=0.1, seed=1, sweeps=0, print_every=0, store_inputs=True, note=""):
+        ref = po.Ref(conf, T=T, seed=seed)
+        if sweeps:
+            ref.run(sweeps, print_every)
+        r, O, box = ref.state()
+        ref.set_state(r, O, box)
+        b, ov = ref.energies()
+        out = dict(bonds=b, overlap=ov, box=box, delta=0.12, cosmax=0.92, note=note)
+        out["r"], out["O"] = r, O
+        np.savez_compressed(os.path.join(HERE, f"snap_{name}.npz"), **out)
+        print(f"snap_{name}: N={len(b)} bonds={b.sum() // 2} overlaps={int(ov.sum())}")
+        return r, O, box
+
+    # C1 start: N=1000 rho=0.1 random sequential insertion, seed 1
+    r1, O1, b1 = po.generate(1000, 0.1, seed=1)
+    c1 = os.path.join(tmp, "c1.rrr")
+    write_conf(c1, r1, O1, b1)
+    snapshot("c1_start", c1, note="N=1000 rho=0.1 RSA seed 1 (as loaded from .rrr)")
+    # bonded liquid: N=500 rho=0.5, 300 reference sweeps at T=0.1
+    r2, O2, b2 = po.generate(500, 0.5, seed=2)
+    c2 = os.path.join(tmp, "c2.rrr")
+    write_conf(c2, r2, O2, b2)
+    snapshot("rho05_T01", c2, T=0.1, seed=2, sweeps=300, print_every=100,
+             note="N=500 rho=0.5 after 300 reference sweeps at T=0.1")
+    # low temperature, strongly bonded
+    r3, O3, b3 = po.generate(400, 0.3, seed=4)
+    c3 = os.path.join(tmp, "c3.rrr")
+    write_conf(c3, r3, O3, b3)
+    snapshot("rho03_T005", c3, T=0.05, seed=4, sweeps=400, print_every=100,
+             note="N=400 rho=0.3 after 400 reference sweeps at T=0.05")
+    # the reference's own crystal seeds
+    for name in ("fcc", "bcc"):
+        snapshot(name, os.path.join(REF_SRC, f"{name}.rrr"), note=f"{name}.rrr of the reference, as loaded")
+    # unwrapped coordinates far outside the box (positions are never wrapped)
+    ref = po.Ref(c2, T=0.1, seed=2)
+    ref.run(200, 100)
+    r, O, box = ref.state()
+    shift = rng.integers(-5, 6, size=r.shape) * box
+    ref.set_state(r + shift, O, box)
+    b, ov = ref.energies()
+    np.savez_compressed(os.path.join(HERE, "snap_unwrapped.npz"), r=r + shift, O=O, box=box, bonds=b, overlap=ov,
+                        delta=0.12, cosmax=0.92, note="N=500 rho=0.5, positions moved by random box multiples")
+    print("snap_unwrapped:", b.sum() // 2)
+    # a snapshot with overlaps (MC_energy returns 0 and flags them)
+    r_ov = r.copy()
+    r_ov[7] = r_ov[8] + np.array([0.5, 0.1, 0.0])
+    r_ov[100] = r_ov[101] + np.array([0.0, 0.99, 0.0])
+    ref.set_state(r_ov, O, box)
+    b, ov = ref.energies()
+    np.savez_compressed(os.path.join(HERE, "snap_overlap.npz"), r=r_ov, O=O, box=box, bonds=b, overlap=ov,
+                        delta=0.12, cosmax=0.92, note="two planted overlaps")
+    print("snap_overlap: overlaps", int(ov.sum()))
+
+    # --- trajectories ----------------------------------------------------------
+    def traj(name, conf, steps, print_every, **kw):
+        ref = po.Ref(conf, **kw)
+        step, box, r_in, O_in = po.rrr_load(conf)
+        ref.run(steps, print_every)
+        r, O, box_out = ref.state()
+        tries, acc = ref.counters()
+        np.savez_compressed(os.path.join(HERE, f"traj_{name}.npz"), r_in=r_in, O_in=O_in, box_in=box,
+                            r=r, O=O, box=box_out, energy=ref.energy, steps=steps, print_every=print_every,
+                            params=json.dumps(kw))
+        print(f"traj_{name}: E={ref.energy} tries={tries} acc={acc} box={box_out}")
+
+    r4, O4, b4 = po.generate(150, 0.3, seed=6)
+    c4 = os.path.join(tmp, "c4.rrr")
+    write_conf(c4, r4, O4, b4)
+    traj("nvt", c4, 120, 40, T=0.1, seed=9)
+    r5, O5, b5 = po.generate(120, 0.15, seed=8)
+    c5 = os.path.join(tmp, "c5.rrr")
+    write_conf(c5, r5, O5, b5)
+    traj("npt", c5, 150, 50, T=0.12, seed=13, ensemble=5, pressure=0.3, rescale_factor_max=0.1)
+
+    # --- the generator --------------------------------------------------------
+    gens = []
+    for n, rho in ((300, 0.2), (1000, 0.1)):
+        d = tempfile.mkdtemp(prefix="gen_")
+        subprocess.run([os.path.join(ROOT, "oracle", "_ref", "generator"), str(n), str(rho)], cwd=d, check=True,
+                       capture_output=True)
+        with open(os.path.join(d, "generated.rrr"), "rb") as f:
+            gens.append(dict(n=n, rho=rho, sha256=hashlib.sha256(f.read()).hexdigest()))
+    with open(os.path.join(HERE, "generator.json"), "w") as f:
+        json.dump(gens, f, indent=1)
+    print("generator:", gens)
+
+
+if __name__ == "__main__":
+    main()

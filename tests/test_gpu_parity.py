@@ -1,0 +1,174 @@
+"""GPU parity: the HIP kernels against the CPU oracle and the reference's
+golden vectors.  Integer results (bonds, overlap flags) must be bit-exact."""
+import math
+
+import numpy as np
+import pytest
+
+from conftest import golden, golden_files
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def pp():
+    import patchyparticles_amd as pp
+    pp.lib()
+    return pp
+
+
+@pytest.mark.parametrize("name", golden_files("snap_"))
+def test_energy_snapshot_matches_reference(pp, name):
+    g = golden(name)
+    b, ov, tot, anyo = pp.energy_snapshot(g["r"], g["O"], g["box"], float(g["delta"]), float(g["cosmax"]))
+    assert np.array_equal(b, g["bonds"]), name
+    assert np.array_equal(ov, g["overlap"].astype(bool)), name
+    assert anyo == bool(g["overlap"].any())
+
+
+def test_pair_known_answers_on_gpu(pp):
+    """Each reference pair KAT as a two-particle snapshot."""
+    g = golden("pair_kat.npz")
+    for i in range(len(g["expected"])):
+        r = np.stack([g["rp"][i], g["rq"][i]])
+        O = np.stack([g["Op"][i], g["Oq"][i]])
+        b, ov, _, _ = pp.energy_snapshot(r, O, g["box"][i])
+        want = int(g["expected"][i])
+        if want == -1:
+            assert ov.all(), i
+        else:
+            assert not ov.any() and list(b) == [want, want], i
+
+
+def _start(oracle, n, rho, seed):
+    r, O, box = oracle.generate(n, rho, seed=seed)
+    return r, O, box
+
+
+def test_sweeps_track_energy_and_match_oracle(pp, oracle):
+    """N=4000, rho=0.5, T=0.12: the incrementally tracked energy equals a full
+    recompute, and the GPU per-particle energies of the evolved snapshot equal
+    the oracle's (the reference energy routine) bit for bit."""
+    r, O, box = _start(oracle, 4000, 0.5, 3)
+    with pp.System(r, O, box, T=0.12, seed=42) as s:
+        for _ in range(3):
+            s.do_ensemble(40)
+            c = s.counters()
+            b, ov, tot, anyo = s.energy()
+            assert not anyo
+            assert c["bonds"] == tot
+            rr, OO, bb = s.state()
+            ob, oov, otot = oracle.energy(rr, OO, bb)
+            assert np.array_equal(b, ob)
+            assert not oov.any()
+        c = s.counters()
+        assert c["tries"][0] == 120 * 4000
+        assert 0.3 < c["accepted"][0] / c["tries"][0] < 0.99
+        assert tot > 0
+
+
+def test_config2_snapshot_parity(pp, oracle):
+    """N=32768, rho=0.5, T=0.12: snapshot bonds/overlaps bit-exact against the
+    oracle after 1000 sweeps."""
+    r, O, box = _start(oracle, 32768, 0.5, 1)
+    with pp.System(r, O, box, T=0.12, seed=7) as s:
+        s.do_ensemble(1000)
+        rr, OO, bb = s.state()
+        b, ov, tot, anyo = s.energy()
+        ob, oov, otot = oracle.energy(rr, OO, bb)
+        assert np.array_equal(b, ob) and not anyo and not oov.any()
+        assert s.counters()["bonds"] == tot == otot
+        s.check_energy()
+
+
+def test_check_energy_gram_schmidt_bit_exact(pp, oracle):
+    r, O, box = _start(oracle, 2000, 0.3, 5)
+    with pp.System(r, O, box, T=0.1, seed=3) as s:
+        s.do_ensemble(30)
+        r1, O1, _ = s.state()
+        s.check_energy()
+        r2, O2, _ = s.state()
+    assert np.array_equal(r1, r2)
+    want = np.array([oracle.gram_schmidt(o) for o in O1])
+    assert np.array_equal(O2, want)
+
+
+@pytest.mark.parametrize("tpc,block", [(8, 64), (16, 128), (32, 256), (64, 256), (64, 128)])
+def test_trajectory_independent_of_launch_shape(pp, oracle, tpc, block):
+    r, O, box = _start(oracle, 3000, 0.4, 9)
+    ref = None
+    with pp.System(r, O, box, T=0.11, seed=99, threads_per_cell=64, block_size=256) as s:
+        s.do_ensemble(25)
+        ref = s.state()
+    with pp.System(r, O, box, T=0.11, seed=99, threads_per_cell=tpc, block_size=block) as s:
+        s.do_ensemble(25)
+        got = s.state()
+    for a, b in zip(ref, got):
+        assert np.array_equal(a, b)
+
+
+def test_upload_download_roundtrip(pp, oracle):
+    r, O, box = _start(oracle, 1500, 0.3, 2)
+    with pp.System(r, O, box, T=0.1, seed=1) as s:
+        r1, O1, b1 = s.state()
+        assert np.array_equal(r1, r) and np.array_equal(O1, O) and np.array_equal(b1, box)
+        s.do_ensemble(5)
+        r2, O2, b2 = s.state()
+        s.upload(0, r, O, box)
+        r3, O3, b3 = s.state()
+        assert np.array_equal(r3, r) and np.array_equal(O3, O)
+        _, _, tot, _ = s.energy()
+        assert s.counters()["bonds"] == tot
+
+
+def test_overlapping_start_is_refused(pp, oracle):
+    r, O, box = _start(oracle, 200, 0.3, 2)
+    r[5] = r[6] + np.array([0.3, 0.0, 0.0])
+    with pytest.raises(pp.PPError) as e:
+        pp.System(r, O, box, T=0.1)
+    assert e.value.status == 3
+
+
+def test_batched_replicas_match_single(pp, oracle):
+    """Replica 0 of a batch follows the same trajectory as the same system
+    alone (Philox streams are keyed by replica index)."""
+    ra, Oa, ba = _start(oracle, 1000, 0.2, 1)
+    rb, Ob, bb = _start(oracle, 700, 0.5, 2)
+    with pp.System(ra, Oa, ba, T=0.1, seed=5) as s:
+        s.do_ensemble(20)
+        single = s.state()
+    prm = [dict(T=0.1), dict(T=0.15)]
+    with pp.System.batch([ra, rb], [Oa, Ob], [ba, bb], prm, seed=5) as s:
+        s.do_ensemble(20)
+        both = s.state(0)
+        for k in range(2):
+            b, ov, tot, anyo = s.energy(k)
+            assert s.counters(k)["bonds"] == tot and not anyo
+    for a, b in zip(single, both):
+        assert np.array_equal(a, b)
+
+
+def test_two_particle_bonding_probability_gpu(pp):
+    """256 replicas of two particles in a periodic box L = 5 (4^3 sweep cells):
+    P(bonded) = Vb e^{1/T} / (Vb e^{1/T} + V - 4pi/3 - Vb)."""
+    delta, cosmax, T, L = 0.12, 0.92, 0.125, 5.0
+    Vb = 16 * math.pi * (delta ** 3 + 3 * delta ** 2 + 3 * delta) * (1 - cosmax) ** 2 / 3
+    w = Vb * math.exp(1 / T)
+    p_exact = w / (w + L ** 3 - 4 * math.pi / 3 - Vb)
+    R = 256
+    rng = np.random.default_rng(0)
+    rs, Os, bs = [], [], []
+    for _ in range(R):
+        rs.append(np.array([[0.5, 0.5, 0.5], [3.0, 3.0, 3.0]]) + rng.uniform(0, 0.3, (2, 3)))
+        Os.append(np.array([np.eye(3), np.eye(3)]))
+        bs.append(np.array([L, L, L]))
+    prm = [dict(T=T, disp_max=0.8, theta_max=1.5)] * R
+    with pp.System.batch(rs, Os, bs, prm, seed=17) as s:
+        s.do_ensemble(5000)
+        samples = []
+        for _ in range(100):
+            s.do_ensemble(200)
+            samples.append(s.counters_all()["bonds"])
+    per_rep = np.array(samples).mean(axis=0)  # independent replicas
+    m, se = per_rep.mean(), per_rep.std(ddof=1) / np.sqrt(R)
+    assert abs(m - p_exact) < 4 * se + 2e-3, (m, se, p_exact)
