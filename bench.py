@@ -87,6 +87,7 @@ def main():
     ap.add_argument("--occupancy", type=float, default=0.0)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-timing", action="store_true", help="no HIP events in the timed region")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: this build has no multi-GPU path (asked for {args.gpus}); "
@@ -106,7 +107,7 @@ def main():
 
     # timed region: K sweeps, HIP events around every colour-pass launch
     sysm.reset_counters()
-    sysm.set_timing(True)
+    sysm.set_timing(not args.no_timing)
     sysm.set_trace(args.steps)
     t0 = time.perf_counter()
     sysm.do_ensemble(args.steps)  # synchronous: returns when the device is done
@@ -156,7 +157,7 @@ def main():
             "workload": f"plain NVT checkerboard rototranslation, N={N}, rho={RHO}, T={T}, KF_delta={DELTA}, "
                         f"KF_cosmax={COSMAX}, Disp_max={DISP}, Theta_max={THETA}, FP64",
             "grid": nside, "cell_side": round(side[0], 5),
-            "threads_per_cell": args.tpc or 64, "attempts": attempts,
+            "threads_per_cell": args.tpc or 32, "attempts": attempts,
             "acceptance": round(c["accepted"][0] / max(attempts, 1), 5),
             "E_per_N": round(float(-c["bonds"] / N), 6),
         },

@@ -51,7 +51,7 @@ typedef enum pp_status {
 	PP_ERR_HIP = 2,         /* HIP runtime error (no device, launch failure...) */
 	PP_ERR_OVERLAP = 3,     /* configuration contains an overlap (main.c:59, MC.c:779) */
 	PP_ERR_BOX = 4,         /* box too small for the cell grid (< 4 (1+delta) per side) */
-	PP_ERR_CELL_CAP = 5,    /* a cell holds more particles than the sweep kernel stages */
+	PP_ERR_CELL_CAP = 5,    /* a sweep cell holds more particles than a team stages (24) */
 	PP_ERR_ENERGY = 6,      /* tracked energy disagrees with the recomputed one (MC.c:780-782) */
 	PP_ERR_NOMEM = 7,       /* host or device allocation failed */
 	PP_ERR_UNSUPPORTED = 8, /* ensemble other than 0 (NVT) / 5 (NPT), or Dynamics != 0 */
@@ -77,11 +77,18 @@ typedef struct pp_params {
 /* Options of the GPU engine (no reference counterpart). */
 typedef struct pp_options {
 	uint64_t seed;           /* Philox key; the whole trajectory is a function of it */
-	int threads_per_cell;    /* lanes that scan one cell's neighbourhood: 8/16/32/64 (0: 64) */
+	int threads_per_cell;    /* lanes that work on one sweep cell: 32 or 64 (0: 32) */
 	int block_size;          /* threads per workgroup: 64/128/256 (0: 256) */
 	double cell_occupancy;   /* target particles per sweep cell; 0: smallest cells (side >= 1+delta) */
 	int device;              /* HIP device ordinal */
+	int launch;              /* PP_LAUNCH_*: how the 8 colour passes of a sweep are launched */
 } pp_options;
+
+/* pp_options.launch.  On MI355X a grid barrier across the 8 XCDs costs more
+ * than a kernel boundary, so the default is one launch per colour pass. */
+#define PP_LAUNCH_DEFAULT 0   /* PP_LAUNCH_PER_PASS */
+#define PP_LAUNCH_PER_PASS 1  /* one launch per colour pass */
+#define PP_LAUNCH_COOP 2      /* one cooperative launch per sweep (grid barriers between passes); fail if unsupported */
 
 typedef struct pp_counters {
 	long long tries[2];    /* [0] rototranslations, [1] volume moves, since the last reset */

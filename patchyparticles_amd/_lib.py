@@ -54,6 +54,7 @@ class Options(C.Structure):
         ("block_size", C.c_int),
         ("cell_occupancy", C.c_double),
         ("device", C.c_int),
+        ("launch", C.c_int),
     ]
 
 

@@ -18,18 +18,28 @@ constexpr int KF_OVERLAP = -1;
 constexpr int KF_NO_BOND = 0;
 constexpr int KF_BOND = 1;
 
-// Tetrahedral base patches, uploaded bit-for-bit from the host (they are the
-// host's normalize() of (+-h,+-h,+-h), h = 0.5/sqrt(3), system.c:20-31).
+// Tetrahedral base patches (system.c:20-31): normalize() of (+-h,+-h,+-h)
+// with h = 0.5/sqrt(3).  All four share the magnitude c = h / |(h,h,h)|,
+// the double 0x1.279a74590331dp-1 (h = 0.2886751345948129, |.| = 0.5 exactly);
+// the host recomputes it with the same IEEE operations at context creation
+// and refuses to run if the bits differ.  Signs: patch i, component k is
+// negative iff bit 3i+k of KF_NEG is set: (-,-,+) (+,-,-) (+,+,+) (-,+,-).
+constexpr double KF_C = 0x1.279a74590331dp-1;
+constexpr unsigned KF_NEG = 0b101000110011u;  // bits 0..11 = (i=0:k0,k1,k2), (i=1:...), ...
+
 struct Base {
 	double b[4][3];
 };
 
+__host__ __device__ constexpr double kf_base(int i, int k) { return ((KF_NEG >> (3 * i + k)) & 1u) ? -KF_C : KF_C; }
+
 // patch_i = O * b_i over the rows of O, left-to-right sums (utils.h:17-21).
-__device__ __forceinline__ void kf_patches(const double O[9], const Base &B, double P[4][3]) {
+__device__ __forceinline__ void kf_patches(const double O[9], double P[4][3]) {
 #pragma unroll
 	for(int i = 0; i < 4; i++)
 #pragma unroll
-		for(int k = 0; k < 3; k++) P[i][k] = O[3 * k + 0] * B.b[i][0] + O[3 * k + 1] * B.b[i][1] + O[3 * k + 2] * B.b[i][2];
+		for(int k = 0; k < 3; k++)
+			P[i][k] = O[3 * k + 0] * kf_base(i, 0) + O[3 * k + 1] * kf_base(i, 1) + O[3 * k + 2] * kf_base(i, 2);
 }
 
 // Minimum-image separation d = r_q - r_p, component by component.  The image
@@ -93,18 +103,110 @@ __device__ __forceinline__ bool kf_any_p(double ux, double uy, double uz, const 
 	return any;
 }
 
-// any patch q_b = Oq * b_b of q with -(d^.q_b) > cosmax
-__device__ __forceinline__ bool kf_any_q(double ux, double uy, double uz, const double Oq[9], const Base &B,
-                                         double cosmax) {
-	bool any = false;
+// any patch q_b = Oq * b_b of q with -(d^.q_b) > cosmax.  Every base
+// component is +-c, and (+-x)*c = +-(x*c) exactly, so the nine products
+// Oq[k][j]*c are formed once and each patch component is their signed sum in
+// the reference's order (utils.h:17-21).  The patches are built one at a
+// time (a scheduling barrier keeps them from being hoisted together), each
+// tested against up to two directions: the old (uo, if want_o) and the new
+// (un, if want_n) separation of a trial.
+__device__ __forceinline__ void kf_any_q2(double uox, double uoy, double uoz, bool want_o, double unx, double uny,
+                                          double unz, bool want_n, const double Oq[9], double cosmax, bool &hit_o,
+                                          bool &hit_n) {
+	double m[9];
+#pragma unroll
+	for(int e = 0; e < 9; e++) m[e] = Oq[e] * KF_C;
+	hit_o = false;
+	hit_n = false;
 #pragma unroll
 	for(int b = 0; b < 4; b++) {
-		double q0 = Oq[0] * B.b[b][0] + Oq[1] * B.b[b][1] + Oq[2] * B.b[b][2];
-		double q1 = Oq[3] * B.b[b][0] + Oq[4] * B.b[b][1] + Oq[5] * B.b[b][2];
-		double q2 = Oq[6] * B.b[b][0] + Oq[7] * B.b[b][1] + Oq[8] * B.b[b][2];
-		any |= -kf_dot(ux, uy, uz, q0, q1, q2) > cosmax;
+		const bool n0 = (KF_NEG >> (3 * b)) & 1u, n1 = (KF_NEG >> (3 * b + 1)) & 1u, n2 = (KF_NEG >> (3 * b + 2)) & 1u;
+		double q[3];
+#pragma unroll
+		for(int k = 0; k < 3; k++) {
+			double a0 = n0 ? -m[3 * k] : m[3 * k];
+			double a1 = n1 ? -m[3 * k + 1] : m[3 * k + 1];
+			double a2 = n2 ? -m[3 * k + 2] : m[3 * k + 2];
+			q[k] = a0 + a1 + a2;
+		}
+		hit_o |= want_o && (-kf_dot(uox, uoy, uoz, q[0], q[1], q[2]) > cosmax);
+		hit_n |= want_n && (-kf_dot(unx, uny, unz, q[0], q[1], q[2]) > cosmax);
+		__builtin_amdgcn_sched_barrier(0);
 	}
-	return any;
+}
+
+__device__ __forceinline__ bool kf_any_q(double ux, double uy, double uz, const double Oq[9], double cosmax) {
+	bool h, unused;
+	kf_any_q2(ux, uy, uz, true, 0., 0., 0., false, Oq, cosmax, h, unused);
+	return h;
+}
+
+// ---------------------------------------------------------------------
+// Filtered angular tests: the decisions of kf_any_p / kf_any_q without the
+// square root and the divisions of d^ for all but a vanishing share of pairs.
+//
+// For a patch vector v (|v| = 1 to rounding) and the minimum-image d with
+// r2 = d.d as above, the reference decides (d / sqrt(r2)).v > cosmax; its
+// roundings keep that dot within 2e-15 of the exact cosine (d.v)/|d| for
+// |d| ~ 1.  Here s = d.v (rounded: error below 6e-16 |d|) and
+// S = s*s - fl(cosmax^2) r2, whose error against the exact
+// (d.v)^2 - cosmax^2 |d|^2 stays below 2e-15 r2.  Hence s > 0 with
+// S > eps r2 puts the exact cosine more than eps/2 above cosmax, and s <= 0 or
+// S < -eps r2 puts it as far below; either way the reference's rounded dot is
+// on the same side of cosmax.  Inside the band |S| <= eps r2 (one pair in
+// ~10^11) the test falls back to the reference's own arithmetic.  eps = 1e-12.
+// The filter needs cosmax >= KF_FILTER_MIN_COS; for a smaller (or negative)
+// cosmax every test takes the exact form.
+constexpr double KF_EPS = 1e-12;
+constexpr double KF_FILTER_MIN_COS = 1e-3;
+enum : int { KF_NO = 0, KF_YES = 1, KF_UNSURE = 2 };
+
+// Branch-free classification of s = d.v (or -(d.v) for the q side) against
+// cosmax on a separation with r2 = d.d: bit 0 set if certainly above, bit 1
+// set if inside the band (undecided); c2r2 = fl(cosmax^2) r2, er2 = eps r2.
+__device__ __forceinline__ unsigned kf_fast_cos(double s, double c2r2, double er2) {
+	const double S = s * s - c2r2;
+	const bool pos = s > 0.;
+	const bool yes = pos && (S > er2);
+	const bool band = pos && !(S > er2) && !(S < -er2);
+	return (unsigned) yes | ((unsigned) band << 1);
+}
+
+// p side on the raw separation: OR over the 4 patches of P (12 doubles,
+// patch-major) of kf_fast_cos
+__device__ __forceinline__ unsigned kf_fast_p(double dx, double dy, double dz, const double *P, double c2r2,
+                                              double er2) {
+	unsigned f = 0;
+#pragma unroll
+	for(int a = 0; a < 4; a++) f |= kf_fast_cos(kf_dot(dx, dy, dz, P[3 * a], P[3 * a + 1], P[3 * a + 2]), c2r2, er2);
+	return f;
+}
+
+// q side on the raw old (o) and new (n) separations against the patches of
+// Oq, built as in kf_any_q2: OR over the patches of kf_fast_cos(-(d.q_b))
+__device__ __forceinline__ void kf_fast_q2(double ox, double oy, double oz, double oc2r2, double oer2, double nx,
+                                           double ny, double nz, double nc2r2, double ner2, const double Oq[9],
+                                           unsigned &fo, unsigned &fn) {
+	double m[9];
+#pragma unroll
+	for(int e = 0; e < 9; e++) m[e] = Oq[e] * KF_C;
+	fo = 0;
+	fn = 0;
+#pragma unroll
+	for(int b = 0; b < 4; b++) {
+		const bool n0 = (KF_NEG >> (3 * b)) & 1u, n1 = (KF_NEG >> (3 * b + 1)) & 1u, n2 = (KF_NEG >> (3 * b + 2)) & 1u;
+		double q[3];
+#pragma unroll
+		for(int k = 0; k < 3; k++) {
+			double a0 = n0 ? -m[3 * k] : m[3 * k];
+			double a1 = n1 ? -m[3 * k + 1] : m[3 * k + 1];
+			double a2 = n2 ? -m[3 * k + 2] : m[3 * k + 2];
+			q[k] = a0 + a1 + a2;
+		}
+		fo |= kf_fast_cos(-kf_dot(ox, oy, oz, q[0], q[1], q[2]), oc2r2, oer2);
+		fn |= kf_fast_cos(-kf_dot(nx, ny, nz, q[0], q[1], q[2]), nc2r2, ner2);
+		__builtin_amdgcn_sched_barrier(0);
+	}
 }
 
 // Cell coordinate of x on a grid of n cells over [s, s+L) (periodic).
@@ -186,12 +288,9 @@ __device__ __forceinline__ int colour_of_pass(uint64_t seed, int rep, uint64_t s
 }
 
 // Rodrigues rotation about a unit axis by t (get_rotation_matrix,
-// utils.c:186-202), then O' = O R (matrix_matrix_multiplication,
-// utils.c:172-184; utils_rotate_matrix utils.c:204-209).
-__device__ __forceinline__ void rotate_body(const double O[9], const double a[3], double st, double ct,
-                                            double On[9]) {
+// utils.c:186-202).
+__device__ __forceinline__ void rotation_matrix(const double a[3], double st, double ct, double R[9]) {
 	double uct = 1. - ct;
-	double R[9];
 	R[0] = a[0] * a[0] + (1. - a[0] * a[0]) * ct;
 	R[1] = a[0] * a[1] * uct - a[2] * st;
 	R[2] = a[0] * a[2] * uct + a[1] * st;
@@ -201,6 +300,11 @@ __device__ __forceinline__ void rotate_body(const double O[9], const double a[3]
 	R[6] = a[0] * a[2] * uct - a[1] * st;
 	R[7] = a[1] * a[2] * uct + a[0] * st;
 	R[8] = a[2] * a[2] + (1. - a[2] * a[2]) * ct;
+}
+
+// O' = O R, rows of O (matrix_matrix_multiplication, utils.c:172-184;
+// utils_rotate_matrix, utils.c:204-209).
+__device__ __forceinline__ void mat_mul3(const double O[9], const double R[9], double On[9]) {
 #pragma unroll
 	for(int i = 0; i < 3; i++)
 #pragma unroll

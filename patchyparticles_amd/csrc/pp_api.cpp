@@ -63,6 +63,7 @@ struct pp_ctx {
 	std::vector<int> np, p_off, act_off;
 	DevState d{};
 	int T = 32, block = 256;
+	int coop_blocks = 0;  // > 0: the passes of a sweep run as one cooperative launch of that many workgroups
 	long long sweep = 0;
 	bool any_npt = false;
 	long long trace_len = -1;  // entries written to d.trace; -1 = tracing off
@@ -137,6 +138,13 @@ pp_status check_err(pp_ctx *c) {
 	HIPCHECK(hipGetLastError());
 	if(e & ERR_CELL_CAP) return fail(PP_ERR_CELL_CAP, "a sweep cell holds more than %d particles", CELL_CAP);
 	if(e & ERR_BOX) return fail(PP_ERR_BOX, "box too small for the cell grid");
+	if(e & ERR_SCAN) return fail(PP_ERR_HIP, "cell scan look-back did not complete");
+	if(e & ERR_BARRIER) {
+		launch_fill_i32((int *) c->d.bar, 0, 2, c->stream);
+		launch_fill_i32(c->d.err, 0, 1, c->stream);
+		HIPCHECK(hipStreamSynchronize(c->stream));
+		return fail(PP_ERR_HIP, "a grid barrier of the sweep kernel timed out");
+	}
 	return PP_OK;
 }
 
@@ -175,8 +183,7 @@ pp_status collect_timing(pp_ctx *c) {
 void rebuild_now(pp_ctx *c) {
 	hipEvent_t a{};
 	tic(c, TK_REBUILD, &a);
-	launch_rebuild(c->d, 0, c->stream);
-	launch_flip(c->d, c->stream);
+	launch_rebuild(c->d, REB_KEEP, 0, -1, c->stream);
 	toc(c, TK_REBUILD, a);
 }
 
@@ -184,7 +191,7 @@ pp_status energy_now(pp_ctx *c, int per_particle) {
 	rebuild_now(c);
 	hipEvent_t a{};
 	tic(c, TK_ENERGY, &a);
-	launch_energy(c->d, 0, per_particle, c->T, c->stream);
+	launch_energy(c->d, 0, per_particle, c->stream);
 	toc(c, TK_ENERGY, a);
 	HIPCHECK(hipStreamSynchronize(c->stream));
 	return PP_OK;
@@ -209,12 +216,13 @@ pp_status create_impl(int R, const pp_params *prm, const pp_options *opts, const
 	if(R <= 0 || !prm || !n || !box) return fail(PP_ERR_ARG, "bad replica arguments");
 	pp_options o{};
 	if(opts) o = *opts;
-	if(o.threads_per_cell == 0) o.threads_per_cell = 64;
+	if(o.threads_per_cell == 0) o.threads_per_cell = 32;
 	if(o.block_size == 0) o.block_size = 256;
 	const int Tv = o.threads_per_cell;
-	if(Tv != 8 && Tv != 16 && Tv != 32 && Tv != 64) return fail(PP_ERR_ARG, "threads_per_cell must be 8/16/32/64");
+	if(Tv != 32 && Tv != 64) return fail(PP_ERR_ARG, "threads_per_cell must be 32 or 64");
 	if(o.block_size % 64 || o.block_size < 64 || o.block_size > 256)
 		return fail(PP_ERR_ARG, "block_size must be 64, 128, 192 or 256");
+	if(o.launch < PP_LAUNCH_DEFAULT || o.launch > PP_LAUNCH_COOP) return fail(PP_ERR_ARG, "launch must be 0, 1 or 2");
 	if(pass_lds_bytes(Tv, o.block_size) > MAX_LDS_BYTES)
 		return fail(PP_ERR_ARG, "threads_per_cell %d with block_size %d needs %zu bytes of LDS (max %zu)", Tv,
 		            o.block_size, pass_lds_bytes(Tv, o.block_size), MAX_LDS_BYTES);
@@ -261,6 +269,7 @@ pp_status create_impl(int R, const pp_params *prm, const pp_options *opts, const
 		rp.rc2 = rcut * rcut;  // SQR(1. + delta) (system.c:86)
 		rp.cosmax = p.kf_cosmax;
 		rp.T = p.temperature;
+		for(int e = 0; e < ACC_TAB; e++) rp.acc_tab[e] = exp(-(double) e / p.temperature);  // MC.c:332
 		rp.disp_max = p.disp_max;
 		rp.theta_max = p.theta_max;
 		rp.P = p.pressure;
@@ -295,6 +304,7 @@ pp_status create_impl(int R, const pp_params *prm, const pp_options *opts, const
 		rp.np = n[k];
 		rp.cell_off = (int) coff;
 		rp.cell_cap = rp.cap_n[0] * rp.cap_n[1] * rp.cap_n[2];
+		rp.act_cap = (rp.cap_n[0] / 2) * (rp.cap_n[1] / 2) * (rp.cap_n[2] / 2);
 		if(rp.cell_cap >= (1 << 24))
 			return bail(fail(PP_ERR_BOX, "replica %d: %d cells exceed the 2^24 cells one replica may hold", k,
 			                 rp.cell_cap));
@@ -316,19 +326,51 @@ pp_status create_impl(int R, const pp_params *prm, const pp_options *opts, const
 	d.C_total = (int) coff;
 	d.ntiles = (d.C_total + SCAN_TILE - 1) / SCAN_TILE;
 	d.total_act = (int) aoff;
-	d.base = host_base_patches();
+	{
+		// the kernels carry the base patches as compile-time constants
+		const Base hb = host_base_patches();
+		for(int i = 0; i < 4; i++)
+			for(int k = 0; k < 3; k++)
+				if(hb.b[i][k] != kf_base(i, k))
+					return bail(fail(PP_ERR_HIP, "base patch (%d,%d) computed on the host differs from the kernels'", i, k));
+	}
 	d.seed = o.seed;
+	if(o.launch == PP_LAUNCH_COOP) {
+		// the passes of a sweep as one cooperative launch: as many workgroups
+		// as the lists of one pass can use, at most as many as stay resident
+		int coop = 0, cus = 0;
+		hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, o.device);
+		hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, o.device);
+		const int tpb = o.block_size / Tv;
+		const int need = std::max(1, (d.total_act + tpb - 1) / tpb);
+		const int fit = (coop && cus > 0) ? sweep_blocks_max(Tv, o.block_size, cus) : 0;
+		if(fit <= 0)
+			return bail(fail(PP_ERR_UNSUPPORTED, "device %d cannot launch the sweep kernel cooperatively", o.device));
+		c->coop_blocks = std::min(need, fit);
+	}
 	pp_status st;
 	if((st = dev_alloc(c, &d.f[0], (size_t) NCOMP * d.stride)) || (st = dev_alloc(c, &d.f[1], (size_t) NCOMP * d.stride)) ||
 	   (st = dev_alloc(c, &d.pid[0], d.stride)) || (st = dev_alloc(c, &d.pid[1], d.stride)) ||
 	   (st = dev_alloc(c, &d.reps, R)) || (st = dev_alloc(c, (int **) &d.prep, d.stride)) ||
 	   (st = dev_alloc(c, (int **) &d.act_off, R + 1)) || (st = dev_alloc(c, &d.count, d.C_total + 1)) ||
-	   (st = dev_alloc(c, &d.loc, d.C_total + 1)) || (st = dev_alloc(c, &d.start, d.C_total + 1)) ||
-	   (st = dev_alloc(c, &d.slot, d.stride)) || (st = dev_alloc(c, &d.cellid, d.stride)) ||
-	   (st = dev_alloc(c, &d.idx, d.stride)) || (st = dev_alloc(c, &d.tile_sum, std::max(d.ntiles, 1))) ||
+	   (st = dev_alloc(c, &d.start, d.C_total + 1)) || (st = dev_alloc(c, &d.slot, d.stride)) ||
+	   (st = dev_alloc(c, &d.cellid, d.stride)) || (st = dev_alloc(c, &d.scan_status, std::max(d.ntiles, 1))) ||
+	   (st = dev_alloc(c, &d.scan_ticket, 2)) || (st = dev_alloc(c, &d.trials, (size_t) TRIAL_W * d.stride)) ||
+	   (st = dev_alloc(c, &d.clist, d.C_total + 1)) || (st = dev_alloc(c, &d.ccount, (size_t) R * 8)) ||
+	   (st = dev_alloc(c, &d.pfx, (size_t) 8 * (R + 1))) ||
+	   (st = dev_alloc(c, &d.rec, (size_t) 8 * std::max(d.total_act, 1) * REC_W)) ||
 	   (st = dev_alloc(c, &d.ctr, (size_t) R * NCTR * NSLOT)) || (st = dev_alloc(c, &d.err, 1)) ||
-	   (st = dev_alloc(c, &d.out_bonds, d.stride)) || (st = dev_alloc(c, &d.out_ovl, d.stride)))
+	   (st = dev_alloc(c, &d.out_bonds, d.stride)) || (st = dev_alloc(c, &d.out_ovl, d.stride)) ||
+	   (st = dev_alloc(c, &d.bar, 2)))
 		return bail(st);
+	{
+		// every buffer a kernel may touch must exist before the first launch
+		const void *need[] = {d.f[0], d.f[1], d.pid[0], d.pid[1], d.reps, d.prep, d.act_off, d.count, d.start,
+		                      d.slot, d.cellid, d.scan_status, d.clist, d.ccount, d.pfx, d.rec, d.scan_ticket,
+		                      d.ctr, d.err, d.out_bonds, d.out_ovl, d.trials, d.bar};
+		for(const void *q : need)
+			if(!q) return bail(fail(PP_ERR_NOMEM, "a device buffer was not allocated"));
+	}
 
 	// host staging: SoA in particle-index order, slot j = particle j
 	{
@@ -352,6 +394,7 @@ pp_status create_impl(int R, const pp_params *prm, const pp_options *opts, const
 	}
 	launch_fill_i32(d.count, 0, d.C_total + 1, c->stream);
 	launch_fill_i32(d.err, 0, 1, c->stream);
+	launch_fill_i32((int *) d.bar, 0, 2, c->stream);
 	launch_fill_u64(d.ctr, 0ull, (long long) R * NCTR * NSLOT, c->stream);
 
 	// initial energy (main.c:54-61): any overlap is an error
@@ -395,22 +438,27 @@ pp_status pp_sweeps(pp_ctx *c, long long nsw) {
 		tic(c, TK_REBUILD, &a);
 		long long tp = -1;
 		if(c->trace_len >= 0 && c->trace_len < c->d.trace_cap) tp = c->trace_len++;
-		launch_prep(c->d, sw, tp, c->stream);
-		launch_rebuild(c->d, 0, c->stream);
-		launch_flip(c->d, c->stream);
+		launch_rebuild(c->d, REB_SHIFT, sw, tp, c->stream);
 		toc(c, TK_REBUILD, a);
-		for(int k = 0; k < 8; k++) {
+		if(c->coop_blocks > 0) {
 			hipEvent_t b{};
 			tic(c, TK_PASS, &b);
-			launch_pass(c->d, sw, k, c->T, c->block, c->stream);
+			HIPCHECK(launch_sweep_passes(c->d, sw, c->T, c->block, c->coop_blocks, c->stream));
 			toc(c, TK_PASS, b);
 		}
+		else
+			for(int k = 0; k < 8; k++) {
+				hipEvent_t b{};
+				tic(c, TK_PASS, &b);
+				launch_pass(c->d, sw, k, c->T, c->block, c->stream);
+				toc(c, TK_PASS, b);
+			}
 		if(c->any_npt) {
 			hipEvent_t v{};
 			tic(c, TK_VOLUME, &v);
 			launch_vol_propose(c->d, sw, c->stream);
-			launch_rebuild(c->d, 1, c->stream);
-			launch_energy(c->d, 1, 0, c->T, c->stream);
+			launch_rebuild(c->d, REB_TRIAL, sw, -1, c->stream);
+			launch_energy(c->d, 1, 0, c->stream);
 			launch_vol_decide(c->d, sw, c->stream);
 			toc(c, TK_VOLUME, v);
 		}

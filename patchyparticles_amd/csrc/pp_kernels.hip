@@ -3,12 +3,13 @@
 // Data layout in HBM (per context, all replicas concatenated):
 //   f[2][12][stride]  double  x, y, z, O00..O22 of slot j (SoA, two buffers)
 //   pid[2][stride]    int     particle index of slot j
-//   cells             int     count/loc/start over the concatenated cell
-//                             regions of all replicas (one region per replica,
-//                             sized for its largest grid)
-// Slots are sorted by cell at every rebuild; inside a cell by particle index,
-// so the order (and with it the whole trajectory) depends only on the state,
-// never on thread timing.
+//   start[C+1]        int     first slot of each cell, over the concatenated
+//                             cell regions of all replicas (one region per
+//                             replica, sized for its largest grid)
+// Slots are grouped by cell at every rebuild (order inside a cell is
+// whatever the atomics gave); the colour pass ranks a cell's members by
+// particle index before it draws from them, so the trajectory depends only
+// on the state and the seed, never on thread timing.
 //
 // Sweep = rebuild on a randomly shifted grid + 8 colour passes in random
 // order.  In a pass every cell of one colour (i,j,k parities) is active; its
@@ -40,6 +41,17 @@ __device__ __forceinline__ int team_sum(int v) {
 	return v;
 }
 
+// Sum over a team of small non-negative integers (< 8): one ballot per bit.
+template <int T>
+__device__ __forceinline__ int team_sum_small(int v) {
+	const int off = (threadIdx.x & 63) & ~(T - 1);
+	const unsigned long long tm = (T == 64) ? ~0ull : ((1ull << T) - 1ull);
+	int s = 0;
+#pragma unroll
+	for(int b = 0; b < 3; b++) s += __popcll((__ballot((v >> b) & 1) >> off) & tm) << b;
+	return s;
+}
+
 template <int T>
 __device__ __forceinline__ bool team_any(bool p) {
 	unsigned long long m = __ballot(p);
@@ -67,12 +79,6 @@ __device__ __forceinline__ double unid(double v) {
 	else return v;
 }
 
-template <int T>
-__device__ __forceinline__ u32x4 uni4(u32x4 v) {
-	return u32x4{(uint32_t) uni<T>((int) v.x), (uint32_t) uni<T>((int) v.y), (uint32_t) uni<T>((int) v.z),
-	             (uint32_t) uni<T>((int) v.w)};
-}
-
 // Lanes of one team live in one wavefront: order their LDS traffic.
 __device__ __forceinline__ void team_sync() {
 	__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -83,6 +89,8 @@ __device__ __forceinline__ void team_sync() {
 __device__ __forceinline__ unsigned long long *ctr_at(unsigned long long *ctr, int r, int row, int slot) {
 	return ctr + ((long long) (r * NCTR + row) * NSLOT + slot);
 }
+
+__device__ __forceinline__ int wrapi(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
 
 // ------------------------------------------------------------------------
 // fills
@@ -117,63 +125,98 @@ __global__ void k_zero_energy_rows(unsigned long long *ctr, int R) {
 }
 
 // ------------------------------------------------------------------------
-// per-sweep replica set-up: random grid shift in [0, side) per axis
+// rebuild: count -> scan (flips the buffer selector) -> scatter
+//
+// mode REB_KEEP   regroup on the current grid and shift
+//      REB_SHIFT  new random grid shift for this sweep (uniform in one cell
+//                 side per axis), recorded by the first slot of each replica
+//      REB_TRIAL  NPT trial: the replicas with a trial in flight are counted
+//                 on their trial grid with the trial scaling of one axis
+//                 (MC.c:475-478); the others on their grid, unscaled, so that
+//                 every replica's cells still start at its own first slot
 
-__global__ void k_prep(DevState d, uint64_t sweep, long long trace_pos) {
-	int r = blockIdx.x * blockDim.x + threadIdx.x;
-	if(r >= d.R) return;
-	Rep &rp = d.reps[r];
-	// energy trace: bonded pairs as they stand after the previous sweep
-	if(d.trace && trace_pos >= 0 && trace_pos < d.trace_cap) {
-		unsigned long long db = 0;
-		for(int k = 0; k < NSLOT; k++) db += *ctr_at(d.ctr, r, C_DBONDS, k);
-		d.trace[(long long) r * d.trace_cap + trace_pos] = rp.bonds_base + (long long) db;
-	}
+__device__ __forceinline__ void sweep_shift(const DevState &d, int r, const Grid &G, uint64_t sweep, double sh[3]) {
 	u32x4 v = rng(d.seed, r, sweep, 0, TAG_SHIFT, 0, 0);
-	uint32_t w[3] = {v.x, v.y, v.z};
-	for(int k = 0; k < 3; k++) rp.g.shift[k] = u01(w[k]) * (rp.g.box[k] / (double) rp.g.n[k]);
-	rp.trial_active = 0;
+	sh[0] = u01(v.x) * (G.box[0] / (double) G.n[0]);
+	sh[1] = u01(v.y) * (G.box[1] / (double) G.n[1]);
+	sh[2] = u01(v.z) * (G.box[2] / (double) G.n[2]);
 }
 
-void launch_prep(const DevState &d, uint64_t sweep, long long trace_pos, hipStream_t s) {
-	hipLaunchKernelGGL(k_prep, dim3((d.R + 63) / 64), dim3(64), 0, s, d, sweep, trace_pos);
-}
-
-__global__ void k_flip(Rep *reps, int R) {
-	int r = blockIdx.x * blockDim.x + threadIdx.x;
-	if(r < R) reps[r].sel ^= 1;
-}
-
-void launch_flip(const DevState &d, hipStream_t s) {
-	hipLaunchKernelGGL(k_flip, dim3((d.R + 63) / 64), dim3(64), 0, s, d.reps, d.R);
-}
-
-// ------------------------------------------------------------------------
-// rebuild: count -> scan -> scatter -> per-cell sort -> gather
-
-__global__ void k_count(DevState d, int trial) {
+__global__ void k_count(DevState d, int mode, uint64_t sweep, long long trace_pos) {
 	int j = blockIdx.x * blockDim.x + threadIdx.x;
+	// reset the scan's look-back state for the scan that follows
+	for(int t = j; t < d.ntiles; t += gridDim.x * blockDim.x) d.scan_status[t] = 0ull;
+	for(int t = j; t < d.R * 8; t += gridDim.x * blockDim.x) d.ccount[t] = 0;
+	if(j == 0) {
+		d.scan_ticket[0] = 0;
+		d.scan_ticket[1] = 0;
+	}
 	if(j >= d.Ntot) return;
-	const Rep &rp = d.reps[d.prep[j]];
-	const bool tr = trial && rp.trial_active;
+	const int r = d.prep[j];
+	Rep &rp = d.reps[r];
+	const bool tr = (mode == REB_TRIAL) && rp.trial_active;
 	const Grid &G = tr ? rp.tg : rp.g;
+	double sh[3] = {G.shift[0], G.shift[1], G.shift[2]};
+	if(mode == REB_SHIFT) {
+		sweep_shift(d, r, G, sweep, sh);
+		if(j == rp.p_off) {
+			// energy trace: bonded pairs as they stand after the previous sweep
+			if(d.trace && trace_pos >= 0 && trace_pos < d.trace_cap) {
+				unsigned long long db = 0;
+				for(int k = 0; k < NSLOT; k++) db += *ctr_at(d.ctr, r, C_DBONDS, k);
+				d.trace[(long long) r * d.trace_cap + trace_pos] = rp.bonds_base + (long long) db;
+			}
+			rp.g.shift[0] = sh[0];
+			rp.g.shift[1] = sh[1];
+			rp.g.shift[2] = sh[2];
+			// random colour order of this sweep's 8 passes
+			u32x4 v = rng(d.seed, r, sweep, 0, TAG_COLOUR, 0, 0);
+			const uint32_t w4[4] = {v.x, v.y, v.z, v.w};
+			rp.col_perm = colour_perm_from(w4);
+		}
+	}
 	const double *F = d.f[rp.sel];
 	const int S = d.stride;
 	double p[3] = {F[j], F[S + j], F[2 * S + j]};
-	if(tr) p[rp.trial_dir] *= rp.trial_f;  // MC.c:475-478
-	int ix = cell_coord(p[0], G.shift[0], G.inv_box[0], G.n[0]);
-	int iy = cell_coord(p[1], G.shift[1], G.inv_box[1], G.n[1]);
-	int iz = cell_coord(p[2], G.shift[2], G.inv_box[2], G.n[2]);
+	if(tr) p[rp.trial_dir] *= rp.trial_f;
+	int ix = cell_coord(p[0], sh[0], G.inv_box[0], G.n[0]);
+	int iy = cell_coord(p[1], sh[1], G.inv_box[1], G.n[1]);
+	int iz = cell_coord(p[2], sh[2], G.inv_box[2], G.n[2]);
 	int c = rp.cell_off + (ix * G.n[1] + iy) * G.n[2] + iz;
 	d.cellid[j] = c;
 	d.slot[j] = atomicAdd(&d.count[c], 1);
 }
 
-// Tile-local exclusive scan of count (and zero count for the next rebuild).
-__global__ __launch_bounds__(256) void k_scan1(DevState d) {
+// (replica, colour) keys of the non-empty-cell lists counted per scan workgroup
+constexpr int SCAN_KEYS = 512;
+
+// Exclusive scan of the cell counts in one pass (decoupled look-back):
+// workgroups take tiles in dispatch order from a ticket, publish their tile
+// sum, then read their predecessors' published values, 64 at a time, back to
+// the nearest one that carries an inclusive prefix.  A workgroup only ever waits for tiles with a
+// smaller ticket, which are already running.  start[c] = first slot of cell
+// c; count is cleared for the next rebuild; a flipping scan also switches
+// every replica to the buffer the scatter fills.
+__global__ __launch_bounds__(256) void k_scan(DevState d, int flip, int lists) {
 	__shared__ int wsum[4];
+	__shared__ int sh_tile, sh_excl, sh_r0, sh_last;
+	__shared__ int lcnt[SCAN_KEYS], lbase[SCAN_KEYS];
 	const int t = threadIdx.x;
-	const int base = blockIdx.x * SCAN_TILE + t * 8;
+	if(t == 0) sh_tile = atomicAdd(d.scan_ticket, 1);
+	for(int k = t; k < SCAN_KEYS; k += 256) lcnt[k] = 0;
+	__syncthreads();
+	const int tile = sh_tile;
+	const int base = tile * SCAN_TILE + t * 8;
+	if(t == 0 && lists) {
+		// replica of the tile's first cell
+		int lo = 0, hi = d.R;
+		while(hi - lo > 1) {
+			int mid = (lo + hi) >> 1;
+			if(d.reps[mid].cell_off <= tile * SCAN_TILE) lo = mid;
+			else hi = mid;
+		}
+		sh_r0 = lo;
+	}
 	int v[8];
 	int tot = 0;
 #pragma unroll
@@ -182,7 +225,6 @@ __global__ __launch_bounds__(256) void k_scan1(DevState d) {
 		v[e] = c < d.C_total ? d.count[c] : 0;
 		tot += v[e];
 	}
-	// inclusive scan of thread totals within the wavefront
 	int lane = t & 63, wid = t >> 6;
 	int inc = tot;
 #pragma unroll
@@ -192,190 +234,127 @@ __global__ __launch_bounds__(256) void k_scan1(DevState d) {
 	}
 	if(lane == 63) wsum[wid] = inc;
 	__syncthreads();
+	if(wid == 0) {
+		// wavefront 0 publishes the tile's sum and looks back over up to 64
+		// predecessors at once: the nearest one with an inclusive prefix ends
+		// the walk; one that has published nothing yet is polled again
+		const int agg = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+		int excl = 0;
+		if(lane == 0)
+			__hip_atomic_store(&d.scan_status[tile], ((tile == 0 ? 2ull : 1ull) << 32) | (unsigned) agg,
+			                   __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+		int p = tile - 1, spins = 0;
+		while(p >= 0) {
+			const int q = p - lane;
+			unsigned long long st = 2ull << 32;  // before tile 0: an inclusive prefix of 0
+			if(q >= 0) st = __hip_atomic_load(&d.scan_status[q], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+			const unsigned flag = (unsigned) (st >> 32);
+			const unsigned long long m2 = __ballot(flag == 2u);
+			const int l2 = m2 ? __ffsll((long long) m2) - 1 : 64;
+			const unsigned long long upto = l2 >= 63 ? ~0ull : ((2ull << l2) - 1ull);
+			if(__ballot(flag == 0u) & upto) {
+				if(++spins > SCAN_SPIN_MAX) {
+					if(lane == 0) atomicOr(d.err, (int) ERR_SCAN);
+					break;
+				}
+				__builtin_amdgcn_s_sleep(1);
+				continue;
+			}
+			int val = lane <= l2 ? (int) (st & 0xffffffffull) : 0;
+#pragma unroll
+			for(int o = 32; o > 0; o >>= 1) val += __shfl_xor(val, o, 64);
+			excl += val;
+			if(l2 < 64) break;
+			p -= 64;
+		}
+		if(lane == 0) {
+			if(tile > 0)
+				__hip_atomic_store(&d.scan_status[tile], (2ull << 32) | (unsigned) (excl + agg), __ATOMIC_RELEASE,
+				                   __HIP_MEMORY_SCOPE_AGENT);
+			sh_excl = excl;
+		}
+	}
+	__syncthreads();
 	int woff = 0;
 	for(int w = 0; w < wid; w++) woff += wsum[w];
-	int run = woff + inc - tot;
+	int run = sh_excl + woff + inc - tot;
+	int lkey[8], lpos[8];
 #pragma unroll
 	for(int e = 0; e < 8; e++) {
 		int c = base + e;
+		lkey[e] = -1;
+		lpos[e] = 0;
 		if(c < d.C_total) {
-			d.loc[c] = run;
+			d.start[c] = run;
 			d.count[c] = 0;
+			if(lists && v[e] > 0) {
+				// the next passes visit only non-empty cells: list them by
+				// colour, first counted per workgroup in LDS
+				int lo = sh_r0, hi = d.R;
+				while(hi - lo > 1) {
+					int mid = (lo + hi) >> 1;
+					if(d.reps[mid].cell_off <= c) lo = mid;
+					else hi = mid;
+				}
+				const Rep &rp = d.reps[lo];
+				const int cloc = c - rp.cell_off;
+				const int n1 = rp.g.n[1], n2 = rp.g.n[2];
+				const int ix = cloc / (n1 * n2), iy = (cloc / n2) % n1, iz = cloc % n2;
+				const int col = ((ix & 1) << 2) | ((iy & 1) << 1) | (iz & 1);
+				const int key = (lo - sh_r0) * 8 + col;
+				if(key < SCAN_KEYS) {
+					lkey[e] = key;
+					lpos[e] = atomicAdd(&lcnt[key], 1);
+				}
+				else {
+					// a tile spanning more than SCAN_KEYS / 8 replicas: direct
+					const int k = atomicAdd(&d.ccount[lo * 8 + col], 1);
+					d.clist[rp.cell_off + col * rp.act_cap + k] = cloc;
+				}
+			}
 		}
 		run += v[e];
 	}
-	if(t == 255) d.tile_sum[blockIdx.x] = woff + inc;
-}
-
-// Exclusive scan of the tile sums, one workgroup of 1024.
-__global__ __launch_bounds__(1024) void k_scan2(DevState d) {
-	__shared__ int wsum[16];
-	const int t = threadIdx.x;
-	const int per = (d.ntiles + 1023) / 1024;
-	const int b = t * per;
-	int tot = 0;
-	for(int e = 0; e < per; e++)
-		if(b + e < d.ntiles) tot += d.tile_sum[b + e];
-	int lane = t & 63, wid = t >> 6;
-	int inc = tot;
-#pragma unroll
-	for(int o = 1; o < 64; o <<= 1) {
-		int u = __shfl_up(inc, o, 64);
-		if(lane >= o) inc += u;
-	}
-	if(lane == 63) wsum[wid] = inc;
-	__syncthreads();
-	int woff = 0;
-	for(int w = 0; w < wid; w++) woff += wsum[w];
-	int run = woff + inc - tot;
-	for(int e = 0; e < per; e++)
-		if(b + e < d.ntiles) {
-			int x = d.tile_sum[b + e];
-			d.tile_sum[b + e] = run;
-			run += x;
+	if(lists) {
+		__syncthreads();
+		for(int k = t; k < SCAN_KEYS; k += 256) {
+			const int n = lcnt[k];
+			if(n > 0) lbase[k] = atomicAdd(&d.ccount[(sh_r0 + k / 8) * 8 + k % 8], n);
 		}
-}
-
-__global__ void k_fix_scatter(DevState d) {
-	int t = blockIdx.x * blockDim.x + threadIdx.x;
-	if(t < d.C_total) d.start[t] = d.loc[t] + d.tile_sum[t / SCAN_TILE];
-	else if(t == d.C_total) d.start[t] = d.Ntot;
-	if(t < d.Ntot) {
-		int c = d.cellid[t];
-		d.idx[d.loc[c] + d.tile_sum[c / SCAN_TILE] + d.slot[t]] = t;
-	}
-}
-
-// Order each cell's slots by particle index (insertion sort; cells are small).
-__global__ void k_sort(DevState d) {
-	int c = blockIdx.x * blockDim.x + threadIdx.x;
-	if(c >= d.C_total) return;
-	int s = d.start[c], e = d.start[c + 1];
-	if(e - s < 2) return;
-	const int *pid = d.pid[d.reps[d.prep[d.idx[s]]].sel];
-	for(int a = s + 1; a < e; a++) {
-		int v = d.idx[a];
-		int key = pid[v];
-		int b = a - 1;
-		while(b >= s && pid[d.idx[b]] > key) {
-			d.idx[b + 1] = d.idx[b];
-			b--;
+		__syncthreads();
+#pragma unroll
+		for(int e = 0; e < 8; e++)
+			if(lkey[e] >= 0) {
+				const int r = sh_r0 + lkey[e] / 8;
+				const Rep &rp = d.reps[r];
+				const int cloc = base + e - rp.cell_off;
+				d.clist[rp.cell_off + (lkey[e] & 7) * rp.act_cap + lbase[lkey[e]] + lpos[e]] = cloc;
+			}
+		// the workgroup that finishes last lays out the passes' lists:
+		// pfx[k][r] = first entry of replica r in the list of pass k
+		__builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+		__syncthreads();
+		if(t == 0)
+			sh_last = __hip_atomic_fetch_add(&d.scan_ticket[1], 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) ==
+			          (int) gridDim.x - 1;
+		__syncthreads();
+		if(sh_last) {
+			__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+			if(t < 8) {
+				int acc = 0;
+				d.pfx[t * (d.R + 1)] = 0;
+				for(int r = 0; r < d.R; r++) {
+					const int col = (int) ((d.reps[r].col_perm >> (4 * t)) & 0xFu);
+					acc += d.ccount[r * 8 + col];
+					d.pfx[t * (d.R + 1) + r + 1] = acc;
+				}
+			}
 		}
-		d.idx[b + 1] = v;
 	}
-}
-
-__global__ void k_gather(DevState d, int trial) {
-	int j = blockIdx.x * blockDim.x + threadIdx.x;
-	if(j >= d.Ntot) return;
-	const Rep &rp = d.reps[d.prep[j]];
-	const bool tr = trial && rp.trial_active;
-	if(trial && !tr) return;
-	const int src = d.idx[j];
-	const double *F = d.f[rp.sel];
-	double *G = d.f[rp.sel ^ 1];
-	const int S = d.stride;
-#pragma unroll
-	for(int c = 0; c < NCOMP; c++) {
-		double v = F[c * S + src];
-		if(tr && c == rp.trial_dir) v *= rp.trial_f;
-		G[c * S + j] = v;
-	}
-	d.pid[rp.sel ^ 1][j] = d.pid[rp.sel][src];
-}
-
-void launch_rebuild(const DevState &d, int trial, hipStream_t s) {
-	const int B = 256;
-	unsigned gp = (unsigned) ((d.Ntot + B - 1) / B);
-	hipLaunchKernelGGL(k_count, dim3(gp), dim3(B), 0, s, d, trial);
-	hipLaunchKernelGGL(k_scan1, dim3(d.ntiles), dim3(256), 0, s, d);
-	hipLaunchKernelGGL(k_scan2, dim3(1), dim3(1024), 0, s, d);
-	long long nfix = (d.C_total + 1 > d.Ntot) ? d.C_total + 1 : d.Ntot;
-	hipLaunchKernelGGL(k_fix_scatter, dim3((unsigned) ((nfix + B - 1) / B)), dim3(B), 0, s, d);
-	hipLaunchKernelGGL(k_sort, dim3((unsigned) ((d.C_total + B - 1) / B)), dim3(B), 0, s, d);
-	hipLaunchKernelGGL(k_gather, dim3(gp), dim3(B), 0, s, d, trial);
-}
-
-// ------------------------------------------------------------------------
-// neighbour-cell enumeration shared by the pass and energy kernels:
-// lane l of a team of T takes the cells nb = l/S, l/S + T/S, ... (nb in
-// 0..26, dx = nb/9-1, dy = (nb/3)%3-1, dz = nb%3-1) and, inside each, the
-// members m = l%S, l%S + S, ...   S = 2 for T = 64, else 1.
-
-template <int T>
-struct NbLayout {
-	static constexpr int S = (T >= 54) ? 2 : 1;
-	static constexpr int STEP = T / S;
-	static constexpr int PER = (27 + STEP - 1) / STEP;
-};
-
-__device__ __forceinline__ int wrapi(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
-
-// ------------------------------------------------------------------------
-// the colour pass
-
-struct PassArgs {
-	DevState d;
-	uint64_t sweep;
-	int pass;
-};
-
-// per-team LDS: the cell's own particles, then the trial's patch vectors
-// (old, new) and new orientation
-constexpr int TEAM_LDS = CELL_CAP * NCOMP + 36;
-
-// The 26 cells around an active cell (the cell itself is handled apart,
-// from the team's LDS copy).  Lane l takes e = l/S, l/S + T/S, ...
-template <int T>
-struct Nb26 {
-	static constexpr int S = (T >= 52) ? 2 : 1;
-	static constexpr int STEP = T / S;
-	static constexpr int PER = (26 + STEP - 1) / STEP;
-};
-
-// Trial state shared by the helpers of one team.
-struct Trial {
-	double px, py, pz, nx, ny, nz;
-	double Lx, Ly, Lz, iLx, iLy, iLz, rc2, cosmax;
-};
-
-// hard-core test of the new position against q
-__device__ __forceinline__ bool overlaps_new(const Trial &t, double qx, double qy, double qz) {
-	double dx = kf_min_image(qx - t.nx, t.Lx, t.iLx);
-	double dy = kf_min_image(qy - t.ny, t.Ly, t.iLy);
-	double dz = kf_min_image(qz - t.nz, t.Lz, t.iLz);
-	return (dx * dx + dy * dy + dz * dz) < 1.;
-}
-
-// bonds of q with the old and the new position; q's orientation is read
-// only when one of the two passes the p-side angular test.  OWN selects the
-// team's LDS copy (compile time, so no pointer ever mixes address spaces).
-template <bool OWN>
-__device__ __forceinline__ void bonds_old_new(const Trial &t, const double *Ps, const Base &B, const double *src,
-                                              int S, int q, double qx, double qy, double qz, int &bo, int &bn) {
-	double ox = kf_min_image(qx - t.px, t.Lx, t.iLx);
-	double oy = kf_min_image(qy - t.py, t.Ly, t.iLy);
-	double oz = kf_min_image(qz - t.pz, t.Lz, t.iLz);
-	double o2 = ox * ox + oy * oy + oz * oz;
-	double wx = kf_min_image(qx - t.nx, t.Lx, t.iLx);
-	double wy = kf_min_image(qy - t.ny, t.Ly, t.iLy);
-	double wz = kf_min_image(qz - t.nz, t.Lz, t.iLz);
-	double w2 = wx * wx + wy * wy + wz * wz;
-	bool po = false, pn = false;
-	if(o2 < t.rc2) {
-		kf_unit(ox, oy, oz, o2);
-		po = kf_any_p(ox, oy, oz, Ps, t.cosmax);
-	}
-	if(w2 < t.rc2) {
-		kf_unit(wx, wy, wz, w2);
-		pn = kf_any_p(wx, wy, wz, Ps + 12, t.cosmax);
-	}
-	if(po || pn) {
-		double Oq[9];
-#pragma unroll
-		for(int c = 0; c < 9; c++) Oq[c] = OWN ? src[q * NCOMP + 3 + c] : src[(3 + c) * S + q];
-		if(po && kf_any_q(ox, oy, oz, Oq, B, t.cosmax)) bo++;
-		if(pn && kf_any_q(wx, wy, wz, Oq, B, t.cosmax)) bn++;
+	if(tile == 0) {
+		if(t == 0) d.start[d.C_total] = d.Ntot;
+		if(flip)
+			for(int r = t; r < d.R; r += 256) d.reps[r].sel ^= 1;
 	}
 }
 
@@ -411,210 +390,594 @@ __device__ __forceinline__ void sphere_axis(uint64_t seed, int r, uint64_t sweep
 	a[2] = (v.x & 1u) ? 1. : -1.;
 }
 
-template <int T>
-__global__ __launch_bounds__(256) void k_pass(PassArgs a) {
-	extern __shared__ double lds[];
-	using NB = Nb26<T>;
-	const DevState &d = a.d;
-	const int lane = threadIdx.x & (T - 1);
-	const int team = threadIdx.x / T;
-	const int tpb = blockDim.x / T;
-	double *own = lds + team * TEAM_LDS;
-	double *Ps = own + CELL_CAP * NCOMP;  // Po[12], Pn[12], On[9]
-	const int g = uni<T>(xcd_remap(blockIdx.x, gridDim.x) * tpb + team);
-	if(g >= d.total_act) return;
+// Trial t of cell cloc (colour col) in sweep `sweep`: member rank among
+// cnt, displacement uniform in +-disp_max/2 per axis (MC.c:317-319), rotation
+// by an angle uniform in [0, theta_max) about an axis uniform on the sphere
+// (MC.c:321-325, utils.c:186-202), and the acceptance draw (MC.c:332).
+__device__ __forceinline__ void make_trial(const DevState &d, const Rep &rp, int r, uint64_t sweep, int cloc, int col,
+                                           int t, int cnt, double *out) {
+	u32x4 ra = rng(d.seed, r, sweep, (uint32_t) cloc, TAG_TRIAL_A, (uint32_t) col, (uint32_t) t);
+	u32x4 rb = rng(d.seed, r, sweep, (uint32_t) cloc, TAG_TRIAL_B, (uint32_t) col, (uint32_t) t);
+	out[0] = (double) __umulhi(ra.x, (uint32_t) cnt);
+	out[1] = (u01(ra.y) - 0.5) * rp.disp_max;
+	out[2] = (u01(ra.z) - 0.5) * rp.disp_max;
+	out[3] = (u01(ra.w) - 0.5) * rp.disp_max;
+	double axis[3], st, ct, R[9];
+	sphere_axis(d.seed, r, sweep, (uint32_t) cloc, (uint32_t) col, (uint32_t) t, axis);
+	sincos(u01(rb.x) * rp.theta_max, &st, &ct);
+	rotation_matrix(axis, st, ct, R);
+#pragma unroll
+	for(int c = 0; c < 9; c++) out[4 + c] = R[c];
+	out[13] = u01(rb.w);
+}
 
+// Record of entry i of the cell lists (replica r's list of colour col at
+// clist[cell_off + col * act_cap + k]), stored at the entry's position in the
+// list of the pass that works on that colour: the 26 surrounding cells' first
+// slots [0, 26), the exclusive prefix of their populations [32, 59), the
+// cell's first slot, population, local index, replica and buffer selector
+// [59, 64).  A pass finds all its data from this one load.
+__device__ __forceinline__ void build_record(const DevState &d, int i) {
 	int lo = 0, hi = d.R;
 	while(hi - lo > 1) {
 		int mid = (lo + hi) >> 1;
-		if(d.act_off[mid] <= g) lo = mid;
+		if(d.reps[mid].cell_off <= i) lo = mid;
 		else hi = mid;
 	}
-	const int r = uni<T>(lo);
+	const Rep &rp = d.reps[lo];
+	const int local = i - rp.cell_off;
+	const int col = local / rp.act_cap, kk = local - col * rp.act_cap;
+	if(col >= 8 || kk >= d.ccount[lo * 8 + col]) return;
+	const int cloc = d.clist[i];
+	int k = 0;
+	while(k < 7 && (int) ((rp.col_perm >> (4 * k)) & 0xFu) != col) k++;
+	const int gi = d.pfx[k * (d.R + 1) + lo] + kk;
+	const int n0 = rp.g.n[0], n1 = rp.g.n[1], n2 = rp.g.n[2];
+	const int ix = cloc / (n1 * n2), iy = (cloc / n2) % n1, iz = cloc % n2;
+	const int *st = d.start + rp.cell_off;
+	int rec[REC_W];
+	int acc = 0;
+#pragma unroll
+	for(int nb = 0; nb < 26; nb++) {
+		const int m = nb < 13 ? nb : nb + 1;
+		const int cx = wrapi(ix + m / 9 - 1, n0), cy = wrapi(iy + (m / 3) % 3 - 1, n1), cz = wrapi(iz + m % 3 - 1, n2);
+		const int cc = (cx * n1 + cy) * n2 + cz;
+		const int a = st[cc], b = st[cc + 1];
+		rec[nb] = a;
+		rec[32 + nb] = acc;
+		acc += b - a;
+	}
+#pragma unroll
+	for(int e = 26; e < 32; e++) rec[e] = 0;
+	const int s0 = st[cloc];
+	rec[58] = acc;
+	rec[59] = s0;
+	rec[60] = st[cloc + 1] - s0;
+	rec[61] = cloc;
+	rec[62] = lo;
+	rec[63] = rp.sel;
+	int4 *out = (int4 *) (d.rec + ((size_t) k * d.total_act + gi) * REC_W);
+#pragma unroll
+	for(int e = 0; e < REC_W / 4; e++) out[e] = make_int4(rec[4 * e], rec[4 * e + 1], rec[4 * e + 2], rec[4 * e + 3]);
+}
+
+// Move every particle to its cell's slot range.  After a flipping scan the
+// source is the previous buffer (sel ^ 1); a trial writes into sel ^ 1.
+// With a new grid shift the workers also write the cell records of the
+// sweep's passes.
+__global__ void k_scatter(DevState d, int mode, uint64_t sweep) {
+	int j = blockIdx.x * blockDim.x + threadIdx.x;
+	if(mode == REB_SHIFT && j < d.C_total) build_record(d, j);
+	if(j >= d.Ntot) return;
+	const int r = d.prep[j];
 	const Rep &rp = d.reps[r];
-	const int n0 = uni<T>(rp.g.n[0]), n1 = uni<T>(rp.g.n[1]), n2 = uni<T>(rp.g.n[2]);
-	const int h1 = n1 >> 1, h2 = n2 >> 1;
-	const int local = g - d.act_off[r];
-	if(local >= (n0 >> 1) * h1 * h2) return;
-	const int col = uni<T>(colour_of_pass(d.seed, r, a.sweep, a.pass));
-	const int ix = uni<T>(2 * (local / (h2 * h1)) + ((col >> 2) & 1));
-	const int iy = uni<T>(2 * ((local / h2) % h1) + ((col >> 1) & 1));
-	const int iz = uni<T>(2 * (local % h2) + (col & 1));
-	const int cloc = uni<T>((ix * n1 + iy) * n2 + iz);
-	const int coff = uni<T>(rp.cell_off);
-	const int s0 = uni<T>(d.start[coff + cloc]);
-	const int cnt = uni<T>(d.start[coff + cloc + 1]) - s0;
-	if(cnt == 0) return;
-	if(cnt > CELL_CAP) {
-		if(lane == 0) atomicOr(d.err, (int) ERR_CELL_CAP);
-		return;
-	}
-	double *F = d.f[uni<T>(rp.sel)];
+	const bool tr = (mode == REB_TRIAL) && rp.trial_active;
+	if(mode == REB_TRIAL && !tr) return;
+	const int sb = tr ? rp.sel : (rp.sel ^ 1);
+	const double *F = d.f[sb];
+	double *G = d.f[sb ^ 1];
 	const int S = d.stride;
-
-	// stage the cell's own particles in LDS; the team owns them for the pass
-	for(int e = lane; e < cnt * NCOMP; e += T) {
-		int m = e / NCOMP, c = e - m * NCOMP;
-		own[e] = F[c * S + s0 + m];
-	}
-	// this lane's share of the 26 surrounding cells (fixed during the pass)
-	int nb_s[NB::PER], nb_n[NB::PER];
+	const int dst = d.start[d.cellid[j]] + d.slot[j];
 #pragma unroll
-	for(int u = 0; u < NB::PER; u++) {
-		int e = lane / NB::S + u * NB::STEP;
-		nb_s[u] = 0;
-		nb_n[u] = 0;
-		if(e < 26) {
-			int nb = e < 13 ? e : e + 1;
-			int cx = wrapi(ix + nb / 9 - 1, n0), cy = wrapi(iy + (nb / 3) % 3 - 1, n1), cz = wrapi(iz + nb % 3 - 1, n2);
-			int cc = coff + (cx * n1 + cy) * n2 + cz;
-			nb_s[u] = d.start[cc];
-			nb_n[u] = d.start[cc + 1] - nb_s[u];
+	for(int c = 0; c < NCOMP; c++) {
+		double v = F[c * S + j];
+		if(tr && c == rp.trial_dir) v *= rp.trial_f;
+		G[c * S + dst] = v;
+	}
+	d.pid[sb ^ 1][dst] = d.pid[sb][j];
+	if(mode == REB_SHIFT) {
+		// the worker for slot t of a cell writes that cell's trial t of this
+		// sweep (the trial depends on the cell, its colour and t only)
+		const int c = d.cellid[j];
+		const int cnt = d.start[c + 1] - d.start[c];
+		const int cloc = c - rp.cell_off;
+		const int n1 = rp.g.n[1], n2 = rp.g.n[2];
+		const int ix = cloc / (n1 * n2), iy = (cloc / n2) % n1, iz = cloc % n2;
+		const int col = ((ix & 1) << 2) | ((iy & 1) << 1) | (iz & 1);
+		double tr[TRIAL_W];
+		make_trial(d, rp, r, sweep, cloc, col, d.slot[j], cnt, tr);
+#pragma unroll
+		for(int e = 0; e < TRIAL_W; e++) d.trials[(long long) dst * TRIAL_W + e] = tr[e];
+	}
+}
+
+void launch_rebuild(const DevState &d, int mode, uint64_t sweep, long long trace_pos, hipStream_t s) {
+	const int B = 256;
+	unsigned gp = (unsigned) ((d.Ntot + B - 1) / B);
+	if(gp == 0) gp = 1;
+	const int flip = mode != REB_TRIAL;
+	hipLaunchKernelGGL(k_count, dim3(gp), dim3(B), 0, s, d, mode, sweep, trace_pos);
+	hipLaunchKernelGGL(k_scan, dim3(d.ntiles), dim3(256), 0, s, d, flip, mode == REB_SHIFT ? 1 : 0);
+	unsigned gs = gp;
+	if(mode == REB_SHIFT) gs = std::max(gp, (unsigned) ((d.C_total + B - 1) / B));
+	hipLaunchKernelGGL(k_scatter, dim3(gs), dim3(B), 0, s, d, mode, sweep);
+}
+
+// ------------------------------------------------------------------------
+// the colour pass
+
+struct PassArgs {
+	DevState d;
+	uint64_t sweep;
+	int pass;
+};
+
+// Per-team LDS (doubles): the cell's members ranked by particle index
+// (x, y, z, O, and their 4 patch vectors), their particle indices, the
+// cell's trials, the current trial's O' and patch vectors, the cell's record
+// (surrounding slot ranges) and the replica's acceptance table.
+constexpr int OWN_W = NCOMP + 12;   // x, y, z, O[9], P[12]
+constexpr int LDS_OWN = CELL_CAP * OWN_W;
+constexpr int LDS_PID = (CELL_CAP + 1) / 2;  // CELL_CAP ints
+constexpr int LDS_TRIAL = CELL_CAP * TRIAL_W;
+constexpr int LDS_NEW = 24;            // O'[9] at 0, P'[12] at 12
+constexpr int LDS_NB = REC_W / 2;      // the cell record, REC_W ints
+constexpr int LDS_ACC = ACC_TAB;       // exp(-k/T), k < ACC_TAB
+constexpr int TEAM_LDS = LDS_OWN + LDS_PID + LDS_TRIAL + LDS_NEW + LDS_NB + LDS_ACC;
+
+// Trial geometry shared by the helpers of one team.
+struct Trial {
+	double px, py, pz, nx, ny, nz;
+	double Lx, Ly, Lz, iLx, iLy, iLz, rc2, cosmax;
+	double c2;  // cosmax^2 for the filtered angular tests, -1: exact tests only
+};
+
+__device__ __forceinline__ bool overlaps_new(const Trial &t, double qx, double qy, double qz) {
+	double dx = kf_min_image(qx - t.nx, t.Lx, t.iLx);
+	double dy = kf_min_image(qy - t.ny, t.Ly, t.iLy);
+	double dz = kf_min_image(qz - t.nz, t.Lz, t.iLz);
+	return (dx * dx + dy * dy + dz * dz) < 1.;
+}
+
+// Bonds of neighbour q with the old and the new position (one load of q for
+// both: the particle stays in its cell, so the neighbourhood is the same),
+// with the reference's arithmetic (MC.c:242-262).  q's orientation comes from
+// load_o only when one of the two passes the p-side test.
+template <class LoadO>
+__device__ __forceinline__ void bonds_old_new_exact(const Trial &t, const double *Po, const double *Pn, double qx,
+                                                    double qy, double qz, LoadO load_o, int &bo, int &bn) {
+	double ox = kf_min_image(qx - t.px, t.Lx, t.iLx);
+	double oy = kf_min_image(qy - t.py, t.Ly, t.iLy);
+	double oz = kf_min_image(qz - t.pz, t.Lz, t.iLz);
+	double o2 = ox * ox + oy * oy + oz * oz;
+	double wx = kf_min_image(qx - t.nx, t.Lx, t.iLx);
+	double wy = kf_min_image(qy - t.ny, t.Ly, t.iLy);
+	double wz = kf_min_image(qz - t.nz, t.Lz, t.iLz);
+	double w2 = wx * wx + wy * wy + wz * wz;
+	bool po = false, pn = false;
+	if(o2 < t.rc2) {
+		kf_unit(ox, oy, oz, o2);
+		po = kf_any_p(ox, oy, oz, Po, t.cosmax);
+	}
+	if(w2 < t.rc2) {
+		kf_unit(wx, wy, wz, w2);
+		pn = kf_any_p(wx, wy, wz, Pn, t.cosmax);
+	}
+	if(po || pn) {
+		double Oq[9];
+		load_o(Oq);
+		bool ho, hn;
+		kf_any_q2(ox, oy, oz, po, wx, wy, wz, pn, Oq, t.cosmax, ho, hn);
+		bo += ho;
+		bn += hn;
+	}
+}
+
+// The same decisions through the filtered tests (no square root, no
+// divisions, no branch per patch).  A pair inside a filter band sets `band`;
+// the caller then redoes that lane's neighbours with bonds_old_new_exact.
+template <class LoadO>
+__device__ __forceinline__ void bonds_old_new(const Trial &t, const double *Po, const double *Pn, double qx, double qy,
+                                              double qz, LoadO load_o, int &bo, int &bn, bool &band) {
+	const double ox = kf_min_image(qx - t.px, t.Lx, t.iLx);
+	const double oy = kf_min_image(qy - t.py, t.Ly, t.iLy);
+	const double oz = kf_min_image(qz - t.pz, t.Lz, t.iLz);
+	const double o2 = ox * ox + oy * oy + oz * oz;
+	const double wx = kf_min_image(qx - t.nx, t.Lx, t.iLx);
+	const double wy = kf_min_image(qy - t.ny, t.Ly, t.iLy);
+	const double wz = kf_min_image(qz - t.nz, t.Lz, t.iLz);
+	const double w2 = wx * wx + wy * wy + wz * wz;
+	const double oc = t.c2 * o2, oe = KF_EPS * o2, nc = t.c2 * w2, ne = KF_EPS * w2;
+	const unsigned fo = o2 < t.rc2 ? kf_fast_p(ox, oy, oz, Po, oc, oe) : 0u;
+	const unsigned fn = w2 < t.rc2 ? kf_fast_p(wx, wy, wz, Pn, nc, ne) : 0u;
+	unsigned unsure = ((fo | fn) >> 1) & 1u;
+	if((fo | fn) & 1u) {
+		double Oq[9];
+		load_o(Oq);
+		unsigned qo, qn;
+		kf_fast_q2(ox, oy, oz, oc, oe, wx, wy, wz, nc, ne, Oq, qo, qn);
+		qo = (fo & 1u) ? qo : 0u;
+		qn = (fn & 1u) ? qn : 0u;
+		unsure |= ((qo | qn) >> 1) & 1u;
+		bo += (int) (qo & 1u);
+		bn += (int) (qn & 1u);
+	}
+	band |= unsure != 0u;
+}
+
+// One cell of a colour pass: entry gi of the pass's list of non-empty cells
+// (all replicas), worked on by the T lanes of one team with its LDS block
+// `own`.  The cell's members
+// make cnt sequential trials.  Each lane keeps up to CPL of the cell's static
+// neighbours (the 26 surrounding cells, fixed during the pass) in registers;
+// a denser neighbourhood is handled T at a time from HBM.
+template <int T, int CPL>
+__device__ __forceinline__ void pass_cell(const DevState &d, int pass, int gi, int lane, double *own) {
+	int *own_pid = (int *) (own + LDS_OWN);
+	double *trials = own + LDS_OWN + LDS_PID;
+	double *nw = trials + LDS_TRIAL;       // O'[9] at 0, P'[12] at 12
+	int *nb_tab = (int *) (nw + LDS_NEW);  // the cell's record (build_record)
+	double *acc_tab = nw + LDS_NEW + LDS_NB;
+	const int S = d.stride;
+	{
+		// 1. the cell's record: surrounding slot ranges, own range, replica
+		const int4 *rec = (const int4 *) (d.rec + ((size_t) pass * d.total_act + gi) * REC_W);
+		for(int e = lane; e < REC_W / 4; e += T) ((int4 *) nb_tab)[e] = rec[e];
+		team_sync();
+		const int M = uni<T>(nb_tab[58]);
+		const int s0 = uni<T>(nb_tab[59]);
+		const int cnt = uni<T>(nb_tab[60]);
+		const int cloc = uni<T>(nb_tab[61]);
+		const int r = uni<T>(nb_tab[62]);
+		const int sel = uni<T>(nb_tab[63]);
+		if(cnt > CELL_CAP) {
+			if(lane == 0) atomicOr(d.err, (int) ERR_CELL_CAP);
+			return;
 		}
-	}
-	team_sync();
+		const Rep &rp = d.reps[r];
+		double *F = d.f[sel];
+		int *PID = d.pid[sel];
 
-	Trial tr;
-	tr.Lx = unid<T>(rp.g.box[0]);
-	tr.Ly = unid<T>(rp.g.box[1]);
-	tr.Lz = unid<T>(rp.g.box[2]);
-	tr.iLx = unid<T>(rp.g.inv_box[0]);
-	tr.iLy = unid<T>(rp.g.inv_box[1]);
-	tr.iLz = unid<T>(rp.g.inv_box[2]);
-	tr.rc2 = unid<T>(rp.rc2);
-	tr.cosmax = unid<T>(rp.cosmax);
-	const double disp_max = unid<T>(rp.disp_max), theta_max = unid<T>(rp.theta_max), temp = unid<T>(rp.T);
-	const double sh0 = unid<T>(rp.g.shift[0]), sh1 = unid<T>(rp.g.shift[1]), sh2 = unid<T>(rp.g.shift[2]);
-	int n_try = 0, n_acc = 0, d_bonds = 0;
-
-	for(int t = 0; t < cnt; t++) {
-		u32x4 ra = uni4<T>(rng(d.seed, r, a.sweep, (uint32_t) cloc, TAG_TRIAL_A, (uint32_t) col, (uint32_t) t));
-		const int k = uni<T>((int) __umulhi(ra.x, (uint32_t) cnt));
-		const double *me = own + k * NCOMP;
-		tr.px = unid<T>(me[0]);
-		tr.py = unid<T>(me[1]);
-		tr.pz = unid<T>(me[2]);
-		n_try++;
-		// displacement uniform in +-disp_max/2 (MC.c:317-319)
-		tr.nx = unid<T>(tr.px + (u01(ra.y) - 0.5) * disp_max);
-		tr.ny = unid<T>(tr.py + (u01(ra.z) - 0.5) * disp_max);
-		tr.nz = unid<T>(tr.pz + (u01(ra.w) - 0.5) * disp_max);
-		// a trial that leaves the cell is rejected
-		if(uni<T>(cell_coord(tr.nx, sh0, tr.iLx, n0)) != ix || uni<T>(cell_coord(tr.ny, sh1, tr.iLy, n1)) != iy ||
-		   uni<T>(cell_coord(tr.nz, sh2, tr.iLz, n2)) != iz)
-			continue;
-
-		// phase 1: hard cores at the new position, wavefront-ballot early exit
-		bool ov = false;
-		for(int m = lane; m < cnt && !ov; m += T)
-			if(m != k) ov = overlaps_new(tr, own[m * NCOMP], own[m * NCOMP + 1], own[m * NCOMP + 2]);
+		// 2. every load of the cell at once: the members, the cell's trials
+		// (written by the rebuild), the static neighbours' positions and the
+		// replica's geometry
+		int my_pid = 0;
+		double mine[NCOMP];
+		if(lane < cnt) {
+			my_pid = PID[s0 + lane];
 #pragma unroll
-		for(int u = 0; u < NB::PER; u++)
-			for(int m = lane % NB::S; m < nb_n[u] && !ov; m += NB::S) {
-				const int q = nb_s[u] + m;
-				ov = overlaps_new(tr, F[q], F[S + q], F[2 * S + q]);
+			for(int c = 0; c < NCOMP; c++) mine[c] = F[c * S + s0 + lane];
+		}
+		const double *tsrc = d.trials + (long long) s0 * TRIAL_W;
+		const int ntr = cnt * TRIAL_W;
+		double tv0 = 0., tv1 = 0.;
+		if(lane < ntr) tv0 = tsrc[lane];
+		if(lane + T < ntr) tv1 = tsrc[lane + T];
+		// slot of static neighbour i (0 <= i < M)
+		auto nb_slot = [&](int i) {
+			int lo2 = 0, hi2 = 26;
+			while(hi2 - lo2 > 1) {
+				int mid = (lo2 + hi2) >> 1;
+				if(nb_tab[32 + mid] <= i) lo2 = mid;
+				else hi2 = mid;
 			}
-		if(team_any<T>(ov)) continue;
+			return nb_tab[lo2] + (i - nb_tab[32 + lo2]);
+		};
+		// the first T * CPL static neighbours' positions into registers; the
+		// rest (dense neighbourhoods only) are read again at every trial
+		double cxr[CPL], cyr[CPL], czr[CPL], cor[CPL][9];
+		int cqr[CPL];
+#pragma unroll
+		for(int u = 0; u < CPL; u++) {
+			const int i = lane + u * T;
+			cqr[u] = -1;
+			cxr[u] = cyr[u] = czr[u] = 0.;
+#pragma unroll
+			for(int c = 0; c < 9; c++) cor[u][c] = 0.;
+			if(i < M) {
+				const int q = nb_slot(i);
+				cqr[u] = q;
+				cxr[u] = F[q];
+				cyr[u] = F[S + q];
+				czr[u] = F[2 * S + q];
+#pragma unroll
+				for(int c = 0; c < 9; c++) cor[u][c] = F[(3 + c) * S + q];
+			}
+		}
+		const double acc_v = lane < ACC_TAB ? rp.acc_tab[lane] : 0.;
+		Trial tr;
+		tr.Lx = unid<T>(rp.g.box[0]);
+		tr.Ly = unid<T>(rp.g.box[1]);
+		tr.Lz = unid<T>(rp.g.box[2]);
+		tr.iLx = unid<T>(rp.g.inv_box[0]);
+		tr.iLy = unid<T>(rp.g.inv_box[1]);
+		tr.iLz = unid<T>(rp.g.inv_box[2]);
+		tr.rc2 = unid<T>(rp.rc2);
+		tr.cosmax = unid<T>(rp.cosmax);
+		tr.c2 = tr.cosmax >= KF_FILTER_MIN_COS ? tr.cosmax * tr.cosmax : -1.;
+		const double temp = unid<T>(rp.T);
+		const double sh0 = rp.g.shift[0], sh1 = rp.g.shift[1], sh2 = rp.g.shift[2];
+		const int n0 = uni<T>(rp.g.n[0]), n1 = uni<T>(rp.g.n[1]), n2 = uni<T>(rp.g.n[2]);
+		const int ix = cloc / (n1 * n2), iy = (cloc / n2) % n1, iz = cloc % n2;
 
-		// rotation: angle uniform in [0, theta_max), axis uniform on the sphere,
-		// applied in the body frame O' = O R (MC.c:321-325); lane 0 stages the
-		// old and new patch vectors and O' in LDS for the whole team
-		u32x4 rb = uni4<T>(rng(d.seed, r, a.sweep, (uint32_t) cloc, TAG_TRIAL_B, (uint32_t) col, (uint32_t) t));
+		// 3. trials and particle indices into LDS
+		if(lane < ntr) trials[lane] = tv0;
+		if(lane + T < ntr) trials[lane + T] = tv1;
+		for(int e = lane + 2 * T; e < ntr; e += T) trials[e] = tsrc[e];
+		if(lane < cnt) own_pid[lane] = my_pid;
+		if(lane < ACC_TAB) acc_tab[lane] = acc_v;
+		team_sync();
+		// 4. members into LDS ranked by particle index, with their patch vectors
+		if(lane < cnt) {
+			int rank = 0;
+			for(int m = 0; m < cnt; m++) rank += own_pid[m] < my_pid;
+			double *w = own + rank * OWN_W;
+#pragma unroll
+			for(int c = 0; c < NCOMP; c++) w[c] = mine[c];
+			double P[4][3];
+			kf_patches(mine + 3, P);
+#pragma unroll
+			for(int i = 0; i < 4; i++)
+#pragma unroll
+				for(int c = 0; c < 3; c++) w[NCOMP + 3 * i + c] = P[i][c];
+		}
+		team_sync();
+		int c_acc = 0, d_bonds = 0;
+
+		for(int t = 0; t < cnt; t++) {
+			const double *tp = trials + t * TRIAL_W;
+			const int k = uni<T>((int) tp[0]);
+			const double *me = own + k * OWN_W;
+			tr.px = unid<T>(me[0]);
+			tr.py = unid<T>(me[1]);
+			tr.pz = unid<T>(me[2]);
+			tr.nx = unid<T>(tr.px + tp[1]);
+			tr.ny = unid<T>(tr.py + tp[2]);
+			tr.nz = unid<T>(tr.pz + tp[3]);
+			// a trial that leaves the cell is rejected
+			if(uni<T>(cell_coord(tr.nx, sh0, tr.iLx, n0)) != ix || uni<T>(cell_coord(tr.ny, sh1, tr.iLy, n1)) != iy ||
+			   uni<T>(cell_coord(tr.nz, sh2, tr.iLz, n2)) != iz)
+				continue;
+
+			// phase 1: hard cores at the new position, wavefront-ballot early exit
+			bool ov = false;
+#pragma unroll
+			for(int u = 0; u < CPL; u++)
+				if(cqr[u] >= 0) ov |= overlaps_new(tr, cxr[u], cyr[u], czr[u]);
+			if(lane < cnt && lane != k)
+				ov |= overlaps_new(tr, own[lane * OWN_W], own[lane * OWN_W + 1], own[lane * OWN_W + 2]);
+			for(int b = T * CPL; b < M; b += T)
+				if(b + lane < M) {
+					const int q = nb_slot(b + lane);
+					ov |= overlaps_new(tr, F[q], F[S + q], F[2 * S + q]);
+				}
+			if(team_any<T>(ov)) continue;
+
+			// O' = O R in the body frame (utils.c:172-184, 204-209): lane 3i+j
+			// forms element (i, j); lane 12+3a+k forms row k of O' again and from
+			// it component k of patch a of O' (utils.h:17-21)
+			if(lane < 9) {
+				const int i = lane / 3, j = lane % 3;
+				nw[lane] = me[3 + 3 * i] * tp[4 + j] + me[3 + 3 * i + 1] * tp[4 + 3 + j] + me[3 + 3 * i + 2] * tp[4 + 6 + j];
+			}
+			else if(lane >= 12 && lane < 24) {
+				const int pa = (lane - 12) / 3, kk = (lane - 12) % 3;
+				const double *o = me + 3 + 3 * kk;
+				double row[3];
+#pragma unroll
+				for(int j = 0; j < 3; j++) row[j] = o[0] * tp[4 + j] + o[1] * tp[4 + 3 + j] + o[2] * tp[4 + 6 + j];
+				nw[lane] = row[0] * kf_base(pa, 0) + row[1] * kf_base(pa, 1) + row[2] * kf_base(pa, 2);
+			}
+			team_sync();
+
+			// phase 2: bonds at the old and the new position, over this lane's
+			// neighbours: the static ones in registers, one member of the cell,
+			// and (dense neighbourhoods) the rest from HBM
+			const double *Po = me + NCOMP, *Pn = nw + 12;
+			auto each_neighbour = [&](auto &&f) {
+#pragma unroll
+				for(int u = 0; u < CPL; u++)
+					if(cqr[u] >= 0)
+						f(cxr[u], cyr[u], czr[u], [&](double *Oq) {
+#pragma unroll
+							for(int c = 0; c < 9; c++) Oq[c] = cor[u][c];
+						});
+				if(lane < cnt && lane != k) {
+					const double *w = own + lane * OWN_W;
+					f(w[0], w[1], w[2], [&](double *Oq) {
+#pragma unroll
+						for(int c = 0; c < 9; c++) Oq[c] = w[3 + c];
+					});
+				}
+				for(int b = T * CPL; b < M; b += T)
+					if(b + lane < M) {
+						const int q = nb_slot(b + lane);
+						f(F[q], F[S + q], F[2 * S + q], [&](double *Oq) {
+#pragma unroll
+							for(int c = 0; c < 9; c++) Oq[c] = F[(3 + c) * S + q];
+						});
+					}
+			};
+			int bo = 0, bn = 0;
+			bool band = tr.c2 < 0.;  // no filter: every lane takes the exact form
+			if(!band)
+				each_neighbour([&](double qx, double qy, double qz, auto load_o) {
+					bonds_old_new(tr, Po, Pn, qx, qy, qz, load_o, bo, bn, band);
+				});
+			if(team_any<T>(band) && band) {
+				bo = 0;
+				bn = 0;
+				each_neighbour([&](double qx, double qy, double qz, auto load_o) {
+					bonds_old_new_exact(tr, Po, Pn, qx, qy, qz, load_o, bo, bn);
+				});
+			}
+			bo = team_sum_small<T>(bo);
+			bn = team_sum_small<T>(bn);
+			const int db = bn - bo;  // E = -bonds, dE = -db
+			const double dE = (double) (-db);
+			// MC.c:332: accept if dE < 0 or a uniform draw < exp(-dE/T); dE is
+			// an integer, exp(-dE/T) comes from the replica's table
+			const int idE = -db;
+			const double w = idE <= 0 ? 1. : (idE < ACC_TAB ? acc_tab[idE] : exp(-dE / temp));
+			const bool acc = (dE < 0.) || (tp[13] < w);
+			if(acc) {
+				c_acc++;
+				d_bonds += db;
+				// member k takes the new position, orientation and patches
+				double *w = own + k * OWN_W;
+				if(lane < 3) w[lane] = lane == 0 ? tr.nx : (lane == 1 ? tr.ny : tr.nz);
+				else if(lane < 12) w[lane] = nw[lane - 3];
+				else if(lane < 24) w[lane] = nw[lane];
+			}
+			team_sync();
+		}
+		// 5. write the members back, ranked (the ranking permutes the cell's slots)
+		if(c_acc > 0 || cnt > 1) {
+			for(int e = lane; e < cnt * NCOMP; e += T) {
+				int m = e / NCOMP, c = e - m * NCOMP;
+				F[c * S + s0 + m] = own[m * OWN_W + c];
+			}
+			if(lane < cnt) {
+				int rank = 0;
+				for(int m = 0; m < cnt; m++) rank += own_pid[m] < my_pid;
+				PID[s0 + rank] = my_pid;
+			}
+		}
 		if(lane == 0) {
-			double O[9], On[9], P[4][3], axis[3];
-#pragma unroll
-			for(int c = 0; c < 9; c++) O[c] = me[3 + c];
-			const double theta = u01(rb.x) * theta_max;
-			sphere_axis(d.seed, r, a.sweep, (uint32_t) cloc, (uint32_t) col, (uint32_t) t, axis);
-			double st, ct;
-			sincos(theta, &st, &ct);
-			rotate_body(O, axis, st, ct, On);
-			kf_patches(O, d.base, P);
-#pragma unroll
-			for(int i = 0; i < 4; i++)
-#pragma unroll
-				for(int c = 0; c < 3; c++) Ps[3 * i + c] = P[i][c];
-			kf_patches(On, d.base, P);
-#pragma unroll
-			for(int i = 0; i < 4; i++)
-#pragma unroll
-				for(int c = 0; c < 3; c++) Ps[12 + 3 * i + c] = P[i][c];
-#pragma unroll
-			for(int c = 0; c < 9; c++) Ps[24 + c] = On[c];
-		}
-		team_sync();
-
-		// phase 2: bonds at the old and the new position with one load of each
-		// neighbour (the particle stays in its cell: same neighbourhood)
-		int bo = 0, bn = 0;
-		for(int m = lane; m < cnt; m += T)
-			if(m != k) bonds_old_new<true>(tr, Ps, d.base, own, 0, m, own[m * NCOMP], own[m * NCOMP + 1],
-			                               own[m * NCOMP + 2], bo, bn);
-#pragma unroll
-		for(int u = 0; u < NB::PER; u++)
-			for(int m = lane % NB::S; m < nb_n[u]; m += NB::S) {
-				const int q = nb_s[u] + m;
-				bonds_old_new<false>(tr, Ps, d.base, F, S, q, F[q], F[S + q], F[2 * S + q], bo, bn);
+			const int slot = gi & (NSLOT - 1);
+			atomicAdd(ctr_at(d.ctr, r, C_TRY_ROTO, slot), (unsigned long long) cnt);
+			if(c_acc) {
+				atomicAdd(ctr_at(d.ctr, r, C_ACC_ROTO, slot), (unsigned long long) c_acc);
+				atomicAdd(ctr_at(d.ctr, r, C_DBONDS, slot), (unsigned long long) (long long) d_bonds);
 			}
-		bo = team_sum<T>(bo);
-		bn = team_sum<T>(bn);
-		const int db = bn - bo;  // E = -bonds, dE = -db
-		const double dE = (double) (-db);
-		const bool acc = (dE < 0.) || (u01(rb.w) < exp(-dE / temp));  // MC.c:332
-		if(acc) {
-			n_acc++;
-			d_bonds += db;
-			if(lane == 0) {
-				double *w = own + k * NCOMP;
-				w[0] = tr.nx;
-				w[1] = tr.ny;
-				w[2] = tr.nz;
-#pragma unroll
-				for(int c = 0; c < 9; c++) w[3 + c] = Ps[24 + c];
-			}
-		}
-		team_sync();
-	}
-	if(n_acc > 0) {
-		for(int e = lane; e < cnt * NCOMP; e += T) {
-			int m = e / NCOMP, c = e - m * NCOMP;
-			F[c * S + s0 + m] = own[e];
-		}
-	}
-	if(lane == 0) {
-		int slot = g & (NSLOT - 1);
-		atomicAdd(ctr_at(d.ctr, r, C_TRY_ROTO, slot), (unsigned long long) n_try);
-		if(n_acc) {
-			atomicAdd(ctr_at(d.ctr, r, C_ACC_ROTO, slot), (unsigned long long) n_acc);
-			atomicAdd(ctr_at(d.ctr, r, C_DBONDS, slot), (unsigned long long) (long long) d_bonds);
 		}
 	}
 }
 
-template <int T>
+// One colour pass per launch: team g of the grid takes list entry g; teams
+// past the end of the list leave at once.
+template <int T, int CPL>
+__global__ __launch_bounds__(256) void k_pass(PassArgs a) {
+	extern __shared__ double lds[];
+	const int lane = threadIdx.x & (T - 1);
+	const int team = threadIdx.x / T;
+	const int gi = uni<T>(blockIdx.x * (blockDim.x / T) + team);
+	if(gi >= a.d.pfx[a.pass * (a.d.R + 1) + a.d.R]) return;
+	pass_cell<T, CPL>(a.d, a.pass, gi, lane, lds + team * TEAM_LDS);
+}
+
+// Grid-wide barrier of a cooperative launch (every workgroup resident).
+// Arrivals count up in bar[0]; the last one resets it and advances the
+// generation bar[1], which the others wait for.  The agent-scope fences
+// publish each workgroup's writes before its arrival and make the others'
+// visible after the wait.  The wait is bounded: past BAR_SPIN_MAX polls the
+// workgroup reports ERR_BARRIER and goes on, so the grid always drains.
+__device__ __forceinline__ void grid_barrier(unsigned *bar, int *err) {
+	__builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+	__syncthreads();
+	if(threadIdx.x == 0) {
+		const unsigned gen = __hip_atomic_load(&bar[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+		const unsigned arrived = __hip_atomic_fetch_add(&bar[0], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+		if(arrived == gridDim.x - 1) {
+			__hip_atomic_store(&bar[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+			__hip_atomic_store(&bar[1], gen + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+		}
+		else {
+			int spins = 0;
+			while(__hip_atomic_load(&bar[1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == gen) {
+				if(++spins > BAR_SPIN_MAX) {
+					atomicOr(err, (int) ERR_BARRIER);
+					break;
+				}
+				__builtin_amdgcn_s_sleep(2);
+			}
+		}
+	}
+	__syncthreads();
+	__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+}
+
+// The 8 colour passes of a sweep in one cooperative launch, separated by
+// grid barriers.  Team g takes list entries g, g + teams, ... of each pass.
+template <int T, int CPL>
+__global__ __launch_bounds__(256) void k_sweep_passes(PassArgs a) {
+	extern __shared__ double lds[];
+	const DevState &d = a.d;
+	const int lane = threadIdx.x & (T - 1);
+	const int team = threadIdx.x / T;
+	const int tpb = blockDim.x / T;
+	const int nteams = gridDim.x * tpb;
+	const int g0 = uni<T>(blockIdx.x * tpb + team);
+	double *own = lds + team * TEAM_LDS;
+	for(int k = 0; k < 8; k++) {
+		const int total = uni<T>(d.pfx[k * (d.R + 1) + d.R]);
+		for(int gi = g0; gi < total; gi += nteams) pass_cell<T, CPL>(d, k, gi, lane, own);
+		if(k < 7) grid_barrier(d.bar, d.err);
+	}
+}
+
+template <int T, int CPL>
 static void pass_T(const DevState &d, uint64_t sweep, int pass, int block, hipStream_t s) {
-	int tpb = block / T;
+	const int tpb = block / T;
 	int nb = (d.total_act + tpb - 1) / tpb;
-	nb = (nb + 7) & ~7;
+	if(nb < 1) nb = 1;
 	size_t lds = (size_t) tpb * TEAM_LDS * sizeof(double);
 	PassArgs a{d, sweep, pass};
-	hipLaunchKernelGGL(k_pass<T>, dim3(nb), dim3(block), lds, s, a);
+	hipLaunchKernelGGL((k_pass<T, CPL>), dim3(nb), dim3(block), lds, s, a);
 }
 
 size_t pass_lds_bytes(int T, int block) { return (size_t) (block / T) * TEAM_LDS * sizeof(double); }
 
 void launch_pass(const DevState &d, uint64_t sweep, int pass, int T, int block, hipStream_t s) {
-	switch(T) {
-	case 8: pass_T<8>(d, sweep, pass, block, s); break;
-	case 16: pass_T<16>(d, sweep, pass, block, s); break;
-	case 64: pass_T<64>(d, sweep, pass, block, s); break;
-	default: pass_T<32>(d, sweep, pass, block, s); break;
-	}
+	if(T == 32) pass_T<32, 1>(d, sweep, pass, block, s);
+	else pass_T<64, 1>(d, sweep, pass, block, s);
+}
+
+template <int T, int CPL>
+static int sweep_blocks_T(int block, int cus) {
+	int per_cu = 0;
+	if(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *) k_sweep_passes<T, CPL>, block,
+	                                                pass_lds_bytes(T, block)) != hipSuccess)
+		return 0;
+	return per_cu * cus;
+}
+
+int sweep_blocks_max(int T, int block, int cus) {
+	return T == 32 ? sweep_blocks_T<32, 1>(block, cus) : sweep_blocks_T<64, 1>(block, cus);
+}
+
+template <int T, int CPL>
+static hipError_t sweep_T(const DevState &d, uint64_t sweep, int block, int blocks, hipStream_t s) {
+	PassArgs a{d, sweep, 0};
+	void *args[] = {&a};
+	return hipLaunchCooperativeKernel((const void *) k_sweep_passes<T, CPL>, dim3(blocks), dim3(block), args,
+	                                  (unsigned) pass_lds_bytes(T, block), s);
+}
+
+hipError_t launch_sweep_passes(const DevState &d, uint64_t sweep, int T, int block, int blocks, hipStream_t s) {
+	return T == 32 ? sweep_T<32, 1>(d, sweep, block, blocks, s) : sweep_T<64, 1>(d, sweep, block, blocks, s);
 }
 
 // ------------------------------------------------------------------------
 // energy: one team per slot, MC_energy semantics (MC.c:272-309): bonds of
 // the particle with every neighbour, 0 and an overlap flag on any overlap.
+
+template <int T>
+struct NbLayout {
+	static constexpr int S = (T >= 54) ? 2 : 1;
+	static constexpr int STEP = T / S;
+	static constexpr int PER = (27 + STEP - 1) / STEP;
+};
 
 template <int T>
 __global__ __launch_bounds__(256) void k_energy(DevState d, int trial, int per_particle) {
@@ -636,7 +999,7 @@ __global__ __launch_bounds__(256) void k_energy(DevState d, int trial, int per_p
 #pragma unroll
 	for(int c = 0; c < 9; c++) O[c] = F[(3 + c) * S + j];
 	double P[4][3];
-	kf_patches(O, d.base, P);
+	kf_patches(O, P);
 	const int n0 = G.n[0], n1 = G.n[1], n2 = G.n[2];
 	const int ix = cell_coord(px, G.shift[0], G.inv_box[0], n0);
 	const int iy = cell_coord(py, G.shift[1], G.inv_box[1], n1);
@@ -665,7 +1028,7 @@ __global__ __launch_bounds__(256) void k_energy(DevState d, int trial, int per_p
 					double Oq[9];
 #pragma unroll
 					for(int c = 0; c < 9; c++) Oq[c] = F[(3 + c) * S + q];
-					if(kf_any_q(dx, dy, dz, Oq, d.base, rp.cosmax)) nbond++;
+					if(kf_any_q(dx, dy, dz, Oq, rp.cosmax)) nbond++;
 				}
 			}
 		}
@@ -685,27 +1048,11 @@ __global__ __launch_bounds__(256) void k_energy(DevState d, int trial, int per_p
 	}
 }
 
-void launch_energy(const DevState &d, int trial, int per_particle, int T, hipStream_t s) {
+void launch_energy(const DevState &d, int trial, int per_particle, hipStream_t s) {
 	hipLaunchKernelGGL(k_zero_energy_rows, dim3((d.R * 2 * NSLOT + 255) / 256), dim3(256), 0, s, d.ctr, d.R);
 	const int block = 256;
-	switch(T) {
-	case 8:
-		hipLaunchKernelGGL(k_energy<8>, dim3((d.Ntot + block / 8 - 1) / (block / 8)), dim3(block), 0, s, d, trial,
-		                   per_particle);
-		break;
-	case 16:
-		hipLaunchKernelGGL(k_energy<16>, dim3((d.Ntot + block / 16 - 1) / (block / 16)), dim3(block), 0, s, d,
-		                   trial, per_particle);
-		break;
-	case 64:
-		hipLaunchKernelGGL(k_energy<64>, dim3((d.Ntot + block / 64 - 1) / (block / 64)), dim3(block), 0, s, d,
-		                   trial, per_particle);
-		break;
-	default:
-		hipLaunchKernelGGL(k_energy<32>, dim3((d.Ntot + block / 32 - 1) / (block / 32)), dim3(block), 0, s, d,
-		                   trial, per_particle);
-		break;
-	}
+	hipLaunchKernelGGL(k_energy<32>, dim3((d.Ntot + block / 32 - 1) / (block / 32)), dim3(block), 0, s, d, trial,
+	                   per_particle);
 }
 
 // ------------------------------------------------------------------------

@@ -93,18 +93,25 @@ def test_check_energy_gram_schmidt_bit_exact(pp, oracle):
     assert np.array_equal(O2, want)
 
 
-@pytest.mark.parametrize("tpc,block", [(8, 64), (16, 128), (32, 256), (64, 256), (64, 128)])
-def test_trajectory_independent_of_launch_shape(pp, oracle, tpc, block):
+@pytest.mark.parametrize("tpc,block,launch", [(32, 64, 1), (32, 128, 2), (32, 256, 2), (64, 64, 2), (64, 128, 1),
+                                              (64, 256, 2)])
+def test_trajectory_independent_of_launch_shape(pp, oracle, tpc, block, launch):
+    """Teams per cell, workgroup size and launch form (one launch per colour
+    pass, or one cooperative launch per sweep with grid barriers) leave the
+    trajectory unchanged bit for bit."""
     r, O, box = _start(oracle, 3000, 0.4, 9)
     ref = None
-    with pp.System(r, O, box, T=0.11, seed=99, threads_per_cell=64, block_size=256) as s:
+    with pp.System(r, O, box, T=0.11, seed=99, threads_per_cell=64, block_size=256, launch=1) as s:
         s.do_ensemble(25)
         ref = s.state()
-    with pp.System(r, O, box, T=0.11, seed=99, threads_per_cell=tpc, block_size=block) as s:
+        cref = s.counters()
+    with pp.System(r, O, box, T=0.11, seed=99, threads_per_cell=tpc, block_size=block, launch=launch) as s:
         s.do_ensemble(25)
         got = s.state()
+        c = s.counters()
     for a, b in zip(ref, got):
         assert np.array_equal(a, b)
+    assert c == cref
 
 
 def test_upload_download_roundtrip(pp, oracle):
@@ -172,3 +179,40 @@ def test_two_particle_bonding_probability_gpu(pp):
     per_rep = np.array(samples).mean(axis=0)  # independent replicas
     m, se = per_rep.mean(), per_rep.std(ddof=1) / np.sqrt(R)
     assert abs(m - p_exact) < 4 * se + 2e-3, (m, se, p_exact)
+
+
+def _fcc(cells, a, seed):
+    """FCC lattice of cells^3 unit cells (lattice constant a), random rotations."""
+    base = np.array([[0, 0, 0], [0.5, 0.5, 0], [0.5, 0, 0.5], [0, 0.5, 0.5]])
+    g = np.stack(np.meshgrid(*[np.arange(cells)] * 3, indexing="ij"), -1).reshape(-1, 1, 3)
+    r = ((g + base[None]) * a).reshape(-1, 3) + 0.25 * a
+    rng = np.random.default_rng(seed)
+    q, rr = np.linalg.qr(rng.normal(size=(len(r), 3, 3)))
+    q = q * np.sign(np.diagonal(rr, axis1=1, axis2=2))[:, None, :]
+    q[np.linalg.det(q) < 0, :, 0] *= -1
+    return r, q, np.full(3, cells * a)
+
+
+@pytest.mark.parametrize("tpc", [32, 64])
+def test_dense_neighbourhood_beyond_registers(pp, oracle, tpc):
+    """FCC at rho=0.89 in a box of 4^3 sweep cells of side 1.65: the 26 cells
+    around a sweep cell hold ~104 particles, more than one team keeps in
+    registers (64), so the colour pass reads the rest from memory.  Overlaps
+    and bonds must still be seen: the run stays overlap-free, the tracked
+    energy equals a full recompute, and the bonds equal the oracle's."""
+    r, O, box = _fcc(4, 1.65, 4)
+    with pp.System(r, O, box, T=0.12, seed=21, threads_per_cell=tpc) as s:
+        assert s.grid()[0] == [4, 4, 4]
+        s.do_ensemble(40)
+        rr, OO, bb = s.state()
+        b, ov, tot, anyo = s.energy()
+        c = s.counters()
+    ob, oov, otot = oracle.energy(rr, OO, bb)
+    assert not anyo and not oov.any()
+    assert np.array_equal(b, ob) and c["bonds"] == tot == otot
+    assert c["accepted"][0] > 0.2 * c["tries"][0]
+    with pp.System(r, O, box, T=0.12, seed=21, threads_per_cell=96 - tpc) as s:
+        s.do_ensemble(40)
+        other = s.state()
+    for x, y in zip((rr, OO, bb), other):
+        assert np.array_equal(x, y)
