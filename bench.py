@@ -77,6 +77,18 @@ def cpu_baseline(r, O, box, seconds):
                 sample=f"{j['sweeps']} sweeps of the N={N} rho={RHO} T={T} start with oracle/cpu_bench")
 
 
+def measured_traffic(path=os.path.join(ROOT, "profiles", "k_pass_n32768_rho0.5_pmc.txt")):
+    """HBM bytes per colour-pass launch from the committed rocprofv3 PMC
+    summary (FETCH_SIZE + WRITE_SIZE, scaled by a plain copy), or None."""
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -88,7 +100,22 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-timing", action="store_true", help="no HIP events in the timed region")
+    ap.add_argument("--time-every", type=int, default=16,
+                    help="HIP events bracket the kernels of every k-th sweep of the timed region")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only copy a MIB-sized device buffer 5 times (the reference point that scales "
+                         "the FETCH_SIZE / WRITE_SIZE counters of a profile)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        import torch
+        n = args.calibrate_copy * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.int32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_per_dispatch": 4 * n}))
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: this build has no multi-GPU path (asked for {args.gpus}); "
               "independent replicas would shard whole systems across GPUs.", file=sys.stderr)
@@ -105,9 +132,10 @@ def main():
     sysm.do_ensemble(args.warmup)
     print(f"# warmup {args.warmup} sweeps done", file=sys.stderr, flush=True)
 
-    # timed region: K sweeps, HIP events around every colour-pass launch
+    # timed region: K sweeps; HIP events bracket the kernels of every
+    # --time-every-th sweep (the per-kernel averages of the roofline line)
     sysm.reset_counters()
-    sysm.set_timing(not args.no_timing)
+    sysm.set_timing(0 if args.no_timing else args.time_every)
     sysm.set_trace(args.steps)
     t0 = time.perf_counter()
     sysm.do_ensemble(args.steps)  # synchronous: returns when the device is done
@@ -124,7 +152,7 @@ def main():
     sweeps_per_s = attempts / N / elapsed
     nside, side = sysm.grid()
     bpm = algorithmic_bytes_per_move(RHO, side[0])
-    moves_per_pass = attempts / max(n_pass, 1)
+    moves_per_pass = attempts / (8 * args.steps)  # a sweep is 8 colour passes
     avg_pass_s = ms_pass / max(n_pass, 1) / 1e3
     achieved = bpm * moves_per_pass / avg_pass_s / 1e9 if avg_pass_s > 0 else 0.0
     tau = tau_int(e_series)
@@ -165,10 +193,11 @@ def main():
                                        "learned": None},
         "roofline": {
             "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+            "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": measured_traffic(),
             "kernel": "k_pass (colour pass)", "bytes_per_move": round(bpm, 1),
             "moves_per_launch": round(moves_per_pass, 1), "avg_launch_us": round(avg_pass_s * 1e6, 3),
-            "rebuild_us_per_sweep": round(ms_reb / max(args.steps, 1) * 1e3, 3),
+            "rebuild_us_per_sweep": round(ms_reb / max(n_reb, 1) * 1e3, 3),
+            "timed_sweeps": n_reb,
         },
         "cpu_baseline": cpu,
     }

@@ -82,6 +82,9 @@ typedef struct pp_options {
 	double cell_occupancy;   /* target particles per sweep cell; 0: smallest cells (side >= 1+delta) */
 	int device;              /* HIP device ordinal */
 	int launch;              /* PP_LAUNCH_*: how the 8 colour passes of a sweep are launched */
+	int trials_per_cell;     /* 0: each sweep cell makes as many trials as it holds, so a sweep is
+	                          * exactly N attempts (MC.c:21); s in [1, 16]: each non-empty cell
+	                          * makes s trials per pass, and a sweep s x (non-empty cells) attempts */
 } pp_options;
 
 /* pp_options.launch.  On MI355X a grid barrier across the 8 XCDs costs more
@@ -160,8 +163,10 @@ pp_status pp_set_trace(pp_ctx *ctx, int capacity);
 pp_status pp_get_trace(pp_ctx *ctx, int rep, long long *bonds, int capacity, int *count);
 
 /* Kernel timing with HIP events on the context's stream (off by default).
+ * every = 0: off; every = k >= 1: the launches of every k-th sweep (and every
+ * energy call) are bracketed by events.  Resets the accumulated times.
  * name: "pass", "rebuild", "energy", "volume" or NULL for the sum. */
-pp_status pp_set_timing(pp_ctx *ctx, int on);
+pp_status pp_set_timing(pp_ctx *ctx, int every);
 pp_status pp_get_timing(pp_ctx *ctx, const char *name, long long *launches, double *total_ms);
 
 /* Grid of replica rep: cells per side and cell side lengths. */

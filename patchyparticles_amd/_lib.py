@@ -55,6 +55,7 @@ class Options(C.Structure):
         ("cell_occupancy", C.c_double),
         ("device", C.c_int),
         ("launch", C.c_int),
+        ("trials_per_cell", C.c_int),
     ]
 
 

@@ -69,7 +69,8 @@ struct pp_ctx {
 	long long trace_len = -1;  // entries written to d.trace; -1 = tracing off
 	long long trace_from = 0;  // sweep index at which tracing began
 	// timing
-	bool timing = false;
+	bool timing = false;   // events around the launches being enqueued
+	int timing_every = 0;  // sweeps between timed sweeps (0: off)
 	std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> pending;  // (kind, events)
 	long long t_launch[4] = {0, 0, 0, 0};
 	double t_ms[4] = {0, 0, 0, 0};
@@ -217,12 +218,19 @@ pp_status create_impl(int R, const pp_params *prm, const pp_options *opts, const
 	pp_options o{};
 	if(opts) o = *opts;
 	if(o.threads_per_cell == 0) o.threads_per_cell = 32;
-	if(o.block_size == 0) o.block_size = 256;
+	if(o.block_size == 0) {
+		// the largest workgroup whose teams' LDS fits
+		o.block_size = 256;
+		while(o.block_size > 64 && pass_lds_bytes(o.threads_per_cell, o.block_size) > MAX_LDS_BYTES)
+			o.block_size /= 2;
+	}
 	const int Tv = o.threads_per_cell;
 	if(Tv != 32 && Tv != 64) return fail(PP_ERR_ARG, "threads_per_cell must be 32 or 64");
 	if(o.block_size % 64 || o.block_size < 64 || o.block_size > 256)
 		return fail(PP_ERR_ARG, "block_size must be 64, 128, 192 or 256");
 	if(o.launch < PP_LAUNCH_DEFAULT || o.launch > PP_LAUNCH_COOP) return fail(PP_ERR_ARG, "launch must be 0, 1 or 2");
+	if(o.trials_per_cell < 0 || o.trials_per_cell > MAX_TRIALS_PER_CELL)
+		return fail(PP_ERR_ARG, "trials_per_cell must be in [0, %d]", MAX_TRIALS_PER_CELL);
 	if(pass_lds_bytes(Tv, o.block_size) > MAX_LDS_BYTES)
 		return fail(PP_ERR_ARG, "threads_per_cell %d with block_size %d needs %zu bytes of LDS (max %zu)", Tv,
 		            o.block_size, pass_lds_bytes(Tv, o.block_size), MAX_LDS_BYTES);
@@ -335,6 +343,7 @@ pp_status create_impl(int R, const pp_params *prm, const pp_options *opts, const
 					return bail(fail(PP_ERR_HIP, "base patch (%d,%d) computed on the host differs from the kernels'", i, k));
 	}
 	d.seed = o.seed;
+	d.trials_per_cell = o.trials_per_cell;
 	if(o.launch == PP_LAUNCH_COOP) {
 		// the passes of a sweep as one cooperative launch: as many workgroups
 		// as the lists of one pass can use, at most as many as stay resident
@@ -355,7 +364,7 @@ pp_status create_impl(int R, const pp_params *prm, const pp_options *opts, const
 	   (st = dev_alloc(c, (int **) &d.act_off, R + 1)) || (st = dev_alloc(c, &d.count, d.C_total + 1)) ||
 	   (st = dev_alloc(c, &d.start, d.C_total + 1)) || (st = dev_alloc(c, &d.slot, d.stride)) ||
 	   (st = dev_alloc(c, &d.cellid, d.stride)) || (st = dev_alloc(c, &d.scan_status, std::max(d.ntiles, 1))) ||
-	   (st = dev_alloc(c, &d.scan_ticket, 2)) || (st = dev_alloc(c, &d.trials, (size_t) TRIAL_W * d.stride)) ||
+	   (st = dev_alloc(c, &d.scan_ticket, 2)) ||
 	   (st = dev_alloc(c, &d.clist, d.C_total + 1)) || (st = dev_alloc(c, &d.ccount, (size_t) R * 8)) ||
 	   (st = dev_alloc(c, &d.pfx, (size_t) 8 * (R + 1))) ||
 	   (st = dev_alloc(c, &d.rec, (size_t) 8 * std::max(d.total_act, 1) * REC_W)) ||
@@ -367,7 +376,7 @@ pp_status create_impl(int R, const pp_params *prm, const pp_options *opts, const
 		// every buffer a kernel may touch must exist before the first launch
 		const void *need[] = {d.f[0], d.f[1], d.pid[0], d.pid[1], d.reps, d.prep, d.act_off, d.count, d.start,
 		                      d.slot, d.cellid, d.scan_status, d.clist, d.ccount, d.pfx, d.rec, d.scan_ticket,
-		                      d.ctr, d.err, d.out_bonds, d.out_ovl, d.trials, d.bar};
+		                      d.ctr, d.err, d.out_bonds, d.out_ovl, d.bar};
 		for(const void *q : need)
 			if(!q) return bail(fail(PP_ERR_NOMEM, "a device buffer was not allocated"));
 	}
@@ -434,6 +443,7 @@ pp_status pp_sweeps(pp_ctx *c, long long nsw) {
 	HIPCHECK(hipSetDevice(c->device));
 	for(long long s = 0; s < nsw; s++) {
 		const uint64_t sw = (uint64_t) c->sweep;
+		c->timing = c->timing_every > 0 && c->sweep % c->timing_every == 0;
 		hipEvent_t a{};
 		tic(c, TK_REBUILD, &a);
 		long long tp = -1;
@@ -465,6 +475,7 @@ pp_status pp_sweeps(pp_ctx *c, long long nsw) {
 		c->sweep++;
 		if(c->timing && c->pending.size() > 4096) TRY(collect_timing(c));
 	}
+	c->timing = c->timing_every > 0;
 	TRY(collect_timing(c));
 	return check_err(c);
 }
@@ -716,9 +727,11 @@ pp_status pp_get_trace(pp_ctx *c, int rep, long long *bonds, int capacity, int *
 	return PP_OK;
 }
 
-pp_status pp_set_timing(pp_ctx *c, int on) {
+pp_status pp_set_timing(pp_ctx *c, int every) {
 	if(!c) return fail(PP_ERR_ARG, "ctx is NULL");
-	c->timing = on != 0;
+	if(every < 0) return fail(PP_ERR_ARG, "negative timing interval");
+	c->timing_every = every;
+	c->timing = every > 0;
 	for(int k = 0; k < 4; k++) {
 		c->t_ms[k] = 0.;
 		c->t_launch[k] = 0;

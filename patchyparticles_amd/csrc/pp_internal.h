@@ -13,6 +13,7 @@ constexpr int CELL_CAP = 24;   // particles of one sweep cell staged in LDS
 constexpr int NSLOT = 64;      // spread of per-replica atomic counters
 constexpr int NCOMP = 12;      // x, y, z, O00..O22 per particle
 constexpr int TRIAL_W = 14;    // one trial: member rank, dx, dy, dz, R[9], acceptance draw
+constexpr int MAX_TRIALS_PER_CELL = 16;
 constexpr int SCAN_TILE = 2048;
 constexpr int REC_W = 64;      // ints of one cell record of a pass
 constexpr int ACC_TAB = 16;    // Metropolis factors exp(-k/T) tabulated per replica, k < ACC_TAB
@@ -112,7 +113,7 @@ struct DevState {
 	int *err;
 	int32_t *out_bonds;
 	uint8_t *out_ovl;
-	double *trials;    // [stride][TRIAL_W] trial t of a cell lives at its slot start + t
+	int trials_per_cell;  // trials of a non-empty cell in one pass; 0: as many as it holds
 	long long *trace;  // [R][trace_cap] bonded pairs at the start of each sweep (optional)
 	int trace_cap;
 	uint64_t seed;

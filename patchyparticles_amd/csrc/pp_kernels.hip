@@ -29,6 +29,7 @@ namespace ppd {
 
 // Workgroups are dispatched round-robin over the 8 XCDs; give each XCD a
 // contiguous run of logical blocks so that neighbouring cells share an L2.
+// nb must be a multiple of 8.
 __device__ __forceinline__ int xcd_remap(int b, int nb) {
 	int per = nb >> 3;
 	return (b & 7) * per + (b >> 3);
@@ -485,20 +486,6 @@ __global__ void k_scatter(DevState d, int mode, uint64_t sweep) {
 		G[c * S + dst] = v;
 	}
 	d.pid[sb ^ 1][dst] = d.pid[sb][j];
-	if(mode == REB_SHIFT) {
-		// the worker for slot t of a cell writes that cell's trial t of this
-		// sweep (the trial depends on the cell, its colour and t only)
-		const int c = d.cellid[j];
-		const int cnt = d.start[c + 1] - d.start[c];
-		const int cloc = c - rp.cell_off;
-		const int n1 = rp.g.n[1], n2 = rp.g.n[2];
-		const int ix = cloc / (n1 * n2), iy = (cloc / n2) % n1, iz = cloc % n2;
-		const int col = ((ix & 1) << 2) | ((iy & 1) << 1) | (iz & 1);
-		double tr[TRIAL_W];
-		make_trial(d, rp, r, sweep, cloc, col, d.slot[j], cnt, tr);
-#pragma unroll
-		for(int e = 0; e < TRIAL_W; e++) d.trials[(long long) dst * TRIAL_W + e] = tr[e];
-	}
 }
 
 void launch_rebuild(const DevState &d, int mode, uint64_t sweep, long long trace_pos, hipStream_t s) {
@@ -524,16 +511,21 @@ struct PassArgs {
 
 // Per-team LDS (doubles): the cell's members ranked by particle index
 // (x, y, z, O, and their 4 patch vectors), their particle indices, the
-// cell's trials, the current trial's O' and patch vectors, the cell's record
-// (surrounding slot ranges) and the replica's acceptance table.
+// cell's trials, the cell's record (surrounding slot ranges) and the
+// replica's acceptance table.
 constexpr int OWN_W = NCOMP + 12;   // x, y, z, O[9], P[12]
 constexpr int LDS_OWN = CELL_CAP * OWN_W;
 constexpr int LDS_PID = (CELL_CAP + 1) / 2;  // CELL_CAP ints
 constexpr int LDS_TRIAL = CELL_CAP * TRIAL_W;
-constexpr int LDS_NEW = 24;            // O'[9] at 0, P'[12] at 12
 constexpr int LDS_NB = REC_W / 2;      // the cell record, REC_W ints
 constexpr int LDS_ACC = ACC_TAB;       // exp(-k/T), k < ACC_TAB
-constexpr int TEAM_LDS = LDS_OWN + LDS_PID + LDS_TRIAL + LDS_NEW + LDS_NB + LDS_ACC;
+constexpr int LDS_FIXED = LDS_OWN + LDS_PID + LDS_TRIAL + LDS_NB + LDS_ACC;
+// + the orientations of the T register-resident neighbours, component-major
+template <int T>
+constexpr int team_lds() {
+	return LDS_FIXED + 9 * T;
+}
+static_assert((LDS_OWN + LDS_PID + LDS_TRIAL) % 2 == 0 && LDS_FIXED % 2 == 0, "16-byte aligned record and team blocks");
 
 // Trial geometry shared by the helpers of one team.
 struct Trial {
@@ -618,16 +610,17 @@ __device__ __forceinline__ void bonds_old_new(const Trial &t, const double *Po, 
 // One cell of a colour pass: entry gi of the pass's list of non-empty cells
 // (all replicas), worked on by the T lanes of one team with its LDS block
 // `own`.  The cell's members
-// make cnt sequential trials.  Each lane keeps up to CPL of the cell's static
+// make sequential trials.  Each lane keeps up to CPL of the cell's static
 // neighbours (the 26 surrounding cells, fixed during the pass) in registers;
 // a denser neighbourhood is handled T at a time from HBM.
 template <int T, int CPL>
-__device__ __forceinline__ void pass_cell(const DevState &d, int pass, int gi, int lane, double *own) {
+__device__ __forceinline__ void pass_cell(const DevState &d, uint64_t sweep, int pass, int gi, int lane,
+                                          double *own) {
 	int *own_pid = (int *) (own + LDS_OWN);
 	double *trials = own + LDS_OWN + LDS_PID;
-	double *nw = trials + LDS_TRIAL;       // O'[9] at 0, P'[12] at 12
-	int *nb_tab = (int *) (nw + LDS_NEW);  // the cell's record (build_record)
-	double *acc_tab = nw + LDS_NEW + LDS_NB;
+	int *nb_tab = (int *) (trials + LDS_TRIAL);  // the cell's record (build_record)
+	double *acc_tab = trials + LDS_TRIAL + LDS_NB;
+	double *ori = own + LDS_FIXED;  // orientation component c of this lane's first neighbour at c * T + lane
 	const int S = d.stride;
 	{
 		// 1. the cell's record: surrounding slot ranges, own range, replica
@@ -648,9 +641,8 @@ __device__ __forceinline__ void pass_cell(const DevState &d, int pass, int gi, i
 		double *F = d.f[sel];
 		int *PID = d.pid[sel];
 
-		// 2. every load of the cell at once: the members, the cell's trials
-		// (written by the rebuild), the static neighbours' positions and the
-		// replica's geometry
+		// 2. every load of the cell at once: the members, the static
+		// neighbours' positions and the replica's geometry
 		int my_pid = 0;
 		double mine[NCOMP];
 		if(lane < cnt) {
@@ -658,11 +650,9 @@ __device__ __forceinline__ void pass_cell(const DevState &d, int pass, int gi, i
 #pragma unroll
 			for(int c = 0; c < NCOMP; c++) mine[c] = F[c * S + s0 + lane];
 		}
-		const double *tsrc = d.trials + (long long) s0 * TRIAL_W;
-		const int ntr = cnt * TRIAL_W;
-		double tv0 = 0., tv1 = 0.;
-		if(lane < ntr) tv0 = tsrc[lane];
-		if(lane + T < ntr) tv1 = tsrc[lane + T];
+		// trials of the cell in this pass: one per member (a sweep is then N
+		// attempts), or a fixed number per non-empty cell
+		const int ntrial = d.trials_per_cell > 0 ? d.trials_per_cell : cnt;
 		// slot of static neighbour i (0 <= i < M)
 		auto nb_slot = [&](int i) {
 			int lo2 = 0, hi2 = 26;
@@ -675,25 +665,24 @@ __device__ __forceinline__ void pass_cell(const DevState &d, int pass, int gi, i
 		};
 		// the first T * CPL static neighbours' positions into registers; the
 		// rest (dense neighbourhoods only) are read again at every trial
-		double cxr[CPL], cyr[CPL], czr[CPL], cor[CPL][9];
+		double cxr[CPL], cyr[CPL], czr[CPL];
 		int cqr[CPL];
 #pragma unroll
 		for(int u = 0; u < CPL; u++) {
 			const int i = lane + u * T;
 			cqr[u] = -1;
 			cxr[u] = cyr[u] = czr[u] = 0.;
-#pragma unroll
-			for(int c = 0; c < 9; c++) cor[u][c] = 0.;
 			if(i < M) {
 				const int q = nb_slot(i);
 				cqr[u] = q;
 				cxr[u] = F[q];
 				cyr[u] = F[S + q];
 				czr[u] = F[2 * S + q];
-#pragma unroll
-				for(int c = 0; c < 9; c++) cor[u][c] = F[(3 + c) * S + q];
 			}
 		}
+		double orv[9];
+#pragma unroll
+		for(int c = 0; c < 9; c++) orv[c] = cqr[0] >= 0 ? F[(3 + c) * S + cqr[0]] : 0.;
 		const double acc_v = lane < ACC_TAB ? rp.acc_tab[lane] : 0.;
 		Trial tr;
 		tr.Lx = unid<T>(rp.g.box[0]);
@@ -710,12 +699,19 @@ __device__ __forceinline__ void pass_cell(const DevState &d, int pass, int gi, i
 		const int n0 = uni<T>(rp.g.n[0]), n1 = uni<T>(rp.g.n[1]), n2 = uni<T>(rp.g.n[2]);
 		const int ix = cloc / (n1 * n2), iy = (cloc / n2) % n1, iz = cloc % n2;
 
-		// 3. trials and particle indices into LDS
-		if(lane < ntr) trials[lane] = tv0;
-		if(lane + T < ntr) trials[lane + T] = tv1;
-		for(int e = lane + 2 * T; e < ntr; e += T) trials[e] = tsrc[e];
+		// 3. the cell's trials, one lane each (Philox: no memory traffic),
+		// and the particle indices into LDS
+		if(lane < ntrial) {
+			double tv[TRIAL_W];
+			const int col = ((ix & 1) << 2) | ((iy & 1) << 1) | (iz & 1);
+			make_trial(d, rp, r, sweep, cloc, col, lane, cnt, tv);
+#pragma unroll
+			for(int e = 0; e < TRIAL_W; e++) trials[lane * TRIAL_W + e] = tv[e];
+		}
 		if(lane < cnt) own_pid[lane] = my_pid;
 		if(lane < ACC_TAB) acc_tab[lane] = acc_v;
+#pragma unroll
+		for(int c = 0; c < 9; c++) ori[c * T + lane] = orv[c];
 		team_sync();
 		// 4. members into LDS ranked by particle index, with their patch vectors
 		if(lane < cnt) {
@@ -734,16 +730,34 @@ __device__ __forceinline__ void pass_cell(const DevState &d, int pass, int gi, i
 		team_sync();
 		int c_acc = 0, d_bonds = 0;
 
-		for(int t = 0; t < cnt; t++) {
+		for(int t = 0; t < ntrial; t++) {
+			// the trial (one LDS read round) and this lane's member of the cell
 			const double *tp = trials + t * TRIAL_W;
-			const int k = uni<T>((int) tp[0]);
+			double tv[TRIAL_W];
+#pragma unroll
+			for(int e = 0; e < TRIAL_W; e++) tv[e] = tp[e];
+			const int k = uni<T>((int) tv[0]);
+			const bool has_m = lane < cnt && lane != k;
+			const double *wl = own + (lane < cnt ? lane : 0) * OWN_W;
+			double mx = 0., my = 0., mz = 0.;
+			if(has_m) {
+				mx = wl[0];
+				my = wl[1];
+				mz = wl[2];
+			}
+			// the moving member's state (a second round)
 			const double *me = own + k * OWN_W;
+			double Om[9], Po[12];
+#pragma unroll
+			for(int c = 0; c < 9; c++) Om[c] = me[3 + c];
+#pragma unroll
+			for(int c = 0; c < 12; c++) Po[c] = me[NCOMP + c];
 			tr.px = unid<T>(me[0]);
 			tr.py = unid<T>(me[1]);
 			tr.pz = unid<T>(me[2]);
-			tr.nx = unid<T>(tr.px + tp[1]);
-			tr.ny = unid<T>(tr.py + tp[2]);
-			tr.nz = unid<T>(tr.pz + tp[3]);
+			tr.nx = unid<T>(tr.px + tv[1]);
+			tr.ny = unid<T>(tr.py + tv[2]);
+			tr.nz = unid<T>(tr.pz + tv[3]);
 			// a trial that leaves the cell is rejected
 			if(uni<T>(cell_coord(tr.nx, sh0, tr.iLx, n0)) != ix || uni<T>(cell_coord(tr.ny, sh1, tr.iLy, n1)) != iy ||
 			   uni<T>(cell_coord(tr.nz, sh2, tr.iLz, n2)) != iz)
@@ -754,8 +768,7 @@ __device__ __forceinline__ void pass_cell(const DevState &d, int pass, int gi, i
 #pragma unroll
 			for(int u = 0; u < CPL; u++)
 				if(cqr[u] >= 0) ov |= overlaps_new(tr, cxr[u], cyr[u], czr[u]);
-			if(lane < cnt && lane != k)
-				ov |= overlaps_new(tr, own[lane * OWN_W], own[lane * OWN_W + 1], own[lane * OWN_W + 2]);
+			if(has_m) ov |= overlaps_new(tr, mx, my, mz);
 			for(int b = T * CPL; b < M; b += T)
 				if(b + lane < M) {
 					const int q = nb_slot(b + lane);
@@ -763,42 +776,45 @@ __device__ __forceinline__ void pass_cell(const DevState &d, int pass, int gi, i
 				}
 			if(team_any<T>(ov)) continue;
 
-			// O' = O R in the body frame (utils.c:172-184, 204-209): lane 3i+j
-			// forms element (i, j); lane 12+3a+k forms row k of O' again and from
-			// it component k of patch a of O' (utils.h:17-21)
-			if(lane < 9) {
-				const int i = lane / 3, j = lane % 3;
-				nw[lane] = me[3 + 3 * i] * tp[4 + j] + me[3 + 3 * i + 1] * tp[4 + 3 + j] + me[3 + 3 * i + 2] * tp[4 + 6 + j];
-			}
-			else if(lane >= 12 && lane < 24) {
-				const int pa = (lane - 12) / 3, kk = (lane - 12) % 3;
-				const double *o = me + 3 + 3 * kk;
-				double row[3];
+			// O' = O R in the body frame (utils.c:172-184, 204-209) and its
+			// patches (utils.h:17-21), formed by every lane in registers:
+			// P'[a][kk] = sum_j O'[kk][j] b_a[j] with b_a[j] = +-c, and
+			// (+-x) c = +-(x c) exactly, so the products x c are shared
+			double On[9], Pn[12], mc[9];
 #pragma unroll
-				for(int j = 0; j < 3; j++) row[j] = o[0] * tp[4 + j] + o[1] * tp[4 + 3 + j] + o[2] * tp[4 + 6 + j];
-				nw[lane] = row[0] * kf_base(pa, 0) + row[1] * kf_base(pa, 1) + row[2] * kf_base(pa, 2);
-			}
-			team_sync();
+			for(int i = 0; i < 3; i++)
+#pragma unroll
+				for(int j = 0; j < 3; j++)
+					On[3 * i + j] = Om[3 * i] * tv[4 + j] + Om[3 * i + 1] * tv[4 + 3 + j] + Om[3 * i + 2] * tv[4 + 6 + j];
+#pragma unroll
+			for(int e = 0; e < 9; e++) mc[e] = On[e] * KF_C;
+#pragma unroll
+			for(int a = 0; a < 4; a++)
+#pragma unroll
+				for(int kk = 0; kk < 3; kk++) {
+					const double a0 = ((KF_NEG >> (3 * a)) & 1u) ? -mc[3 * kk] : mc[3 * kk];
+					const double a1 = ((KF_NEG >> (3 * a + 1)) & 1u) ? -mc[3 * kk + 1] : mc[3 * kk + 1];
+					const double a2 = ((KF_NEG >> (3 * a + 2)) & 1u) ? -mc[3 * kk + 2] : mc[3 * kk + 2];
+					Pn[3 * a + kk] = a0 + a1 + a2;
+				}
 
 			// phase 2: bonds at the old and the new position, over this lane's
 			// neighbours: the static ones in registers, one member of the cell,
 			// and (dense neighbourhoods) the rest from HBM
-			const double *Po = me + NCOMP, *Pn = nw + 12;
 			auto each_neighbour = [&](auto &&f) {
 #pragma unroll
 				for(int u = 0; u < CPL; u++)
 					if(cqr[u] >= 0)
 						f(cxr[u], cyr[u], czr[u], [&](double *Oq) {
+							const int q = cqr[u];
 #pragma unroll
-							for(int c = 0; c < 9; c++) Oq[c] = cor[u][c];
+							for(int c = 0; c < 9; c++) Oq[c] = u == 0 ? ori[c * T + lane] : F[(3 + c) * S + q];
 						});
-				if(lane < cnt && lane != k) {
-					const double *w = own + lane * OWN_W;
-					f(w[0], w[1], w[2], [&](double *Oq) {
+				if(has_m)
+					f(mx, my, mz, [&](double *Oq) {
 #pragma unroll
-						for(int c = 0; c < 9; c++) Oq[c] = w[3 + c];
+						for(int c = 0; c < 9; c++) Oq[c] = wl[3 + c];
 					});
-				}
 				for(int b = T * CPL; b < M; b += T)
 					if(b + lane < M) {
 						const int q = nb_slot(b + lane);
@@ -829,15 +845,21 @@ __device__ __forceinline__ void pass_cell(const DevState &d, int pass, int gi, i
 			// an integer, exp(-dE/T) comes from the replica's table
 			const int idE = -db;
 			const double w = idE <= 0 ? 1. : (idE < ACC_TAB ? acc_tab[idE] : exp(-dE / temp));
-			const bool acc = (dE < 0.) || (tp[13] < w);
+			const bool acc = (dE < 0.) || (tv[13] < w);
 			if(acc) {
 				c_acc++;
 				d_bonds += db;
 				// member k takes the new position, orientation and patches
-				double *w = own + k * OWN_W;
-				if(lane < 3) w[lane] = lane == 0 ? tr.nx : (lane == 1 ? tr.ny : tr.nz);
-				else if(lane < 12) w[lane] = nw[lane - 3];
-				else if(lane < 24) w[lane] = nw[lane];
+				if(lane == 0) {
+					double *wk = own + k * OWN_W;
+					wk[0] = tr.nx;
+					wk[1] = tr.ny;
+					wk[2] = tr.nz;
+#pragma unroll
+					for(int c = 0; c < 9; c++) wk[3 + c] = On[c];
+#pragma unroll
+					for(int c = 0; c < 12; c++) wk[NCOMP + c] = Pn[c];
+				}
 			}
 			team_sync();
 		}
@@ -855,7 +877,7 @@ __device__ __forceinline__ void pass_cell(const DevState &d, int pass, int gi, i
 		}
 		if(lane == 0) {
 			const int slot = gi & (NSLOT - 1);
-			atomicAdd(ctr_at(d.ctr, r, C_TRY_ROTO, slot), (unsigned long long) cnt);
+			atomicAdd(ctr_at(d.ctr, r, C_TRY_ROTO, slot), (unsigned long long) ntrial);
 			if(c_acc) {
 				atomicAdd(ctr_at(d.ctr, r, C_ACC_ROTO, slot), (unsigned long long) c_acc);
 				atomicAdd(ctr_at(d.ctr, r, C_DBONDS, slot), (unsigned long long) (long long) d_bonds);
@@ -864,16 +886,24 @@ __device__ __forceinline__ void pass_cell(const DevState &d, int pass, int gi, i
 	}
 }
 
-// One colour pass per launch: team g of the grid takes list entry g; teams
-// past the end of the list leave at once.
+// One colour pass per launch, one list entry per team.  The grid is sized
+// for the largest list; workgroups reach the 8 XCDs round-robin, and the
+// list is in cell order, so XCD x takes the x-th eighth of the entries
+// actually listed (a slab of the box, whose shared neighbour reads its L2
+// then serves); the surplus workgroups of each XCD leave at once.
 template <int T, int CPL>
-__global__ __launch_bounds__(256) void k_pass(PassArgs a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_pass(PassArgs a) {
 	extern __shared__ double lds[];
 	const int lane = threadIdx.x & (T - 1);
 	const int team = threadIdx.x / T;
-	const int gi = uni<T>(blockIdx.x * (blockDim.x / T) + team);
-	if(gi >= a.d.pfx[a.pass * (a.d.R + 1) + a.d.R]) return;
-	pass_cell<T, CPL>(a.d, a.pass, gi, lane, lds + team * TEAM_LDS);
+	const int tpb = blockDim.x / T;
+	const int total = a.d.pfx[a.pass * (a.d.R + 1) + a.d.R];
+	const int per_xcd = ((total + tpb - 1) / tpb + 7) >> 3;  // workgroups with work on each XCD
+	const int i = blockIdx.x >> 3;
+	if(i >= per_xcd) return;
+	const int gi = uni<T>(((blockIdx.x & 7) * per_xcd + i) * tpb + team);
+	if(gi >= total) return;
+	pass_cell<T, CPL>(a.d, a.sweep, a.pass, gi, lane, lds + team * team_lds<T>());
 }
 
 // Grid-wide barrier of a cooperative launch (every workgroup resident).
@@ -918,10 +948,10 @@ __global__ __launch_bounds__(256) void k_sweep_passes(PassArgs a) {
 	const int tpb = blockDim.x / T;
 	const int nteams = gridDim.x * tpb;
 	const int g0 = uni<T>(blockIdx.x * tpb + team);
-	double *own = lds + team * TEAM_LDS;
+	double *own = lds + team * team_lds<T>();
 	for(int k = 0; k < 8; k++) {
 		const int total = uni<T>(d.pfx[k * (d.R + 1) + d.R]);
-		for(int gi = g0; gi < total; gi += nteams) pass_cell<T, CPL>(d, k, gi, lane, own);
+		for(int gi = g0; gi < total; gi += nteams) pass_cell<T, CPL>(d, a.sweep, k, gi, lane, own);
 		if(k < 7) grid_barrier(d.bar, d.err);
 	}
 }
@@ -930,13 +960,15 @@ template <int T, int CPL>
 static void pass_T(const DevState &d, uint64_t sweep, int pass, int block, hipStream_t s) {
 	const int tpb = block / T;
 	int nb = (d.total_act + tpb - 1) / tpb;
-	if(nb < 1) nb = 1;
-	size_t lds = (size_t) tpb * TEAM_LDS * sizeof(double);
+	nb = (nb + 7) & ~7;  // 8 XCDs x the most workgroups one XCD may need
+	size_t lds = (size_t) tpb * team_lds<T>() * sizeof(double);
 	PassArgs a{d, sweep, pass};
 	hipLaunchKernelGGL((k_pass<T, CPL>), dim3(nb), dim3(block), lds, s, a);
 }
 
-size_t pass_lds_bytes(int T, int block) { return (size_t) (block / T) * TEAM_LDS * sizeof(double); }
+size_t pass_lds_bytes(int T, int block) {
+	return (size_t) (block / T) * (T == 32 ? team_lds<32>() : team_lds<64>()) * sizeof(double);
+}
 
 void launch_pass(const DevState &d, uint64_t sweep, int pass, int T, int block, hipStream_t s) {
 	if(T == 32) pass_T<32, 1>(d, sweep, pass, block, s);

@@ -23,24 +23,25 @@ class System:
 
     def __init__(self, r, O, box, *, T, disp_max=0.05, theta_max=0.1, kf_delta=0.12, kf_cosmax=0.92,
                  ensemble=NVT, P=0.0, rescale_factor_max=0.0, seed=1, threads_per_cell=0, block_size=0,
-                 cell_occupancy=0.0, device=0, launch=0):
+                 cell_occupancy=0.0, device=0, launch=0, trials_per_cell=0):
         self._h = None
         self._make([r], [O], [box], [dict(T=T, disp_max=disp_max, theta_max=theta_max, kf_delta=kf_delta,
                                           kf_cosmax=kf_cosmax, ensemble=ensemble, P=P,
                                           rescale_factor_max=rescale_factor_max)],
-                   seed, threads_per_cell, block_size, cell_occupancy, device, launch)
+                   seed, threads_per_cell, block_size, cell_occupancy, device, launch, trials_per_cell)
 
     @classmethod
     def batch(cls, rs, Os, boxes, params, *, seed=1, threads_per_cell=0, block_size=0, cell_occupancy=0.0,
-              device=0, launch=0):
+              device=0, launch=0, trials_per_cell=0):
         """Many independent replicas in one context; ``params`` is a list of
         dicts with the keyword arguments of the constructor (T, P, ...)."""
         self = cls.__new__(cls)
         self._h = None
-        self._make(rs, Os, boxes, params, seed, threads_per_cell, block_size, cell_occupancy, device, launch)
+        self._make(rs, Os, boxes, params, seed, threads_per_cell, block_size, cell_occupancy, device, launch,
+                   trials_per_cell)
         return self
 
-    def _make(self, rs, Os, boxes, params, seed, tpc, block, occ, device, launch):
+    def _make(self, rs, Os, boxes, params, seed, tpc, block, occ, device, launch, tpc_trials):
         R = len(rs)
         self.n = [int(np.asarray(r).size // 3) for r in rs]
         P = (Params * R)()
@@ -53,7 +54,8 @@ class System:
             P[k].kf_cosmax = float(p.get("kf_cosmax", 0.92))
             P[k].pressure = float(p.get("P", 0.0))
             P[k].rescale_factor_max = float(p.get("rescale_factor_max", 0.0))
-        o = Options(int(seed) & 0xFFFFFFFFFFFFFFFF, int(tpc), int(block), float(occ), int(device), int(launch))
+        o = Options(int(seed) & 0xFFFFFFFFFFFFFFFF, int(tpc), int(block), float(occ), int(device), int(launch),
+                    int(tpc_trials))
         r_all = np.concatenate([_arr(r) for r in rs]) if R else np.zeros(0)
         O_all = np.concatenate([_arr(x) for x in Os]) if R else np.zeros(0)
         b_all = np.concatenate([_arr(b) for b in boxes])
@@ -152,8 +154,10 @@ class System:
         check(lib().pp_get_trace(self._h, rep, _ptr(out), cap, C.byref(cnt)))
         return -out[:cnt.value]
 
-    def set_timing(self, on=True):
-        check(lib().pp_set_timing(self._h, 1 if on else 0))
+    def set_timing(self, every=1):
+        """Bracket the launches of every ``every``-th sweep with HIP events
+        (0 or False: off).  Resets the accumulated times."""
+        check(lib().pp_set_timing(self._h, int(every)))
 
     def timing(self, name=None):
         nl = C.c_longlong(0)
