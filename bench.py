@@ -1,10 +1,13 @@
 #!/usr/bin/env python3
 """Benchmark of the MI355X Kern-Frenkel rototranslation hot path.
 
-Workload (one GPU): plain NVT checkerboard sweeps of N=32768 hard spheres with
-four Kern-Frenkel patches (delta 0.12, cosmax 0.92) at rho=0.5, T=0.12,
+Workload (one GPU): plain NVT checkerboard Monte Carlo of N=32768 hard spheres
+with four Kern-Frenkel patches (delta 0.12, cosmax 0.92) at rho=0.5, T=0.12,
 Disp_max 0.05, Theta_max 0.1, FP64, from a seeded random-insertion start.
-A step is one sweep = N single-particle rototranslation attempts.
+A step is one rebuild on a randomly shifted grid plus 8 colour passes in which
+every non-empty cell makes --trials-per-cell trials (0: one per member, so a
+step is exactly N attempts).  The metric counts sweeps of N attempted moves
+(MC.c:21): value = attempts / N / second.
 
 Prints one JSON line: sweeps/s (value), the dominant kernel's achieved
 algorithmic bandwidth against the HBM roofline (HIP events on the kernel's
@@ -67,7 +70,7 @@ def cpu_baseline(r, O, box, seconds):
         t_short, t_long = run(5), run(5 + k)
         rate = k / max(t_long - t_short, 1e-9)
         return dict(value=rate, unit="sweeps/s", cores=1, kind="reference",
-                    sample=f"{k} sweeps of the N={N} rho={RHO} T={T} start (difference of a {5 + k}- and a "
+                    sample=f"{k} sweeps of the N={N} rho={RHO} T={T} state the GPU run ended in (difference of a {5 + k}- and a "
                            f"5-sweep run of oracle/_ref/PatchyParticles_fast, -O3 -ffast-math, 1 core)")
     exe = os.path.join(ROOT, "oracle", "cpu_bench")
     out = subprocess.run(taskset + [exe, str(N), str(RHO), str(T), str(SEED), str(seconds)], check=True,
@@ -92,8 +95,12 @@ def measured_traffic(path=os.path.join(ROOT, "profiles", "k_pass_n32768_rho0.5_p
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=4000)
-    ap.add_argument("--warmup", type=int, default=400)
+    ap.add_argument("--steps", type=int, default=4000, help="timed steps (a step: rebuild + 8 colour passes)")
+    ap.add_argument("--warmup", type=int, default=6000, help="untimed steps first (equilibration)")
+    ap.add_argument("--trials-per-cell", type=int, default=8,
+                    help="trials per non-empty cell and pass (0: one per member, a step of exactly N attempts)")
+    ap.add_argument("--per-member-steps", type=int, default=2000,
+                    help="steps of the one-trial-per-member schedule timed after the main run (0: skip)")
     ap.add_argument("--tpc", type=int, default=0, help="threads per cell (0: default)")
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--occupancy", type=float, default=0.0)
@@ -126,14 +133,15 @@ def main():
     from patchyparticles_amd.stats import tau_int
 
     r, O, box = pp.generate(N, RHO, SEED)
-    sysm = pp.System(r, O, box, T=T, disp_max=DISP, theta_max=THETA, kf_delta=DELTA, kf_cosmax=COSMAX,
-                     seed=SEED, threads_per_cell=args.tpc, block_size=args.block, cell_occupancy=args.occupancy)
+    opts = dict(T=T, disp_max=DISP, theta_max=THETA, kf_delta=DELTA, kf_cosmax=COSMAX, threads_per_cell=args.tpc,
+                block_size=args.block, cell_occupancy=args.occupancy)
+    sysm = pp.System(r, O, box, seed=SEED, trials_per_cell=args.trials_per_cell, **opts)
     print(f"# start: N={N} box={box[0]:.4f} grid={sysm.grid()}", file=sys.stderr, flush=True)
     sysm.do_ensemble(args.warmup)
-    print(f"# warmup {args.warmup} sweeps done", file=sys.stderr, flush=True)
+    print(f"# warmup {args.warmup} steps done, E/N={-sysm.counters()['bonds'] / N:.4f}", file=sys.stderr, flush=True)
 
-    # timed region: K sweeps; HIP events bracket the kernels of every
-    # --time-every-th sweep (the per-kernel averages of the roofline line)
+    # timed region: K steps; HIP events bracket the kernels of every
+    # --time-every-th step (the per-kernel averages of the roofline line)
     sysm.reset_counters()
     sysm.set_timing(0 if args.no_timing else args.time_every)
     sysm.set_trace(args.steps)
@@ -145,28 +153,43 @@ def main():
     n_reb, ms_reb = sysm.timing("rebuild")
     c = sysm.counters()
     e_series = sysm.trace() / N
-    sysm.set_timing(False)
-    print(f"# timed {args.steps} sweeps in {elapsed:.3f} s", file=sys.stderr, flush=True)
+    sysm.set_timing(0)
+    print(f"# timed {args.steps} steps in {elapsed:.3f} s", file=sys.stderr, flush=True)
 
     attempts = c["tries"][0]
-    sweeps_per_s = attempts / N / elapsed
+    sweeps_per_s = attempts / N / elapsed  # a sweep is N attempted moves (MC.c:21)
     nside, side = sysm.grid()
     bpm = algorithmic_bytes_per_move(RHO, side[0])
-    moves_per_pass = attempts / (8 * args.steps)  # a sweep is 8 colour passes
+    bytes_per_launch = bpm * N / 8  # the data of the N/8 particles one colour pass covers
     avg_pass_s = ms_pass / max(n_pass, 1) / 1e3
-    achieved = bpm * moves_per_pass / avg_pass_s / 1e9 if avg_pass_s > 0 else 0.0
-    tau = tau_int(e_series)
+    achieved = bytes_per_launch / avg_pass_s / 1e9 if avg_pass_s > 0 else 0.0
+    steps_per_sweep = N * args.steps / attempts
+    tau = tau_int(e_series) / steps_per_sweep  # in sweeps of N attempts
     decorrelated = sweeps_per_s / (2 * tau)
 
     # parity spot check of the final snapshot (energy recomputed vs tracked)
     b, ov, tot, anyo = sysm.energy()
     assert not anyo and tot == c["bonds"], "tracked energy disagrees with a full recompute"
+    r_eq, O_eq, box_eq = sysm.state()
     sysm.close()
+
+    per_member = None
+    if args.per_member_steps > 0 and args.trials_per_cell != 0:
+        # the same state advanced with one trial per member of each cell
+        with pp.System(r_eq, O_eq, box_eq, seed=SEED + 1, trials_per_cell=0, **opts) as s0:
+            s0.do_ensemble(50)
+            s0.reset_counters()
+            ta = time.perf_counter()
+            s0.do_ensemble(args.per_member_steps)
+            tb = time.perf_counter()
+            c0 = s0.counters()
+        per_member = {"trials_per_cell": 0, "sweeps_per_s": round(c0["tries"][0] / N / (tb - ta), 2),
+                      "steps": args.per_member_steps}
 
     cpu = None
     if not args.no_cpu:
         print("# cpu baseline ...", file=sys.stderr, flush=True)
-        cpu = cpu_baseline(r, O, box, args.cpu_seconds)
+        cpu = cpu_baseline(r_eq, O_eq, box_eq, args.cpu_seconds)
 
     out = {
         "metric": "particle sweeps/s + decorrelated samples/s (E/N), learned vs plain, 1 GPU",
@@ -185,17 +208,20 @@ def main():
             "workload": f"plain NVT checkerboard rototranslation, N={N}, rho={RHO}, T={T}, KF_delta={DELTA}, "
                         f"KF_cosmax={COSMAX}, Disp_max={DISP}, Theta_max={THETA}, FP64",
             "grid": nside, "cell_side": round(side[0], 5),
-            "threads_per_cell": args.tpc or 32, "attempts": attempts,
+            "threads_per_cell": args.tpc or 32, "trials_per_cell": args.trials_per_cell,
+            "step": "rebuild on a randomly shifted grid + 8 colour passes",
+            "attempts": attempts, "attempts_per_step": round(attempts / args.steps, 1),
             "acceptance": round(c["accepted"][0] / max(attempts, 1), 5),
             "E_per_N": round(float(-c["bonds"] / N), 6),
         },
         "decorrelated_samples_per_s": {"plain": round(decorrelated, 3), "tau_int_sweeps": round(tau, 2),
                                        "learned": None},
+        "one_trial_per_member": per_member,
         "roofline": {
             "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": measured_traffic(),
             "kernel": "k_pass (colour pass)", "bytes_per_move": round(bpm, 1),
-            "moves_per_launch": round(moves_per_pass, 1), "avg_launch_us": round(avg_pass_s * 1e6, 3),
+            "algorithmic_bytes_per_launch": round(bytes_per_launch), "avg_launch_us": round(avg_pass_s * 1e6, 3),
             "rebuild_us_per_sweep": round(ms_reb / max(n_reb, 1) * 1e3, 3),
             "timed_sweeps": n_reb,
         },

@@ -64,20 +64,17 @@ def build_lib(verbose=True):
 
 
 def build_host():
-    """C host programs (input file / .rrr / output driver, generator)."""
-    srcs = [f for f in ("pp_main.c", "pp_input.c", "pp_rrr.c", "pp_output.c") if os.path.exists(os.path.join(HOST, f))]
-    if not srcs:
+    """C host programs: PatchyParticles_hip (input file / .rrr / outputs driver)
+    and generator_hip, both linked to libpp_hip.so next to them."""
+    if not os.path.exists(os.path.join(HOST, "pp_main.c")):
         return None
     os.makedirs(BIN, exist_ok=True)
-    inc = ["-I", os.path.join(ROOT, "include"), "-I", HOST]
-    common = [os.path.join(HOST, f) for f in ("pp_input.c", "pp_rrr.c", "pp_output.c") if os.path.exists(os.path.join(HOST, f))]
+    flags = ["-O2", "-std=gnu99", "-Wall", "-ffp-contract=off", "-I", os.path.join(ROOT, "include"), "-I", HOST]
+    link = ["-L", PKG, "-lpp_hip", "-Wl,-rpath,$ORIGIN/..", "-lm"]
+    common = [os.path.join(HOST, f) for f in ("pp_input.c", "pp_rrr.c", "pp_output.c")]
     out = os.path.join(BIN, "PatchyParticles_hip")
-    _run(["gcc", "-O2", "-std=gnu99", "-Wall", *inc, os.path.join(HOST, "pp_main.c"), *common,
-          "-L", PKG, "-lpp_hip", "-Wl,-rpath,$ORIGIN/..", "-lm", "-o", out])
-    gen = os.path.join(HOST, "pp_generator.c")
-    if os.path.exists(gen):
-        _run(["gcc", "-O2", "-std=gnu99", "-Wall", *inc, gen, os.path.join(HOST, "pp_rrr.c"), "-lm",
-              "-o", os.path.join(BIN, "generator_hip")])
+    _run(["gcc", *flags, os.path.join(HOST, "pp_main.c"), *common, *link, "-o", out])
+    _run(["gcc", *flags, os.path.join(HOST, "pp_generator.c"), *link, "-o", os.path.join(BIN, "generator_hip")])
     return out
 
 

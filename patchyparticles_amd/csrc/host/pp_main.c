@@ -1,0 +1,190 @@
+/*
+ * pp_main.c -- PatchyParticles_hip: the reference's driver program (main.c)
+ * for Dynamics = 0 (rototranslation MC) in Ensemble 0 (NVT) or 5 (NPT), with
+ * every sweep run on the GPU through libpp_hip.so.
+ *
+ * Same input keys (system.c:58-89, output.c:19-110), same .rrr files and the
+ * same energy / density / acceptance outputs and stdout lines at the same
+ * cadence (main.c:78-125): at every Print_every step the outputs are written,
+ * last.rrr is refreshed and, after step 0, the energy is recomputed from
+ * scratch and checked (MC_check_energy, cells_check); at every Save_every
+ * step (after 0) conf_<step>.rrr and last.rrr are written.  Between two such
+ * steps the sweeps run on the GPU in one call.  SIGTERM, SIGABRT, SIGINT and
+ * SIGUSR2 end the run after the current block of sweeps, with the last
+ * configuration saved (main.c:16-24).
+ *
+ * Keys of this program only (optional): GPU_device (HIP ordinal, 0),
+ * Threads_per_cell (32 or 64), Trials_per_cell (0: a sweep is N attempts;
+ * s > 0: s trials per non-empty sweep cell and colour pass).
+ */
+#include <signal.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+
+#include "pp.h"
+#include "pp_input.h"
+#include "pp_output.h"
+#include "pp_rrr.h"
+
+static volatile sig_atomic_t g_stop = 0;
+
+static void on_signal(int sig) {
+	signal(sig, SIG_DFL); /* a second signal ends the program at once */
+	g_stop = 1;
+}
+
+/* sweeps per library call between two output steps: keeps a signal answered
+ * within a fraction of a second at any size */
+#define MAX_BLOCK 2000
+
+typedef struct run {
+	ppo out;
+	pp_ctx *ctx;
+	int n;
+	int npt;
+	double *r, *O;
+} run;
+
+static void check(run *u, pp_status st, const char *what) {
+	if(st == PP_OK) return;
+	ppo_exit(&u->out, "%s failed: %s (%s)\n", what, pp_strerror(st), pp_last_error());
+}
+
+static void save_conf(run *u, long long step, const char *path) {
+	double box[3];
+	check(u, pp_download(u->ctx, 0, u->r, u->O, box), "pp_download");
+	if(pprrr_write(path, step, u->n, box, u->r, u->O) != 0) ppo_exit(&u->out, "File '%s' is not writable\n", path);
+}
+
+static void print_step(run *u, long long step) {
+	pp_counters c;
+	check(u, pp_get_counters(u->ctx, 0, &c), "pp_get_counters");
+	ppo_print(&u->out, step, u->n, -(double) c.bonds, c.volume, c.accepted[0], c.tries[0], c.accepted[1], c.tries[1]);
+	check(u, pp_reset_counters(u->ctx), "pp_reset_counters");
+	save_conf(u, step, u->out.last);
+}
+
+/* MC_check_energy + cells_check (MC.c:766-784, cells.c:65-76) */
+static void check_energy(run *u) {
+	const pp_status st = pp_check_energy(u->ctx, NULL);
+	if(st == PP_ERR_OVERLAP) ppo_exit(&u->out, "\nComputing energy from scratch resulted in an overlap, aborting\n");
+	if(st == PP_ERR_ENERGY || st == PP_ERR_CELLS) ppo_exit(&u->out, "\n%s\n", pp_last_error());
+	check(u, st, "pp_check_energy");
+}
+
+static long long next_multiple(long long x, long long m) { return (x / m + 1) * m; }
+
+int main(int argc, char *argv[]) {
+	if(argc == 1) {
+		fprintf(stderr, "Usage is %s input\n", argv[0]);
+		return 1;
+	}
+	signal(SIGTERM, on_signal);
+	signal(SIGABRT, on_signal);
+	signal(SIGINT, on_signal);
+	signal(SIGUSR2, on_signal);
+
+	ppi_file *in = ppi_load(argv[1]);
+	if(!in) return 1;
+	run u;
+	memset(&u, 0, sizeof u);
+
+	pp_params prm;
+	memset(&prm, 0, sizeof prm);
+	int dynamics = 0;
+	char conf[512];
+	ppi_int(in, "Ensemble", &prm.ensemble, 1);
+	ppi_string(in, "Initial_conditions_file", conf, sizeof conf, 1);
+	ppo_init(&u.out, in, prm.ensemble, conf);
+
+	ppi_int(in, "Dynamics", &dynamics, 1);
+	if(dynamics != 0) ppo_exit(&u.out, "Dynamics %d not supported\n", dynamics);
+	if(prm.ensemble != PP_NVT && prm.ensemble != PP_NPT) ppo_exit(&u.out, "Ensemble %d not supported\n", prm.ensemble);
+	ppi_double(in, "Disp_max", &prm.disp_max, 1);
+	ppi_double(in, "Theta_max", &prm.theta_max, 1);
+	ppi_double(in, "Temperature", &prm.temperature, 1);
+
+	int seed = 0;
+	if(ppi_int(in, "Seed", &seed, 0) == PPI_MISSING) {
+		seed = (int) time(NULL);
+		ppo_log(&u.out, "Using seed %d\n", seed);
+	}
+
+	pprrr c;
+	char err[1024];
+	if(pprrr_read(conf, &c, err, sizeof err) != 0) ppo_exit(&u.out, "%s", err);
+	u.n = c.n;
+	u.npt = prm.ensemble == PP_NPT;
+
+	ppi_double(in, "KF_delta", &prm.kf_delta, 1);
+	ppi_double(in, "KF_cosmax", &prm.kf_cosmax, 1);
+	ppo_log(&u.out, "Patch parameters: cosmax = %lf, delta = %lf\n", prm.kf_cosmax, prm.kf_delta);
+	if(prm.ensemble == PP_NPT) {
+		ppi_double(in, "rescale_factor_max", &prm.rescale_factor_max, 1);
+		ppi_double(in, "P", &prm.pressure, 1);
+	}
+	else {
+		int lx = 0;
+		if(ppi_int(in, "Lx_move", &lx, 0) == PPI_FOUND && lx) ppo_exit(&u.out, "Lx_move is not supported by this program\n");
+	}
+
+	pp_options opt;
+	memset(&opt, 0, sizeof opt);
+	opt.seed = (uint64_t) (uint32_t) seed;
+	ppi_int(in, "GPU_device", &opt.device, 0);
+	ppi_int(in, "Threads_per_cell", &opt.threads_per_cell, 0);
+	ppi_int(in, "Trials_per_cell", &opt.trials_per_cell, 0);
+
+	const pp_status st = pp_create(&prm, &opt, c.n, c.r, c.O, c.box, &u.ctx);
+	if(st == PP_ERR_OVERLAP) ppo_exit(&u.out, "Initial configuration contains an overlap, aborting\n");
+	if(st != PP_OK) ppo_exit(&u.out, "pp_create failed: %s (%s)\n", pp_strerror(st), pp_last_error());
+	pprrr_free(&c);
+	u.r = malloc(sizeof(double) * 3 * (size_t) (u.n > 0 ? u.n : 1));
+	u.O = malloc(sizeof(double) * 9 * (size_t) (u.n > 0 ? u.n : 1));
+	if(!u.r || !u.O) ppo_exit(&u.out, "out of memory\n");
+
+	pp_counters cnt;
+	check(&u, pp_get_counters(u.ctx, 0, &cnt), "pp_get_counters");
+	ppo_log(&u.out, "Initial energy: %lf\n", u.n > 0 ? -(double) cnt.bonds / u.n : 0.);
+
+	long long steps = 0;
+	ppi_llint(in, "Steps", &steps, 1);
+	steps += u.out.start_from;
+
+	long long step = u.out.start_from;
+	while(step < steps && !g_stop) {
+		if(step % u.out.print_every == 0) {
+			print_step(&u, step);
+			if(step != 0) check_energy(&u);
+		}
+		if(step > 0 && step % u.out.save_every == 0) {
+			char name[1100];
+			snprintf(name, sizeof name, "%s/conf_%lld.rrr", u.out.folder, step);
+			save_conf(&u, step, name);
+			save_conf(&u, step, u.out.last);
+		}
+		long long until = next_multiple(step, u.out.print_every);
+		const long long s2 = next_multiple(step, u.out.save_every);
+		if(s2 < until) until = s2;
+		if(steps < until) until = steps;
+		if(until - step > MAX_BLOCK) until = step + MAX_BLOCK;
+		check(&u, pp_sweeps(u.ctx, until - step), "pp_sweeps");
+		step = until;
+	}
+
+	save_conf(&u, step, u.out.last);
+	check(&u, pp_get_counters(u.ctx, 0, &cnt), "pp_get_counters");
+	const double E = -(double) cnt.bonds;
+	printf("%lld %lf %lf", step, u.n > 0 ? E / u.n : 0., E);
+	if(u.npt) printf(" %lf", u.n / cnt.volume);
+	printf("\n");
+
+	pp_destroy(u.ctx);
+	free(u.r);
+	free(u.O);
+	ppo_free(&u.out);
+	ppi_free(in);
+	return 0;
+}

@@ -1,0 +1,43 @@
+/*
+ * pp_output.h -- the reference's output files and log (output.c:19-201,
+ * 355-388): energy.dat, acceptance.dat, density.dat (NPT), the configuration
+ * folder with conf_<step>.rrr and the "last" configuration, and the line
+ * printed on stdout at every print step.
+ */
+#ifndef PP_OUTPUT_H
+#define PP_OUTPUT_H
+
+#include <stdio.h>
+
+#include "pp_input.h"
+
+typedef struct ppo {
+	FILE *log;      /* stderr unless Log_file names a file */
+	FILE *energy;   /* Energy_file (energy.dat) */
+	FILE *acc;      /* Acceptance_file (acceptance.dat) */
+	FILE *density;  /* Density_file (density.dat), NPT only */
+	long long start_from;   /* first step: 0, or the configuration's step if Restart_step_counter = 0 */
+	long long print_every;  /* Print_every */
+	long long save_every;   /* Save_every */
+	char folder[512];       /* Configuration_folder */
+	char last[512];         /* Configuration_last (last.rrr) */
+	int npt;
+} ppo;
+
+/* Reads the output keys and opens the files; exits with a message on error. */
+void ppo_init(ppo *o, ppi_file *in, int ensemble, const char *initial_conf);
+void ppo_free(ppo *o);
+
+/* A message on the log (and on stderr when the log is a file). */
+void ppo_log(ppo *o, const char *fmt, ...);
+/* ppo_log, then exit(1) (output.c:374-388). */
+void ppo_exit(ppo *o, const char *fmt, ...);
+
+/* The print-step lines: energy.dat "step E/N", density.dat "step N/V N"
+ * (NPT), acceptance.dat "step acc_roto [acc_volume]", stdout
+ * "step E/N E [N/V]" (output.c:119-201).  Acceptances are accepted / tries
+ * since the previous print. */
+void ppo_print(ppo *o, long long step, int n, double energy, double volume, long long acc_roto, long long try_roto,
+               long long acc_vol, long long try_vol);
+
+#endif

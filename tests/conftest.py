@@ -1,3 +1,4 @@
+import json
 import os
 import subprocess
 import sys
@@ -19,7 +20,12 @@ def pytest_configure(config):
 
 
 def golden(name):
-    return np.load(os.path.join(GOLDEN, name), allow_pickle=False)
+    """A committed fixture: .npz arrays (no pickles) or .json data."""
+    path = os.path.join(GOLDEN, name)
+    if name.endswith(".json"):
+        with open(path) as f:
+            return json.load(f)
+    return np.load(path, allow_pickle=False)
 
 
 def golden_files(prefix):

@@ -11,8 +11,12 @@ executables.  Outputs are data only (inputs and expected outputs):
   traj_*.npz        positions/orientations/box after k steps of the
                     reference main loop (NVT and NPT), from a given start
   generator.json    sha256 of the reference generator's generated.rrr
+  host_runs.json    the lines the reference program prints at step 0
+                    (stdout, energy.dat, acceptance.dat, density.dat) on the
+                    reference generator's N=1000 rho=0.1 start, NVT and NPT
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py            (everything)
+    python tests/golden/make_golden.py host       (host_runs.json only)
 """
 from __future__ import annotations
 
@@ -233,5 +237,43 @@ def main():
     print("generator:", gens)
 
 
+def host_runs():
+    """Step-0 outputs of the reference program: same input keys and start as
+    the host-program test uses (the start is the reference generator's
+    generated.rrr for N=1000, rho=0.1, which generator_hip reproduces)."""
+    build_ref()
+    d = tempfile.mkdtemp(prefix="host_")
+    subprocess.run([os.path.join(ROOT, "oracle", "_ref", "generator"), "1000", "0.1"], cwd=d, check=True,
+                   capture_output=True)
+    out = {}
+    for name, extra in (("nvt", {}), ("npt", {"Ensemble": 5, "P": 0.05, "rescale_factor_max": 0.05})):
+        keys = {"Dynamics": 0, "Ensemble": 0, "Seed": 17, "Temperature": 0.1, "Disp_max": 0.05, "Theta_max": 0.1,
+                "KF_delta": 0.12, "KF_cosmax": 0.92, "Initial_conditions_file": "generated.rrr", "Steps": 1,
+                "Print_every": 1, "Save_every": 1000, "Configuration_folder": "."}
+        keys.update(extra)
+        with open(os.path.join(d, "input"), "w") as f:
+            f.write("".join(f"{k} = {v}\n" for k, v in keys.items()))
+        for fn in ("energy.dat", "acceptance.dat", "density.dat"):
+            if os.path.exists(os.path.join(d, fn)):
+                os.remove(os.path.join(d, fn))
+        res = subprocess.run([os.path.join(ROOT, "oracle", "_ref", "PatchyParticles"), "input"], cwd=d, check=True,
+                             capture_output=True, text=True)
+
+        def first(fn):
+            p = os.path.join(d, fn)
+            return open(p).readline() if os.path.exists(p) else None
+
+        init = [x for x in res.stderr.splitlines() if x.startswith("Initial energy")]
+        out[name] = dict(keys=keys, stdout0=res.stdout.splitlines()[0] + "\n", energy0=first("energy.dat"),
+                         acceptance0=first("acceptance.dat"), density0=first("density.dat"), initial=init[0])
+    with open(os.path.join(HERE, "host_runs.json"), "w") as f:
+        json.dump(out, f, indent=1)
+    print("host_runs:", json.dumps(out)[:300])
+
+
 if __name__ == "__main__":
-    main()
+    if sys.argv[1:] == ["host"]:
+        host_runs()
+    else:
+        main()
+        host_runs()

@@ -1,0 +1,217 @@
+"""The C host programs (PatchyParticles_hip, generator_hip) and their parts:
+the input-file reader, the .rrr reader/writer and the output files, against
+the reference's own behaviour (golden files written by its programs)."""
+import hashlib
+import json
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import golden
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HOST = os.path.join(ROOT, "patchyparticles_amd", "csrc", "host")
+BIN = os.path.join(ROOT, "patchyparticles_amd", "bin")
+DRIVER = os.path.join(BIN, "PatchyParticles_hip")
+GENERATOR = os.path.join(BIN, "generator_hip")
+
+RRR_PROBE = r"""
+#include <stdio.h>
+#include <stdlib.h>
+#include "pp_rrr.h"
+int main(int argc, char **argv) {
+	char err[1024];
+	if(argv[1][0] == 'r') {
+		pprrr c;
+		if(pprrr_read(argv[2], &c, err, sizeof err) != 0) { fputs(err, stderr); return 3; }
+		FILE *f = fopen(argv[3], "wb");
+		double hdr[5] = {(double) c.step, (double) c.n, c.box[0], c.box[1], c.box[2]};
+		fwrite(hdr, sizeof(double), 5, f);
+		fwrite(c.r, sizeof(double), 3 * (size_t) c.n, f);
+		fwrite(c.O, sizeof(double), 9 * (size_t) c.n, f);
+		fclose(f);
+		pprrr_free(&c);
+		return 0;
+	}
+	FILE *f = fopen(argv[2], "rb");
+	double hdr[5];
+	if(fread(hdr, sizeof(double), 5, f) != 5) return 4;
+	int n = (int) hdr[1];
+	double *r = malloc(sizeof(double) * 3 * n), *O = malloc(sizeof(double) * 9 * n);
+	if(fread(r, sizeof(double), 3 * (size_t) n, f) != 3 * (size_t) n) return 4;
+	if(fread(O, sizeof(double), 9 * (size_t) n, f) != 9 * (size_t) n) return 4;
+	fclose(f);
+	return pprrr_write(argv[3], (long long) hdr[0], n, hdr + 2, r, O) ? 5 : 0;
+}
+"""
+
+INPUT_PROBE = r"""
+#include <stdio.h>
+#include "pp_input.h"
+int main(int argc, char **argv) {
+	ppi_file *f = ppi_load(argv[1]);
+	if(!f) return 2;
+	for(int i = 2; i < argc; i++) {
+		char s[256] = "<unset>";
+		int v = -12345;
+		double x = -1.5;
+		long long w = -7;
+		int a = ppi_string(f, argv[i], s, sizeof s, 0);
+		ppi_int(f, argv[i], &v, 0);
+		ppi_double(f, argv[i], &x, 0);
+		ppi_llint(f, argv[i], &w, 0);
+		printf("%s|%d|%s|%d|%.17g|%lld\n", argv[i], a, s, v, x, w);
+	}
+	int m = 0;
+	ppi_int(f, "Required_key", &m, 1);
+	ppi_free(f);
+	return 0;
+}
+"""
+
+
+def _compile(tmp_path, name, src, parts):
+    c = tmp_path / f"{name}.c"
+    c.write_text(src)
+    exe = tmp_path / name
+    subprocess.run(["gcc", "-O2", "-std=gnu99", "-ffp-contract=off", "-I", HOST, str(c),
+                    *[os.path.join(HOST, p) for p in parts], "-lm", "-o", str(exe)], check=True)
+    return str(exe)
+
+
+@pytest.fixture(scope="module")
+def host_built():
+    if not (os.path.exists(DRIVER) and os.path.exists(GENERATOR)):
+        pytest.skip("host programs not built (run __graft_entry__.build())")
+
+
+def test_generator_program_matches_reference_bytes(host_built, tmp_path):
+    for g in golden("generator.json"):
+        d = tmp_path / f"g{g['n']}"
+        d.mkdir()
+        subprocess.run([GENERATOR, str(g["n"]), str(g["rho"])], cwd=d, check=True, capture_output=True)
+        assert hashlib.sha256((d / "generated.rrr").read_bytes()).hexdigest() == g["sha256"]
+    res = subprocess.run([GENERATOR, "10", "0.8"], cwd=tmp_path, capture_output=True, text=True)
+    assert res.returncode == 1 and "density higher than 0.7" in res.stderr
+
+
+def test_rrr_reader_and_writer_match_reference(oracle, tmp_path):
+    probe = _compile(tmp_path, "rrr_probe", RRR_PROBE, ["pp_rrr.c"])
+    r, O, box = oracle.generate(257, 0.25, seed=11)
+    # rows that are not quite orthonormal, as files written with %lf are
+    O = O + 1e-7 * np.random.default_rng(1).normal(size=O.shape)
+    src = str(tmp_path / "in.rrr")
+    oracle.rrr_save(src, 41, box, r, O)
+    subprocess.run([probe, "r", src, str(tmp_path / "c.bin")], check=True)
+    v = np.fromfile(tmp_path / "c.bin")
+    step, n = int(v[0]), int(v[1])
+    s2, b2, r2, O2 = oracle.rrr_load(src)
+    assert step == s2 == 41 and n == 257
+    assert np.array_equal(v[2:5], b2)
+    assert np.array_equal(v[5:5 + 3 * n].reshape(n, 3), r2)
+    assert np.array_equal(v[5 + 3 * n:].reshape(n, 3, 3), O2)
+    subprocess.run([probe, "w", str(tmp_path / "c.bin"), str(tmp_path / "out.rrr")], check=True)
+    oracle.rrr_save(str(tmp_path / "ref.rrr"), 41, b2, r2, O2)
+    assert (tmp_path / "out.rrr").read_bytes() == (tmp_path / "ref.rrr").read_bytes()
+    # header count and body disagree: the reference's message
+    lines = open(src).read().splitlines()
+    (tmp_path / "short.rrr").write_text("\n".join(lines[:-3]) + "\n")
+    res = subprocess.run([probe, "r", str(tmp_path / "short.rrr"), str(tmp_path / "x.bin")], capture_output=True,
+                         text=True)
+    assert res.returncode == 3 and "Number of particles found in configuration (256)" in res.stderr
+
+
+def test_input_reader_semantics(tmp_path):
+    probe = _compile(tmp_path, "input_probe", INPUT_PROBE, ["pp_input.c"])
+    (tmp_path / "in").write_text(
+        "# comment = ignored\n"
+        "  Temperature =  0.125  \n"
+        "Steps = 1000000000000\n"
+        "\n"
+        "no equals sign here\n"
+        "Empty =\n"
+        "Empty = 3\n"
+        "Dup = 1\n"
+        "Dup = 2\n"
+        "Name = a b c\n")
+    res = subprocess.run([probe, str(tmp_path / "in"), "Temperature", "Steps", "Empty", "Dup", "Name", "Missing"],
+                         capture_output=True, text=True)
+    rows = {ln.split("|")[0]: ln.split("|")[1:] for ln in res.stdout.splitlines()}
+    assert rows["Temperature"] == ["0", "0.125", "0", "0.125", "0"]
+    assert rows["Steps"][4] == "1000000000000"
+    assert rows["Empty"][:2] == ["0", "3"]  # an empty value does not count as the key
+    assert rows["Dup"][:2] == ["0", "1"]  # the first occurrence wins
+    assert rows["Name"][1] == "a b c"
+    assert rows["Missing"] == ["-1", "<unset>", "-12345", "-1.5", "-7"]
+    assert "WARNING: malformed line 'no equals sign here'" in res.stderr
+    assert "INFO: optional key 'Missing' not found" in res.stderr
+    assert "Mandatory key 'Required_key' not found, exiting" in res.stderr
+    assert res.returncode == 255
+
+
+def _write_input(d, keys):
+    (d / "input").write_text("".join(f"{k} = {v}\n" for k, v in keys.items()))
+
+
+def test_driver_usage_and_input_errors(host_built, tmp_path):
+    res = subprocess.run([DRIVER], capture_output=True, text=True)
+    assert res.returncode == 1 and "Usage is" in res.stderr
+    keys = dict(golden("host_runs.json")["nvt"]["keys"])
+    subprocess.run([GENERATOR, "100", "0.1"], cwd=tmp_path, check=True, capture_output=True)
+    bad = dict(keys, Dynamics=1)
+    _write_input(tmp_path, bad)
+    res = subprocess.run([DRIVER, "input"], cwd=tmp_path, capture_output=True, text=True)
+    assert res.returncode == 1 and "Dynamics 1 not supported" in res.stderr
+    bad = dict(keys, Ensemble=3)
+    _write_input(tmp_path, bad)
+    res = subprocess.run([DRIVER, "input"], cwd=tmp_path, capture_output=True, text=True)
+    assert res.returncode == 1 and "Ensemble 3 not supported" in res.stderr
+    bad = {k: v for k, v in keys.items() if k != "Temperature"}
+    _write_input(tmp_path, bad)
+    res = subprocess.run([DRIVER, "input"], cwd=tmp_path, capture_output=True, text=True)
+    assert res.returncode == 255 and "Mandatory key 'Temperature' not found, exiting" in res.stderr
+    bad = dict(keys, Configuration_folder="no/such/dir")
+    _write_input(tmp_path, bad)
+    res = subprocess.run([DRIVER, "input"], cwd=tmp_path, capture_output=True, text=True)
+    assert res.returncode == 1 and "Cannot create files in directory 'no/such/dir'" in res.stderr
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("ensemble", ["nvt", "npt"])
+def test_driver_run_matches_reference_outputs(host_built, oracle, tmp_path, ensemble):
+    """The reference's generated start, its input keys: the step-0 lines equal
+    the reference program's byte for byte; later print steps, conf_<step>.rrr
+    and last.rrr follow the reference cadence, and the printed energies
+    agree with the oracle's energy of the saved configurations."""
+    g = golden("host_runs.json")[ensemble]
+    subprocess.run([GENERATOR, "1000", "0.1"], cwd=tmp_path, check=True, capture_output=True)
+    keys = dict(g["keys"], Steps=300, Print_every=100, Save_every=100, Configuration_folder="confs")
+    (tmp_path / "confs").mkdir()
+    _write_input(tmp_path, keys)
+    res = subprocess.run([DRIVER, "input"], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert res.returncode == 0, res.stderr
+    out = res.stdout.splitlines(keepends=True)
+    assert out[0] == g["stdout0"]
+    assert g["initial"] in res.stderr
+    energy = (tmp_path / "energy.dat").read_text().splitlines(keepends=True)
+    acc = (tmp_path / "acceptance.dat").read_text().splitlines(keepends=True)
+    assert energy[0] == g["energy0"] and acc[0] == g["acceptance0"]
+    assert [int(x.split()[0]) for x in energy] == [0, 100, 200]
+    assert len(out) == 4 and out[3].split()[0] == "300"
+    for line in acc[1:]:
+        assert 0.2 < float(line.split()[1]) < 1.0
+    if ensemble == "npt":
+        dens = (tmp_path / "density.dat").read_text().splitlines(keepends=True)
+        assert dens[0] == g["density0"] and len(dens) == 3
+    for step in (100, 200):
+        s, box, r, O = oracle.rrr_load(str(tmp_path / "confs" / f"conf_{step}.rrr"))
+        assert s == step
+        b, ov, tot = oracle.energy(r, O, box)
+        e_printed = float(energy[step // 100].split()[1])
+        # the file keeps 6 decimals of the orientation rows: a pair at a
+        # threshold may flip when it is read back
+        assert not ov.any() and abs(-tot / 1000 - e_printed) <= 3e-3
+    s, box, r, O = oracle.rrr_load(str(tmp_path / "last.rrr"))
+    assert s == 300
