@@ -84,7 +84,8 @@ typedef struct pp_options {
 	int launch;              /* PP_LAUNCH_*: how the 8 colour passes of a sweep are launched */
 	int trials_per_cell;     /* 0: each sweep cell makes as many trials as it holds, so a sweep is
 	                          * exactly N attempts (MC.c:21); s in [1, 16]: each non-empty cell
-	                          * makes s trials per pass, and a sweep s x (non-empty cells) attempts */
+	                          * makes s trials per pass, and a sweep s x (non-empty cells) attempts
+	                          * (and s NPT volume attempts instead of one) */
 } pp_options;
 
 /* pp_options.launch.  On MI355X a grid barrier across the 8 XCDs costs more

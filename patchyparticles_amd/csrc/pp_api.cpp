@@ -464,13 +464,19 @@ pp_status pp_sweeps(pp_ctx *c, long long nsw) {
 				toc(c, TK_PASS, b);
 			}
 		if(c->any_npt) {
-			hipEvent_t v{};
-			tic(c, TK_VOLUME, &v);
-			launch_vol_propose(c->d, sw, c->stream);
-			launch_rebuild(c->d, REB_TRIAL, sw, -1, c->stream);
-			launch_energy(c->d, 1, 0, c->stream);
-			launch_vol_decide(c->d, sw, c->stream);
-			toc(c, TK_VOLUME, v);
+			// one volume attempt per N particle attempts on average, as do_NPT
+			// (MC.c:60-68): with s trials per non-empty cell a step makes
+			// about s x (non-empty cells) <= s N attempts, and s volume attempts
+			const int nvol = c->d.trials_per_cell > 0 ? c->d.trials_per_cell : 1;
+			for(int k = 0; k < nvol; k++) {
+				hipEvent_t v{};
+				tic(c, TK_VOLUME, &v);
+				launch_vol_propose(c->d, sw, k, c->stream);
+				launch_rebuild(c->d, REB_TRIAL, sw, -1, c->stream);
+				launch_energy(c->d, 1, 0, c->stream);
+				launch_vol_decide(c->d, sw, k, c->stream);
+				toc(c, TK_VOLUME, v);
+			}
 		}
 		c->sweep++;
 		if(c->timing && c->pending.size() > 4096) TRY(collect_timing(c));

@@ -1090,13 +1090,13 @@ void launch_energy(const DevState &d, int trial, int per_particle, hipStream_t s
 // ------------------------------------------------------------------------
 // NPT volume move (MC.c:450-513): log-volume step along one random axis
 
-__global__ void k_vol_propose(DevState d, uint64_t sweep) {
+__global__ void k_vol_propose(DevState d, uint64_t sweep, int k) {
 	int r = blockIdx.x * blockDim.x + threadIdx.x;
 	if(r >= d.R) return;
 	Rep &rp = d.reps[r];
 	rp.trial_active = 0;
 	if(rp.ensemble != 5) return;
-	u32x4 v = rng(d.seed, r, sweep, 0, TAG_VOLUME, 0, 0);
+	u32x4 v = rng(d.seed, r, sweep, (uint32_t) k, TAG_VOLUME, 0, 0);  // attempt k of the sweep
 	const int dir = (int) (u01(v.x) * 3.);
 	const double V0 = rp.g.box[0] * rp.g.box[1] * rp.g.box[2];
 	const double lnV1 = log(V0) + (u01(v.y) - 0.5) * rp.rfm;
@@ -1126,11 +1126,11 @@ __global__ void k_vol_propose(DevState d, uint64_t sweep) {
 	rp.trial_active = 1;
 }
 
-void launch_vol_propose(const DevState &d, uint64_t sweep, hipStream_t s) {
-	hipLaunchKernelGGL(k_vol_propose, dim3((d.R + 63) / 64), dim3(64), 0, s, d, sweep);
+void launch_vol_propose(const DevState &d, uint64_t sweep, int k, hipStream_t s) {
+	hipLaunchKernelGGL(k_vol_propose, dim3((d.R + 63) / 64), dim3(64), 0, s, d, sweep, k);
 }
 
-__global__ void k_vol_decide(DevState d, uint64_t sweep) {
+__global__ void k_vol_decide(DevState d, uint64_t sweep, int k) {
 	int r = blockIdx.x * blockDim.x + threadIdx.x;
 	if(r >= d.R) return;
 	Rep &rp = d.reps[r];
@@ -1146,7 +1146,7 @@ __global__ void k_vol_decide(DevState d, uint64_t sweep) {
 	const double dE = (double) (B0 - B1);  // E' - E with E = -bonds
 	const double dV = rp.trial_V - rp.trial_V0;
 	const double arg = -(rp.P * dV + dE) / rp.T + (double) (rp.np + 1) * log(rp.trial_f);
-	u32x4 v = rng(d.seed, r, sweep, 0, TAG_VOLUME, 0, 0);
+	u32x4 v = rng(d.seed, r, sweep, (uint32_t) k, TAG_VOLUME, 0, 0);  // attempt k of the sweep
 	atomicAdd(ctr_at(d.ctr, r, C_TRY_VOL, 0), 1ull);
 	if(ov == 0 && u01(v.z) < exp(arg)) {
 		rp.sel ^= 1;
@@ -1158,8 +1158,8 @@ __global__ void k_vol_decide(DevState d, uint64_t sweep) {
 	rp.trial_active = 0;
 }
 
-void launch_vol_decide(const DevState &d, uint64_t sweep, hipStream_t s) {
-	hipLaunchKernelGGL(k_vol_decide, dim3((d.R + 63) / 64), dim3(64), 0, s, d, sweep);
+void launch_vol_decide(const DevState &d, uint64_t sweep, int k, hipStream_t s) {
+	hipLaunchKernelGGL(k_vol_decide, dim3((d.R + 63) / 64), dim3(64), 0, s, d, sweep, k);
 }
 
 // ------------------------------------------------------------------------

@@ -11,12 +11,16 @@ executables.  Outputs are data only (inputs and expected outputs):
   traj_*.npz        positions/orientations/box after k steps of the
                     reference main loop (NVT and NPT), from a given start
   generator.json    sha256 of the reference generator's generated.rrr
+  stats_ref.json    equilibrium <E/N> (and NPT <N/V>) of the reference
+                    program at small state points: 8 independent runs each,
+                    block statistics over the runs
   host_runs.json    the lines the reference program prints at step 0
                     (stdout, energy.dat, acceptance.dat, density.dat) on the
                     reference generator's N=1000 rho=0.1 start, NVT and NPT
 
     python tests/golden/make_golden.py            (everything)
     python tests/golden/make_golden.py host       (host_runs.json only)
+    python tests/golden/make_golden.py stats      (stats_ref.json only)
 """
 from __future__ import annotations
 
@@ -271,9 +275,91 @@ def host_runs():
     print("host_runs:", json.dumps(out)[:300])
 
 
+# state points of the statistical comparison: the reference program's own
+# output series (energy.dat, density.dat) from its generator's start
+STAT_POINTS = {
+    "nvt_rho03_T020": dict(n=300, rho=0.3, keys={"Ensemble": 0, "Temperature": 0.2}),
+    "nvt_rho05_T020": dict(n=300, rho=0.5, keys={"Ensemble": 0, "Temperature": 0.2}),
+    # the NPT density relaxes for ~2.5e5 sweeps from the rho = 0.3 start: the
+    # runs start from a reference-equilibrated configuration (a committed
+    # input file, stat_npt_T020_P02_start.rrr) written by a first long run
+    "npt_T020_P02": dict(n=300, rho=0.3, equilibrate=400000,
+                         keys={"Ensemble": 5, "Temperature": 0.2, "P": 0.2, "rescale_factor_max": 0.05}),
+}
+# (lower temperatures keep relaxing for 10^5+ sweeps at these sizes: not an
+# equilibrium comparison)
+STAT_RUNS, STAT_SWEEPS, STAT_PRINT, STAT_DISCARD = 8, 60000, 100, 0.5
+
+
+def stats_ref():
+    """8 independent runs of the reference program (production flags) per
+    state point; the mean of each run over its last 60 % of print steps,
+    then mean and standard error over the runs."""
+    build_ref()
+    exe = os.path.join(ROOT, "oracle", "_ref", "PatchyParticles_fast")
+    out = {}
+    for name, sp in STAT_POINTS.items():
+        d0 = tempfile.mkdtemp(prefix=f"stat_{name}_")
+        subprocess.run([os.path.join(ROOT, "oracle", "_ref", "generator"), str(sp["n"]), str(sp["rho"])], cwd=d0,
+                       check=True, capture_output=True)
+        start = os.path.join(d0, "generated.rrr")
+        if sp.get("equilibrate"):
+            d = os.path.join(d0, "equil")
+            os.makedirs(d)
+            keys = {"Dynamics": 0, "Seed": 999, "Disp_max": 0.05, "Theta_max": 0.1, "KF_delta": 0.12,
+                    "KF_cosmax": 0.92, "Initial_conditions_file": start, "Steps": sp["equilibrate"],
+                    "Print_every": 10000, "Save_every": sp["equilibrate"], "Configuration_folder": "."}
+            keys.update(sp["keys"])
+            with open(os.path.join(d, "input"), "w") as f:
+                f.write("".join(f"{a} = {b}\n" for a, b in keys.items()))
+            subprocess.run([exe, "input"], cwd=d, check=True, capture_output=True)
+            start = os.path.join(HERE, f"stat_{name}_start.rrr")
+            with open(os.path.join(d, "last.rrr")) as f, open(start, "w") as g:
+                g.write(f.read())
+        procs = []
+        for k in range(STAT_RUNS):
+            d = os.path.join(d0, f"run{k}")
+            os.makedirs(d)
+            keys = {"Dynamics": 0, "Seed": 1000 + k, "Disp_max": 0.05, "Theta_max": 0.1, "KF_delta": 0.12,
+                    "KF_cosmax": 0.92, "Initial_conditions_file": start,
+                    "Steps": STAT_SWEEPS, "Print_every": STAT_PRINT, "Save_every": STAT_SWEEPS,
+                    "Configuration_folder": "."}
+            keys.update(sp["keys"])
+            with open(os.path.join(d, "input"), "w") as f:
+                f.write("".join(f"{a} = {b}\n" for a, b in keys.items()))
+            procs.append(subprocess.Popen([exe, "input"], cwd=d, stdout=subprocess.DEVNULL,
+                                          stderr=subprocess.DEVNULL))
+        for p in procs:
+            assert p.wait() == 0
+        e_runs, r_runs = [], []
+        for k in range(STAT_RUNS):
+            d = os.path.join(d0, f"run{k}")
+            e = np.loadtxt(os.path.join(d, "energy.dat"))[:, 1]
+            e_runs.append(e[int(STAT_DISCARD * len(e)):].mean())
+            if sp["keys"]["Ensemble"] == 5:
+                rho = np.loadtxt(os.path.join(d, "density.dat"))[:, 1]
+                r_runs.append(rho[int(STAT_DISCARD * len(rho)):].mean())
+        e_runs = np.array(e_runs)
+        res = dict(n=sp["n"], rho0=sp["rho"], keys=sp["keys"], runs=STAT_RUNS, sweeps=STAT_SWEEPS,
+                   start=os.path.basename(start) if sp.get("equilibrate") else None,
+                   print_every=STAT_PRINT, discard=STAT_DISCARD, E_per_N=float(e_runs.mean()),
+                   E_per_N_se=float(e_runs.std(ddof=1) / np.sqrt(STAT_RUNS)), E_runs=e_runs.tolist())
+        if r_runs:
+            r_runs = np.array(r_runs)
+            res.update(density=float(r_runs.mean()), density_se=float(r_runs.std(ddof=1) / np.sqrt(STAT_RUNS)),
+                       density_runs=r_runs.tolist())
+        out[name] = res
+        print(name, {k: v for k, v in res.items() if not k.endswith("_runs")})
+    with open(os.path.join(HERE, "stats_ref.json"), "w") as f:
+        json.dump(out, f, indent=1)
+
+
 if __name__ == "__main__":
     if sys.argv[1:] == ["host"]:
         host_runs()
+    elif sys.argv[1:] == ["stats"]:
+        stats_ref()
     else:
         main()
         host_runs()
+        stats_ref()

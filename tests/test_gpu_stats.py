@@ -1,0 +1,110 @@
+"""Equilibrium parity: the GPU sampler against the reference program's own
+runs (tests/golden/stats_ref.json, written by make_golden.py from 8
+independent reference runs per state point) within 3 combined standard
+errors, and NPT hard spheres against the Carnahan-Starling equation of state.
+
+The GPU side runs independent replicas of the same start in one batched
+context; each replica's mean over its samples after the reference's
+discarded fraction is one independent estimate, as each reference run's is.
+The state points equilibrate well within the discarded part (T = 0.2);
+colder ones keep relaxing for 10^5+ sweeps at N = 300, where a comparison of
+running means would compare relaxation speeds instead."""
+import math
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import golden
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GENERATOR = os.path.join(ROOT, "patchyparticles_amd", "bin", "generator_hip")
+
+
+@pytest.fixture(scope="module")
+def pp():
+    import patchyparticles_amd as pp
+    pp.lib()
+    return pp
+
+
+def _reference_start(pp, tmp_path, n, rho):
+    """The reference generator's start, read back from its file (the box in
+    the file is rounded to 6 decimals, as the reference runs saw it)."""
+    subprocess.run([GENERATOR, str(n), str(rho)], cwd=tmp_path, check=True, capture_output=True)
+    _, box, r, O = pp.read_rrr(str(tmp_path / "generated.rrr"))
+    return r, O, box
+
+
+def _replica_means(pp, r, O, box, prm, replicas, sweeps, every, discard, trials_per_cell=0):
+    with pp.System.batch([r] * replicas, [O] * replicas, [box] * replicas, [prm] * replicas, seed=2024,
+                         trials_per_cell=trials_per_cell) as s:
+        e, rho = [], []
+        n = len(r)
+        for _ in range(sweeps // every):
+            s.do_ensemble(every)
+            c = s.counters_all()
+            e.append(-c["bonds"] / n)
+            rho.append(n / c["volume"])
+        for k in range(replicas):
+            b, ov, tot, anyo = s.energy(k)
+            assert not anyo
+    e, rho = np.array(e), np.array(rho)  # (samples, replicas)
+    cut = int(discard * len(e))
+    return e[cut:].mean(axis=0), rho[cut:].mean(axis=0)
+
+
+def _mean_se(x):
+    return float(np.mean(x)), float(np.std(x, ddof=1) / math.sqrt(len(x)))
+
+
+@pytest.mark.parametrize("name,trials_per_cell", [("nvt_rho03_T020", 0), ("nvt_rho03_T020", 4),
+                                                  ("nvt_rho05_T020", 0), ("nvt_rho05_T020", 8),
+                                                  ("npt_T020_P02", 0), ("npt_T020_P02", 4)])
+def test_equilibrium_matches_reference_runs(pp, tmp_path, name, trials_per_cell):
+    ref = golden("stats_ref.json")[name]
+    if ref.get("start"):
+        # a start the reference program equilibrated (committed input file)
+        _, box, r, O = pp.read_rrr(os.path.join(ROOT, "tests", "golden", ref["start"]))
+    else:
+        r, O, box = _reference_start(pp, tmp_path, ref["n"], ref["rho0"])
+    k = ref["keys"]
+    prm = dict(T=k["Temperature"], ensemble=k["Ensemble"], P=k.get("P", 0.0),
+               rescale_factor_max=k.get("rescale_factor_max", 0.0))
+    # with s trials per non-empty cell a step makes more than N attempts:
+    # fewer steps reach the same equilibrium
+    sweeps = ref["sweeps"] if trials_per_cell == 0 else ref["sweeps"] // 2
+    e_rep, rho_rep = _replica_means(pp, r, O, box, prm, 32, sweeps, ref["print_every"], ref["discard"],
+                                    trials_per_cell)
+    m, se = _mean_se(e_rep)
+    tol = 3 * math.hypot(se, ref["E_per_N_se"])
+    assert abs(m - ref["E_per_N"]) < tol, (m, se, ref["E_per_N"], ref["E_per_N_se"])
+    if "density" in ref:
+        m, se = _mean_se(rho_rep)
+        tol = 3 * math.hypot(se, ref["density_se"])
+        assert abs(m - ref["density"]) < tol, (m, se, ref["density"], ref["density_se"])
+
+
+def test_npt_hard_spheres_follow_carnahan_starling(pp, tmp_path):
+    """Patches that can never bond (cosmax = 1): hard spheres at T = 1,
+    P = 0.8.  The mean density solves beta P = rho (1 + e + e^2 - e^3) /
+    (1 - e)^3, e = pi rho / 6, to within 1.5 % (finite size and the equation's
+    own error are below that at this density)."""
+    T, P = 1.0, 0.8
+    lo, hi = 0.05, 0.8
+    for _ in range(80):
+        mid = 0.5 * (lo + hi)
+        e = math.pi * mid / 6
+        if mid * (1 + e + e * e - e ** 3) / (1 - e) ** 3 < P / T:
+            lo = mid
+        else:
+            hi = mid
+    rho_cs = 0.5 * (lo + hi)
+    r, O, box = _reference_start(pp, tmp_path, 256, 0.3)
+    prm = dict(T=T, ensemble=5, P=P, rescale_factor_max=0.05, kf_cosmax=1.0, disp_max=0.2)
+    _, rho_rep = _replica_means(pp, r, O, box, prm, 32, 12000, 50, 0.4)
+    m, se = _mean_se(rho_rep)
+    assert abs(m - rho_cs) < 0.015 * rho_cs + 3 * se, (m, se, rho_cs)
