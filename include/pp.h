@@ -35,6 +35,10 @@
  *                                 snapshot, without a context (MC.c:272-309)
  *   pp_generate                   the generator program's random insertion
  *                                 (generator.c:10-124)
+ *   pp_autotune, pp_set_trace/pp_get_trace, pp_set_timing/pp_get_timing,
+ *   pp_get_grid                   no reference counterpart (launch-shape
+ *                                 tuning, per-sweep energy series, kernel
+ *                                 timing, grid introspection)
  */
 #ifndef PP_H
 #define PP_H
@@ -169,6 +173,14 @@ pp_status pp_get_trace(pp_ctx *ctx, int rep, long long *bonds, int capacity, int
  * name: "pass", "rebuild", "energy", "volume" or NULL for the sum. */
 pp_status pp_set_timing(pp_ctx *ctx, int every);
 pp_status pp_get_timing(pp_ctx *ctx, const char *name, long long *launches, double *total_ms);
+
+/* Autotuner (no reference counterpart): runs `sweeps` sweeps (after one
+ * warm-up sweep) with each feasible launch shape -- threads per cell 32 / 64
+ * times workgroups of 64 / 128 / 256 threads -- and keeps the fastest for the
+ * context.  The sweeps are real: the trajectory does not depend on the launch
+ * shape, so tuning only advances the simulation.  Outputs may be NULL.
+ * Per-pass launch form only. */
+pp_status pp_autotune(pp_ctx *ctx, int sweeps, int *threads_per_cell, int *block_size, double *us_per_sweep);
 
 /* Grid of replica rep: cells per side and cell side lengths. */
 pp_status pp_get_grid(pp_ctx *ctx, int rep, int n_side[3], double side[3]);

@@ -6,6 +6,7 @@ ABI of include/pp.h (libpp_hip.so).
 from ._lib import EXPORTS, LIB_PATH, PPError, energy_snapshot, generate, lib
 from .rrr import read_rrr, write_rrr
 from .system import NPT, NVT, System
+from .batch import uneven_replicas
 
 __all__ = ["System", "NVT", "NPT", "energy_snapshot", "generate", "read_rrr", "write_rrr", "PPError", "lib", "LIB_PATH",
-           "EXPORTS"]
+           "EXPORTS", "uneven_replicas"]

@@ -18,7 +18,7 @@ EXPORTS = (
     "pp_create", "pp_create_batch", "pp_sweeps", "pp_energy", "pp_check_energy", "pp_download",
     "pp_upload", "pp_get_counters", "pp_get_counters_all", "pp_reset_counters", "pp_num_replicas", "pp_num_particles",
     "pp_energy_snapshot", "pp_generate", "pp_set_trace", "pp_get_trace", "pp_set_timing", "pp_get_timing", "pp_get_grid", "pp_destroy",
-    "pp_strerror", "pp_last_error",
+    "pp_strerror", "pp_last_error", "pp_autotune",
 )
 
 PP_OK = 0
@@ -122,6 +122,8 @@ def lib():
     L.pp_set_trace.restype = st
     L.pp_get_trace.argtypes = [_vp, C.c_int, _vp, C.c_int, C.POINTER(C.c_int)]
     L.pp_get_trace.restype = st
+    L.pp_autotune.argtypes = [_vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_double)]
+    L.pp_autotune.restype = st
     L.pp_set_timing.argtypes = [_vp, C.c_int]
     L.pp_set_timing.restype = st
     L.pp_get_timing.argtypes = [_vp, C.c_char_p, C.POINTER(C.c_longlong), C.POINTER(C.c_double)]

@@ -437,52 +437,119 @@ pp_status pp_create_batch(int n_rep, const pp_params *params, const pp_options *
 	return create_impl(n_rep, params, opts, n, r, O, box, false, out);
 }
 
+}  // extern "C"
+
+namespace {
+
+// Enqueue sweep c->sweep on the context's stream (no synchronisation).
+pp_status enqueue_sweep(pp_ctx *c) {
+	const uint64_t sw = (uint64_t) c->sweep;
+	c->timing = c->timing_every > 0 && c->sweep % c->timing_every == 0;
+	hipEvent_t a{};
+	tic(c, TK_REBUILD, &a);
+	long long tp = -1;
+	if(c->trace_len >= 0 && c->trace_len < c->d.trace_cap) tp = c->trace_len++;
+	launch_rebuild(c->d, REB_SHIFT, sw, tp, c->stream);
+	toc(c, TK_REBUILD, a);
+	if(c->coop_blocks > 0) {
+		hipEvent_t b{};
+		tic(c, TK_PASS, &b);
+		HIPCHECK(launch_sweep_passes(c->d, sw, c->T, c->block, c->coop_blocks, c->stream));
+		toc(c, TK_PASS, b);
+	}
+	else
+		for(int k = 0; k < 8; k++) {
+			hipEvent_t b{};
+			tic(c, TK_PASS, &b);
+			launch_pass(c->d, sw, k, c->T, c->block, c->stream);
+			toc(c, TK_PASS, b);
+		}
+	if(c->any_npt) {
+		// one volume attempt per N particle attempts on average, as do_NPT
+		// (MC.c:60-68): with s trials per non-empty cell a step makes
+		// about s x (non-empty cells) <= s N attempts, and s volume attempts
+		const int nvol = c->d.trials_per_cell > 0 ? c->d.trials_per_cell : 1;
+		for(int k = 0; k < nvol; k++) {
+			hipEvent_t v{};
+			tic(c, TK_VOLUME, &v);
+			launch_vol_propose(c->d, sw, k, c->stream);
+			launch_rebuild(c->d, REB_TRIAL, sw, -1, c->stream);
+			launch_energy(c->d, 1, 0, c->stream);
+			launch_vol_decide(c->d, sw, k, c->stream);
+			toc(c, TK_VOLUME, v);
+		}
+	}
+	c->sweep++;
+	if(c->timing && c->pending.size() > 4096) TRY(collect_timing(c));
+	return PP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
 pp_status pp_sweeps(pp_ctx *c, long long nsw) {
 	if(!c) return fail(PP_ERR_ARG, "ctx is NULL");
 	if(nsw < 0) return fail(PP_ERR_ARG, "negative sweep count");
 	HIPCHECK(hipSetDevice(c->device));
-	for(long long s = 0; s < nsw; s++) {
-		const uint64_t sw = (uint64_t) c->sweep;
-		c->timing = c->timing_every > 0 && c->sweep % c->timing_every == 0;
-		hipEvent_t a{};
-		tic(c, TK_REBUILD, &a);
-		long long tp = -1;
-		if(c->trace_len >= 0 && c->trace_len < c->d.trace_cap) tp = c->trace_len++;
-		launch_rebuild(c->d, REB_SHIFT, sw, tp, c->stream);
-		toc(c, TK_REBUILD, a);
-		if(c->coop_blocks > 0) {
-			hipEvent_t b{};
-			tic(c, TK_PASS, &b);
-			HIPCHECK(launch_sweep_passes(c->d, sw, c->T, c->block, c->coop_blocks, c->stream));
-			toc(c, TK_PASS, b);
-		}
-		else
-			for(int k = 0; k < 8; k++) {
-				hipEvent_t b{};
-				tic(c, TK_PASS, &b);
-				launch_pass(c->d, sw, k, c->T, c->block, c->stream);
-				toc(c, TK_PASS, b);
-			}
-		if(c->any_npt) {
-			// one volume attempt per N particle attempts on average, as do_NPT
-			// (MC.c:60-68): with s trials per non-empty cell a step makes
-			// about s x (non-empty cells) <= s N attempts, and s volume attempts
-			const int nvol = c->d.trials_per_cell > 0 ? c->d.trials_per_cell : 1;
-			for(int k = 0; k < nvol; k++) {
-				hipEvent_t v{};
-				tic(c, TK_VOLUME, &v);
-				launch_vol_propose(c->d, sw, k, c->stream);
-				launch_rebuild(c->d, REB_TRIAL, sw, -1, c->stream);
-				launch_energy(c->d, 1, 0, c->stream);
-				launch_vol_decide(c->d, sw, k, c->stream);
-				toc(c, TK_VOLUME, v);
-			}
-		}
-		c->sweep++;
-		if(c->timing && c->pending.size() > 4096) TRY(collect_timing(c));
-	}
+	for(long long s = 0; s < nsw; s++) TRY(enqueue_sweep(c));
 	c->timing = c->timing_every > 0;
 	TRY(collect_timing(c));
+	return check_err(c);
+}
+
+pp_status pp_autotune(pp_ctx *c, int sweeps, int *threads_per_cell, int *block_size, double *us_per_sweep) {
+	if(!c) return fail(PP_ERR_ARG, "ctx is NULL");
+	if(sweeps < 1) return fail(PP_ERR_ARG, "sweeps per candidate must be >= 1");
+	if(c->coop_blocks > 0) return fail(PP_ERR_UNSUPPORTED, "the autotuner works on the per-pass launch form");
+	HIPCHECK(hipSetDevice(c->device));
+	TRY(collect_timing(c));
+	const int saved_every = c->timing_every;
+	c->timing_every = 0;
+	hipEvent_t e0, e1;
+	HIPCHECK(hipEventCreate(&e0));
+	HIPCHECK(hipEventCreate(&e1));
+	int best_T = c->T, best_block = c->block;
+	float best = -1.f;
+	pp_status st = PP_OK;
+	// every candidate runs real sweeps: the trajectory does not depend on the
+	// launch shape, so nothing is wasted
+	const int Ts[2] = {32, 64}, blocks[3] = {64, 128, 256};
+	for(int ti = 0; ti < 2 && st == PP_OK; ti++)
+		for(int bi = 0; bi < 3 && st == PP_OK; bi++) {
+			if(pass_lds_bytes(Ts[ti], blocks[bi]) > MAX_LDS_BYTES || blocks[bi] < Ts[ti]) continue;
+			c->T = Ts[ti];
+			c->block = blocks[bi];
+			st = enqueue_sweep(c);  // warm-up of this shape
+			if(st != PP_OK) break;
+			if(hipEventRecord(e0, c->stream) != hipSuccess) {
+				st = fail(PP_ERR_HIP, "hipEventRecord failed");
+				break;
+			}
+			for(int k = 0; k < sweeps && st == PP_OK; k++) st = enqueue_sweep(c);
+			if(st != PP_OK) break;
+			float ms = 0.f;
+			if(hipEventRecord(e1, c->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+			   hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+				st = fail(PP_ERR_HIP, "timing a candidate failed");
+				break;
+			}
+			if(best < 0.f || ms < best) {
+				best = ms;
+				best_T = c->T;
+				best_block = c->block;
+			}
+		}
+	(void) hipEventDestroy(e0);
+	(void) hipEventDestroy(e1);
+	c->T = best_T;
+	c->block = best_block;
+	c->timing_every = saved_every;
+	c->timing = saved_every > 0;
+	TRY(st);
+	if(threads_per_cell) *threads_per_cell = best_T;
+	if(block_size) *block_size = best_block;
+	if(us_per_sweep) *us_per_sweep = best * 1e3 / sweeps;
 	return check_err(c);
 }
 

@@ -154,6 +154,14 @@ class System:
         check(lib().pp_get_trace(self._h, rep, _ptr(out), cap, C.byref(cnt)))
         return -out[:cnt.value]
 
+    def autotune(self, sweeps=20):
+        """Pick the fastest launch shape (threads per cell, workgroup size) by
+        timing `sweeps` real sweeps with each; returns (threads_per_cell,
+        block_size, microseconds per sweep)."""
+        t, b, us = C.c_int(0), C.c_int(0), C.c_double(0)
+        check(lib().pp_autotune(self._h, int(sweeps), C.byref(t), C.byref(b), C.byref(us)))
+        return t.value, b.value, us.value
+
     def set_timing(self, every=1):
         """Bracket the launches of every ``every``-th sweep with HIP events
         (0 or False: off).  Resets the accumulated times."""

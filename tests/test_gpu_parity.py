@@ -236,3 +236,39 @@ def test_fixed_trials_per_cell(pp, oracle, trials_per_cell):
     assert np.array_equal(b, ob) and c["bonds"] == tot == otot
     assert c["tries"][0] % trials_per_cell == 0
     assert 0.3 < c["accepted"][0] / c["tries"][0] < 0.99
+
+
+def test_autotune_keeps_the_trajectory(pp, oracle):
+    """The tuner's sweeps are real sweeps: a tuned context ends where an
+    untuned one with the same seed and sweep count ends, bit for bit."""
+    r, O, box = _start(oracle, 3000, 0.4, 9)
+    with pp.System(r, O, box, T=0.11, seed=5) as a:
+        t, b, us = a.autotune(2)
+        done = a.counters()["sweeps"]
+        a.do_ensemble(5)
+        sa = a.state()
+    assert t in (32, 64) and b in (64, 128, 256) and us > 0 and done >= 6
+    with pp.System(r, O, box, T=0.11, seed=5) as c:
+        c.do_ensemble(done + 5)
+        sc = c.state()
+    for x, y in zip(sa, sc):
+        assert np.array_equal(x, y)
+
+
+@pytest.mark.parametrize("trials_per_cell", [0, 4])
+def test_uneven_batch(pp, oracle, trials_per_cell):
+    """24 replicas of uneven N, T and density in one context: every replica's
+    tracked energy equals a full recompute, without overlaps, and its bonds
+    equal the oracle's."""
+    rs, Os, bs, ps = pp.uneven_replicas(24, seed=3, n_range=(200, 1500))
+    with pp.System.batch(rs, Os, bs, ps, seed=9, trials_per_cell=trials_per_cell) as s:
+        s.autotune(1)
+        s.do_ensemble(60)
+        c = s.counters_all()
+        for k in range(len(rs)):
+            b, ov, tot, anyo = s.energy(k)
+            assert not anyo and tot == c["bonds"][k], k
+            rr, OO, bb = s.state(k)
+            ob, oov, otot = oracle.energy(rr, OO, bb)
+            assert np.array_equal(b, ob) and otot == tot, k
+        assert (c["tries"][:, 0] > 0).all()
