@@ -39,6 +39,12 @@
  *   pp_get_grid                   no reference counterpart (launch-shape
  *                                 tuning, per-sweep energy series, kernel
  *                                 timing, grid introspection)
+ *   pp_net_*, pp_get_learn_counters
+ *                                 no reference counterpart: the learned
+ *                                 proposal of the rototranslation move, a
+ *                                 network-driven alternative to the uniform
+ *                                 displacement/rotation of MC_move_rototranslate
+ *                                 (MC.c:138-161) inside the same Metropolis test
  */
 #ifndef PP_H
 #define PP_H
@@ -170,7 +176,7 @@ pp_status pp_get_trace(pp_ctx *ctx, int rep, long long *bonds, int capacity, int
 /* Kernel timing with HIP events on the context's stream (off by default).
  * every = 0: off; every = k >= 1: the launches of every k-th sweep (and every
  * energy call) are bracketed by events.  Resets the accumulated times.
- * name: "pass", "rebuild", "energy", "volume" or NULL for the sum. */
+ * name: "pass", "rebuild", "energy", "volume", "learned" or NULL for the sum. */
 pp_status pp_set_timing(pp_ctx *ctx, int every);
 pp_status pp_get_timing(pp_ctx *ctx, const char *name, long long *launches, double *total_ms);
 
@@ -184,6 +190,69 @@ pp_status pp_autotune(pp_ctx *ctx, int sweeps, int *threads_per_cell, int *block
 
 /* Grid of replica rep: cells per side and cell side lengths. */
 pp_status pp_get_grid(pp_ctx *ctx, int rep, int n_side[3], double side[3]);
+
+/* ---- learned proposal (no reference counterpart) --------------------------
+ * A move of particle p: the descriptor of its neighbourhood (the 12 nearest
+ * particles within 1.5 among the 27 cells around p's sweep cell: separation in
+ * p's body frame and the quaternion of the relative orientation) -> an MLP
+ * 84 (padded to 96) -> 128 -> 128 -> 128 -> 26 (ReLU; BF16 weights, FP32
+ * accumulation) -> a Gaussian displacement in p's body frame and a
+ * 4-component von Mises-Fisher mixture over the rotation axis; the angle is
+ * uniform in [0, theta_max).  The move is accepted with
+ * min(1, exp(-dE/T) q(x'->x) / q(x->x')), so the stationary distribution is
+ * the Boltzmann one of the Kern-Frenkel model, as with the plain move.
+ *
+ * Weights: layer l of W is an nn_k(l) x nn_n(l) row-major block (inputs x
+ * outputs; k = 96, 128, 128, 128; n = 128, 128, 128, 32), the blocks one after
+ * another (PP_NET_WEIGHTS floats); biases likewise (PP_NET_BIASES floats).
+ * Rows of padded inputs (84..95) and columns of padded outputs (26..31) are
+ * ignored by the forward pass only if kept zero; pp_net_create zeroes them.
+ * Output layout: mu[3], log sigma[3], then per component c < 4: direction[3]
+ * (normalised by the density), log kappa, logit. */
+#define PP_NET_INPUTS 84
+#define PP_NET_OUTPUTS 26
+#define PP_NET_WEIGHTS 49152
+#define PP_NET_BIASES 416
+
+typedef struct pp_net pp_net;
+
+/* A network on `device` with Glorot-uniform weights from `seed`, its last
+ * layer scaled so that the untrained proposal is close to a small isotropic
+ * move (sigma 0.03). */
+pp_status pp_net_create(int device, uint64_t seed, pp_net **out);
+pp_status pp_net_set(pp_net *net, const float *w, const float *b);
+pp_status pp_net_get(pp_net *net, float *w, float *b);
+/* Forward pass of m descriptors x[m][PP_NET_INPUTS] (rounded to BF16 as the
+ * sampler does) into y[m][PP_NET_OUTPUTS]. */
+pp_status pp_net_forward(pp_net *net, int m, const float *x, float *y);
+/* Mean over m rows of -log q(target | x), target[m][6] = body-frame
+ * displacement[3] and unit rotation axis[3], and its gradient with respect to
+ * the weights (gw: PP_NET_WEIGHTS, gb: PP_NET_BIASES; outputs may be NULL). */
+pp_status pp_net_loss_grad(pp_net *net, int m, const float *x, const float *target, double *loss, float *gw,
+                           float *gb);
+void pp_net_destroy(pp_net *net);
+/* Descriptors (as the learned move computes them, BF16 values) of particles
+ * idx[0..m) of a configuration of n particles in `box`, every other particle a
+ * candidate neighbour: x[m][PP_NET_INPUTS]. */
+pp_status pp_net_descriptor(int device, int n, const double *r, const double *O, const double box[3], int m,
+                            const int *idx, float *x);
+
+/* Use `net` for a share `learned_fraction` of the colour passes of every
+ * later sweep (chosen per pass from the seed, the sweep and the pass index;
+ * the other passes are the plain ones).  A learned pass moves one member of
+ * each non-empty cell.  net NULL detaches.  The network must outlive the
+ * attachment. */
+pp_status pp_net_attach(pp_ctx *ctx, pp_net *net, double learned_fraction, double theta_max);
+/* Train `net` on the context's replicas: each step draws `batch` particles,
+ * perturbs each (Gaussian displacement of deviation pos_noise, rotation by an
+ * angle uniform in [0, theta_max) about a uniform axis) and takes one Adam
+ * step (rate lr) on the mean -log q of the move back; `sweeps_between` sweeps
+ * run between steps.  Outputs the mean loss of the first and the last batch
+ * (before their update); outputs may be NULL. */
+pp_status pp_net_train(pp_ctx *ctx, pp_net *net, int steps, int batch, double lr, double pos_noise,
+                       double theta_max, int sweeps_between, double *loss_first, double *loss_last);
+/* Learned moves tried / accepted by replica rep since the last reset. */
+pp_status pp_get_learn_counters(pp_ctx *ctx, int rep, long long *tries, long long *accepted);
 
 void pp_destroy(pp_ctx *ctx);
 const char *pp_strerror(pp_status s);

@@ -19,7 +19,15 @@ EXPORTS = (
     "pp_upload", "pp_get_counters", "pp_get_counters_all", "pp_reset_counters", "pp_num_replicas", "pp_num_particles",
     "pp_energy_snapshot", "pp_generate", "pp_set_trace", "pp_get_trace", "pp_set_timing", "pp_get_timing", "pp_get_grid", "pp_destroy",
     "pp_strerror", "pp_last_error", "pp_autotune",
+    "pp_net_create", "pp_net_set", "pp_net_get", "pp_net_forward", "pp_net_loss_grad", "pp_net_destroy",
+    "pp_net_attach", "pp_net_train", "pp_get_learn_counters", "pp_net_descriptor",
 )
+
+# sizes of the proposal network (PP_NET_* in include/pp.h)
+NET_INPUTS = 84
+NET_OUTPUTS = 26
+NET_WEIGHTS = 49152
+NET_BIASES = 416
 
 PP_OK = 0
 STATUS = {
@@ -130,6 +138,28 @@ def lib():
     L.pp_get_timing.restype = st
     L.pp_get_grid.argtypes = [_vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double)]
     L.pp_get_grid.restype = st
+    fp = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+    L.pp_net_create.argtypes = [C.c_int, C.c_uint64, C.POINTER(_vp)]
+    L.pp_net_create.restype = st
+    L.pp_net_set.argtypes = [_vp, fp, fp]
+    L.pp_net_set.restype = st
+    L.pp_net_get.argtypes = [_vp, _vp, _vp]
+    L.pp_net_get.restype = st
+    L.pp_net_forward.argtypes = [_vp, C.c_int, _vp, _vp]
+    L.pp_net_forward.restype = st
+    L.pp_net_loss_grad.argtypes = [_vp, C.c_int, fp, fp, C.POINTER(C.c_double), _vp, _vp]
+    L.pp_net_loss_grad.restype = st
+    L.pp_net_destroy.argtypes = [_vp]
+    L.pp_net_destroy.restype = None
+    L.pp_net_attach.argtypes = [_vp, _vp, C.c_double, C.c_double]
+    L.pp_net_attach.restype = st
+    L.pp_net_train.argtypes = [_vp, _vp, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_int,
+                               C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    L.pp_net_train.restype = st
+    L.pp_get_learn_counters.argtypes = [_vp, C.c_int, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]
+    L.pp_get_learn_counters.restype = st
+    L.pp_net_descriptor.argtypes = [C.c_int, C.c_int, _dp, _dp, _dp, C.c_int, _ip, _vp]
+    L.pp_net_descriptor.restype = st
     L.pp_destroy.argtypes = [_vp]
     L.pp_destroy.restype = None
     L.pp_strerror.argtypes = [st]
@@ -177,3 +207,14 @@ def generate(n, rho, seed=-1):
     box = np.zeros(3)
     check(lib().pp_generate(int(n), float(rho), int(seed), r, O, box))
     return r[:3 * n].reshape(n, 3), O[:9 * n].reshape(n, 3, 3), box
+
+
+def net_descriptor(r, O, box, idx, device=0):
+    """Descriptors of particles ``idx`` (every other particle a candidate), as
+    the learned move computes them: (len(idx), 84) float32 holding BF16 values."""
+    r = _arr(r)
+    O = _arr(O)
+    idx = np.ascontiguousarray(idx, np.int32)
+    x = np.zeros((len(idx), NET_INPUTS), np.float32)
+    check(lib().pp_net_descriptor(int(device), r.size // 3, r, O, _arr(box), len(idx), idx, _ptr(x)))
+    return x

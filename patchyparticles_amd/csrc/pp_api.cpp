@@ -16,6 +16,7 @@
 
 #include "../../include/pp.h"
 #include "pp_internal.h"
+#include "pp_net.h"
 
 using namespace ppd;
 
@@ -55,6 +56,12 @@ Base host_base_patches() {
 
 }  // namespace
 
+// the error message of the calling thread, set by the other translation units
+pp_status pp_set_error(pp_status s, const char *msg) {
+	g_err = msg;
+	return s;
+}
+
 struct pp_ctx {
 	int device = 0;
 	hipStream_t stream = nullptr;
@@ -72,8 +79,12 @@ struct pp_ctx {
 	bool timing = false;   // events around the launches being enqueued
 	int timing_every = 0;  // sweeps between timed sweeps (0: off)
 	std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> pending;  // (kind, events)
-	long long t_launch[4] = {0, 0, 0, 0};
-	double t_ms[4] = {0, 0, 0, 0};
+	long long t_launch[5] = {0, 0, 0, 0, 0};
+	double t_ms[5] = {0, 0, 0, 0, 0};
+	// learned proposal: the attached network (not owned) and the share of
+	// colour passes that use it
+	pp_net *net = nullptr;
+	double learned_fraction = 0.;
 	std::vector<void *> allocs;
 
 	~pp_ctx() {
@@ -88,7 +99,7 @@ struct pp_ctx {
 
 namespace {
 
-enum { TK_PASS = 0, TK_REBUILD = 1, TK_ENERGY = 2, TK_VOLUME = 3 };
+enum { TK_PASS = 0, TK_REBUILD = 1, TK_ENERGY = 2, TK_VOLUME = 3, TK_LEARN = 4, TK_KINDS = 5 };
 
 template <typename Tp>
 pp_status dev_alloc(pp_ctx *c, Tp **p, size_t n) {
@@ -441,6 +452,21 @@ pp_status pp_create_batch(int n_rep, const pp_params *params, const pp_options *
 
 namespace {
 
+// Whether colour pass k of sweep sw uses the learned proposal: a fixed
+// function of (seed, sweep, pass), true with probability `frac`, so the
+// choice is independent of the state (a mixture of two kernels that each
+// keep the Boltzmann distribution) and trajectories stay reproducible.
+bool learned_pass(uint64_t seed, uint64_t sw, int k, double frac) {
+	if(frac <= 0.) return false;
+	if(frac >= 1.) return true;
+	uint64_t z = seed ^ 0x6C8E9CF570932BD5ull;
+	z += (sw * 8 + (uint64_t) k + 1) * 0x9E3779B97F4A7C15ull;
+	z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+	z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+	z ^= z >> 31;
+	return (double) (z >> 11) * (1.0 / 9007199254740992.0) < frac;
+}
+
 // Enqueue sweep c->sweep on the context's stream (no synchronisation).
 pp_status enqueue_sweep(pp_ctx *c) {
 	const uint64_t sw = (uint64_t) c->sweep;
@@ -451,7 +477,7 @@ pp_status enqueue_sweep(pp_ctx *c) {
 	if(c->trace_len >= 0 && c->trace_len < c->d.trace_cap) tp = c->trace_len++;
 	launch_rebuild(c->d, REB_SHIFT, sw, tp, c->stream);
 	toc(c, TK_REBUILD, a);
-	if(c->coop_blocks > 0) {
+	if(c->coop_blocks > 0 && !c->net) {
 		hipEvent_t b{};
 		tic(c, TK_PASS, &b);
 		HIPCHECK(launch_sweep_passes(c->d, sw, c->T, c->block, c->coop_blocks, c->stream));
@@ -460,9 +486,16 @@ pp_status enqueue_sweep(pp_ctx *c) {
 	else
 		for(int k = 0; k < 8; k++) {
 			hipEvent_t b{};
-			tic(c, TK_PASS, &b);
-			launch_pass(c->d, sw, k, c->T, c->block, c->stream);
-			toc(c, TK_PASS, b);
+			if(c->net && learned_pass(c->d.seed, sw, k, c->learned_fraction)) {
+				tic(c, TK_LEARN, &b);
+				launch_learned_pass(c->d, c->net->d, sw, k, c->stream);
+				toc(c, TK_LEARN, b);
+			}
+			else {
+				tic(c, TK_PASS, &b);
+				launch_pass(c->d, sw, k, c->T, c->block, c->stream);
+				toc(c, TK_PASS, b);
+			}
 		}
 	if(c->any_npt) {
 		// one volume attempt per N particle attempts on average, as do_NPT
@@ -729,7 +762,8 @@ pp_status pp_reset_counters(pp_ctx *c) {
 	std::vector<unsigned long long> h;
 	TRY(pull_ctr(c, h));
 	for(int k = 0; k < c->R; k++)
-		for(int row : {(int) C_TRY_ROTO, (int) C_ACC_ROTO, (int) C_TRY_VOL, (int) C_ACC_VOL})
+		for(int row : {(int) C_TRY_ROTO, (int) C_ACC_ROTO, (int) C_TRY_VOL, (int) C_ACC_VOL, (int) C_TRY_LEARN,
+		               (int) C_ACC_LEARN})
 			for(int s = 0; s < NSLOT; s++) h[((size_t) k * NCTR + row) * NSLOT + s] = 0;
 	HIPCHECK(hipMemcpy(c->d.ctr, h.data(), h.size() * sizeof(unsigned long long), hipMemcpyHostToDevice));
 	return PP_OK;
@@ -805,7 +839,7 @@ pp_status pp_set_timing(pp_ctx *c, int every) {
 	if(every < 0) return fail(PP_ERR_ARG, "negative timing interval");
 	c->timing_every = every;
 	c->timing = every > 0;
-	for(int k = 0; k < 4; k++) {
+	for(int k = 0; k < TK_KINDS; k++) {
 		c->t_ms[k] = 0.;
 		c->t_launch[k] = 0;
 	}
@@ -815,11 +849,11 @@ pp_status pp_set_timing(pp_ctx *c, int every) {
 pp_status pp_get_timing(pp_ctx *c, const char *name, long long *launches, double *total_ms) {
 	if(!c) return fail(PP_ERR_ARG, "ctx is NULL");
 	TRY(collect_timing(c));
-	int lo = 0, hi = 4;
+	int lo = 0, hi = TK_KINDS;
 	if(name) {
-		static const char *names[4] = {"pass", "rebuild", "energy", "volume"};
+		static const char *names[TK_KINDS] = {"pass", "rebuild", "energy", "volume", "learned"};
 		lo = -1;
-		for(int k = 0; k < 4; k++)
+		for(int k = 0; k < TK_KINDS; k++)
 			if(!strcmp(name, names[k])) lo = k;
 		if(lo < 0) return fail(PP_ERR_ARG, "unknown kernel group '%s'", name);
 		hi = lo + 1;
@@ -833,6 +867,89 @@ pp_status pp_get_timing(pp_ctx *c, const char *name, long long *launches, double
 	if(launches) *launches = nl;
 	if(total_ms) *total_ms = ms;
 	return PP_OK;
+}
+
+pp_status pp_get_learn_counters(pp_ctx *c, int rep, long long *tries, long long *accepted) {
+	if(!c) return fail(PP_ERR_ARG, "ctx is NULL");
+	if(rep < 0 || rep >= c->R) return fail(PP_ERR_ARG, "replica %d out of range", rep);
+	HIPCHECK(hipSetDevice(c->device));
+	std::vector<unsigned long long> h;
+	TRY(pull_ctr(c, h));
+	if(tries) *tries = ctr_sum(h, rep, C_TRY_LEARN);
+	if(accepted) *accepted = ctr_sum(h, rep, C_ACC_LEARN);
+	return PP_OK;
+}
+
+pp_status pp_net_attach(pp_ctx *c, pp_net *net, double learned_fraction, double theta_max) {
+	if(!c) return fail(PP_ERR_ARG, "ctx is NULL");
+	if(!net) {
+		c->net = nullptr;
+		c->learned_fraction = 0.;
+		c->d.learn.on = 0;
+		return PP_OK;
+	}
+	if(net->device != c->device) return fail(PP_ERR_ARG, "network on device %d, context on device %d", net->device, c->device);
+	if(!(learned_fraction >= 0. && learned_fraction <= 1.)) return fail(PP_ERR_ARG, "learned_fraction must lie in [0, 1]");
+	if(!(theta_max > 0. && theta_max <= M_PI)) return fail(PP_ERR_ARG, "theta_max must lie in (0, pi]");
+	HIPCHECK(hipSetDevice(c->device));
+	LearnDev &L = c->d.learn;
+	if(!L.xf) {
+		const size_t m = (size_t) std::max(c->d.total_act, 1);
+		TRY(dev_alloc(c, &L.xf, m * NN_IN_PAD));
+		TRY(dev_alloc(c, &L.xr, m * NN_IN_PAD));
+		TRY(dev_alloc(c, &L.yf, m * NN_OUT_PAD));
+		TRY(dev_alloc(c, &L.yr, m * NN_OUT_PAD));
+		TRY(dev_alloc(c, &L.prop, m * NN_PROP_W));
+		TRY(dev_alloc(c, &L.slot, m));
+		HIPCHECK(hipMemsetAsync(L.xf, 0, m * NN_IN_PAD * sizeof(__bf16), c->stream));
+		HIPCHECK(hipMemsetAsync(L.xr, 0, m * NN_IN_PAD * sizeof(__bf16), c->stream));
+		HIPCHECK(hipStreamSynchronize(c->stream));
+	}
+	L.on = 1;
+	L.frac = (float) learned_fraction;
+	L.theta_max = theta_max;
+	c->net = net;
+	c->learned_fraction = learned_fraction;
+	return PP_OK;
+}
+
+pp_status pp_net_train(pp_ctx *c, pp_net *net, int steps, int batch, double lr, double pos_noise, double theta_max,
+                       int sweeps_between, double *loss_first, double *loss_last) {
+	if(!c || !net) return fail(PP_ERR_ARG, "NULL argument");
+	if(net->device != c->device) return fail(PP_ERR_ARG, "network on device %d, context on device %d", net->device, c->device);
+	if(steps < 0 || batch < 1 || batch > (1 << 20)) return fail(PP_ERR_ARG, "steps >= 0 and batch in [1, 2^20] required");
+	if(!(lr > 0.) || !(pos_noise > 0.) || !(theta_max > 0. && theta_max <= M_PI) || sweeps_between < 0)
+		return fail(PP_ERR_ARG, "lr > 0, pos_noise > 0, theta_max in (0, pi] and sweeps_between >= 0 required");
+	HIPCHECK(hipSetDevice(c->device));
+	TRY((pp_status) net_reserve(net, batch));
+	HIPCHECK(hipStreamSynchronize(net->stream));
+	std::vector<float> hl(batch);
+	auto mean_loss = [&](double *out) -> pp_status {
+		HIPCHECK(hipMemcpyAsync(hl.data(), net->loss, sizeof(float) * batch, hipMemcpyDeviceToHost, c->stream));
+		HIPCHECK(hipStreamSynchronize(c->stream));
+		double sum = 0.;
+		for(float v : hl) sum += v;
+		*out = sum / batch;
+		return PP_OK;
+	};
+	const uint64_t tseed = c->d.seed ^ 0x2545F4914F6CDD1Dull;
+	for(int s = 0; s < steps; s++) {
+		if(s > 0)
+			for(int k = 0; k < sweeps_between; k++) TRY(enqueue_sweep(c));
+		// the cell ranges of the current state (an NPT trial may have left
+		// those of its trial grid)
+		rebuild_now(c);
+		launch_train_samples(c->d, tseed, (uint64_t) net->step, batch, pos_noise, theta_max, net->x, net->target,
+		                     c->stream);
+		launch_net_loss_grad(net->d, net->x, net->target, batch, net->h, net->y, net->dy, net->dh, net->loss, c->stream);
+		if(s == 0 && loss_first) TRY(mean_loss(loss_first));
+		if(s == steps - 1 && loss_last) TRY(mean_loss(loss_last));
+		net->step++;
+		launch_net_adam(net->d, net->step, (float) lr, c->stream);
+	}
+	HIPCHECK(hipStreamSynchronize(c->stream));
+	TRY(collect_timing(c));
+	return check_err(c);
 }
 
 pp_status pp_get_grid(pp_ctx *c, int rep, int n_side[3], double side[3]) {

@@ -1,0 +1,182 @@
+// pp_device.hpp -- device helpers shared by the gfx950 kernels of
+// libpp_hip.so (pp_kernels.hip: rebuild, colour passes, energy; pp_learn.hip:
+// the learned proposal): team operations, counters, trial draws and the
+// reference-order pair tests on a trial.
+#pragma once
+#pragma clang fp contract(off)
+
+#include "pp_internal.h"
+
+namespace ppd {
+
+// ------------------------------------------------------------------------
+// helpers
+
+// Workgroups are dispatched round-robin over the 8 XCDs; give each XCD a
+// contiguous run of logical blocks so that neighbouring cells share an L2.
+// nb must be a multiple of 8.
+__device__ __forceinline__ int xcd_remap(int b, int nb) {
+	int per = nb >> 3;
+	return (b & 7) * per + (b >> 3);
+}
+
+template <int T>
+__device__ __forceinline__ int team_sum(int v) {
+#pragma unroll
+	for(int o = T / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, T);
+	return v;
+}
+
+// Sum over a team of small non-negative integers (< 8): one ballot per bit.
+template <int T>
+__device__ __forceinline__ int team_sum_small(int v) {
+	const int off = (threadIdx.x & 63) & ~(T - 1);
+	const unsigned long long tm = (T == 64) ? ~0ull : ((1ull << T) - 1ull);
+	int s = 0;
+#pragma unroll
+	for(int b = 0; b < 3; b++) s += __popcll((__ballot((v >> b) & 1) >> off) & tm) << b;
+	return s;
+}
+
+template <int T>
+__device__ __forceinline__ bool team_any(bool p) {
+	unsigned long long m = __ballot(p);
+	int off = (threadIdx.x & 63) & ~(T - 1);
+	unsigned long long tm = (T == 64) ? ~0ull : ((1ull << T) - 1ull);
+	return ((m >> off) & tm) != 0ull;
+}
+
+// When a team is a whole wavefront (T = 64) every team-uniform value is
+// wave-uniform: readfirstlane moves it to scalar registers.
+template <int T>
+__device__ __forceinline__ int uni(int v) {
+	if constexpr(T == 64) return __builtin_amdgcn_readfirstlane(v);
+	else return v;
+}
+
+template <int T>
+__device__ __forceinline__ double unid(double v) {
+	if constexpr(T == 64) {
+		long long b = __double_as_longlong(v);
+		int lo = __builtin_amdgcn_readfirstlane((int) (b & 0xffffffffll));
+		int hi = __builtin_amdgcn_readfirstlane((int) (b >> 32));
+		return __longlong_as_double(((long long) hi << 32) | (unsigned) lo);
+	}
+	else return v;
+}
+
+// Lanes of one team live in one wavefront: order their LDS traffic.
+__device__ __forceinline__ void team_sync() {
+	__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+	__builtin_amdgcn_wave_barrier();
+	__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ unsigned long long *ctr_at(unsigned long long *ctr, int r, int row, int slot) {
+	return ctr + ((long long) (r * NCTR + row) * NSLOT + slot);
+}
+
+__device__ __forceinline__ int wrapi(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
+
+// Uniform axis on the unit sphere by Marsaglia's rejection method (the
+// method of random_vector_on_sphere, utils.c:115-131), fed by Philox blocks
+// (two candidate pairs per block) until a pair falls inside the unit disc.
+// Each pair is accepted with probability pi/4; after AXIS_BLOCKS blocks
+// (chance 0.215^128) the axis is +z or -z by a random bit, which keeps the
+// axis law symmetric under a -> -a and so the proposal symmetric.
+constexpr uint32_t AXIS_BLOCKS = 64;
+
+__device__ __forceinline__ void sphere_axis(uint64_t seed, int r, uint64_t sweep, uint32_t cell, uint32_t col,
+                                            uint32_t t, double a[3]) {
+	for(uint32_t blk = 0; blk < AXIS_BLOCKS; blk++) {
+		u32x4 v = rng(seed, r, sweep, cell | (blk << 24), TAG_AXIS, col, t);
+		const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+		for(int h = 0; h < 2; h++) {
+			double u1 = 1. - 2. * u01(w[2 * h]), u2 = 1. - 2. * u01(w[2 * h + 1]);
+			double q = u1 * u1 + u2 * u2;
+			if(q < 1.) {
+				double f = 2. * __builtin_sqrt(1. - q);
+				a[0] = u1 * f;
+				a[1] = u2 * f;
+				a[2] = 1. - 2. * q;
+				return;
+			}
+		}
+	}
+	u32x4 v = rng(seed, r, sweep, cell, TAG_AXIS_FALLBACK, col, t);
+	a[0] = 0.;
+	a[1] = 0.;
+	a[2] = (v.x & 1u) ? 1. : -1.;
+}
+
+// Trial t of cell cloc (colour col) in sweep `sweep`: member rank among
+// cnt, displacement uniform in +-disp_max/2 per axis (MC.c:317-319), rotation
+// by an angle uniform in [0, theta_max) about an axis uniform on the sphere
+// (MC.c:321-325, utils.c:186-202), and the acceptance draw (MC.c:332).
+__device__ __forceinline__ void make_trial(const DevState &d, const Rep &rp, int r, uint64_t sweep, int cloc, int col,
+                                           int t, int cnt, double *out) {
+	u32x4 ra = rng(d.seed, r, sweep, (uint32_t) cloc, TAG_TRIAL_A, (uint32_t) col, (uint32_t) t);
+	u32x4 rb = rng(d.seed, r, sweep, (uint32_t) cloc, TAG_TRIAL_B, (uint32_t) col, (uint32_t) t);
+	out[0] = (double) __umulhi(ra.x, (uint32_t) cnt);
+	out[1] = (u01(ra.y) - 0.5) * rp.disp_max;
+	out[2] = (u01(ra.z) - 0.5) * rp.disp_max;
+	out[3] = (u01(ra.w) - 0.5) * rp.disp_max;
+	double axis[3], st, ct, R[9];
+	sphere_axis(d.seed, r, sweep, (uint32_t) cloc, (uint32_t) col, (uint32_t) t, axis);
+	sincos(u01(rb.x) * rp.theta_max, &st, &ct);
+	rotation_matrix(axis, st, ct, R);
+#pragma unroll
+	for(int c = 0; c < 9; c++) out[4 + c] = R[c];
+	out[13] = u01(rb.w);
+}
+
+// Trial geometry shared by the helpers of one team.
+struct Trial {
+	double px, py, pz, nx, ny, nz;
+	double Lx, Ly, Lz, iLx, iLy, iLz, rc2, cosmax;
+	double c2;  // cosmax^2 for the filtered angular tests, -1: exact tests only
+};
+
+__device__ __forceinline__ bool overlaps_new(const Trial &t, double qx, double qy, double qz) {
+	double dx = kf_min_image(qx - t.nx, t.Lx, t.iLx);
+	double dy = kf_min_image(qy - t.ny, t.Ly, t.iLy);
+	double dz = kf_min_image(qz - t.nz, t.Lz, t.iLz);
+	return (dx * dx + dy * dy + dz * dz) < 1.;
+}
+
+// Bonds of neighbour q with the old and the new position (one load of q for
+// both: the particle stays in its cell, so the neighbourhood is the same),
+// with the reference's arithmetic (MC.c:242-262).  q's orientation comes from
+// load_o only when one of the two passes the p-side test.
+template <class LoadO>
+__device__ __forceinline__ void bonds_old_new_exact(const Trial &t, const double *Po, const double *Pn, double qx,
+                                                    double qy, double qz, LoadO load_o, int &bo, int &bn) {
+	double ox = kf_min_image(qx - t.px, t.Lx, t.iLx);
+	double oy = kf_min_image(qy - t.py, t.Ly, t.iLy);
+	double oz = kf_min_image(qz - t.pz, t.Lz, t.iLz);
+	double o2 = ox * ox + oy * oy + oz * oz;
+	double wx = kf_min_image(qx - t.nx, t.Lx, t.iLx);
+	double wy = kf_min_image(qy - t.ny, t.Ly, t.iLy);
+	double wz = kf_min_image(qz - t.nz, t.Lz, t.iLz);
+	double w2 = wx * wx + wy * wy + wz * wz;
+	bool po = false, pn = false;
+	if(o2 < t.rc2) {
+		kf_unit(ox, oy, oz, o2);
+		po = kf_any_p(ox, oy, oz, Po, t.cosmax);
+	}
+	if(w2 < t.rc2) {
+		kf_unit(wx, wy, wz, w2);
+		pn = kf_any_p(wx, wy, wz, Pn, t.cosmax);
+	}
+	if(po || pn) {
+		double Oq[9];
+		load_o(Oq);
+		bool ho, hn;
+		kf_any_q2(ox, oy, oz, po, wx, wy, wz, pn, Oq, t.cosmax, ho, hn);
+		bo += ho;
+		bn += hn;
+	}
+}
+
+}  // namespace ppd

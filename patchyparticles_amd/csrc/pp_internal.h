@@ -19,7 +19,57 @@ constexpr int REC_W = 64;      // ints of one cell record of a pass
 constexpr int ACC_TAB = 16;    // Metropolis factors exp(-k/T) tabulated per replica, k < ACC_TAB
 
 // per-replica counter rows, each NSLOT wide
-enum Ctr : int { C_TRY_ROTO = 0, C_ACC_ROTO, C_DBONDS, C_TRY_VOL, C_ACC_VOL, C_EBONDS, C_OVL, NCTR };
+enum Ctr : int { C_TRY_ROTO = 0, C_ACC_ROTO, C_DBONDS, C_TRY_VOL, C_ACC_VOL, C_EBONDS, C_OVL, C_TRY_LEARN, C_ACC_LEARN, NCTR };
+
+// ------------------------------------------------------------------------
+// learned proposal network: descriptor of up to NN_NB neighbours within
+// NN_RCUT (body-frame position, relative orientation quaternion), an MLP
+// NN_IN_PAD -> 128 -> 128 -> 128 -> NN_OUT_PAD (ReLU), BF16 weights, FP32
+// accumulation, and the outputs of a Gaussian displacement and a 4-component
+// von Mises-Fisher mixture over the rotation axis.
+constexpr int NN_NB = 12;
+constexpr int NN_FEAT = 7;
+constexpr int NN_IN = NN_NB * NN_FEAT;  // 84
+constexpr int NN_IN_PAD = 96;
+constexpr int NN_H = 128;
+constexpr int NN_OUT = 26;  // mu[3], log sigma[3], 4 x (direction[3], log kappa, logit)
+constexpr int NN_OUT_PAD = 32;
+constexpr int NN_MIX = 4;
+constexpr double NN_RCUT = 1.5;
+constexpr int NN_LAYERS = 4;
+__host__ __device__ constexpr int nn_k(int l) { return l == 0 ? NN_IN_PAD : NN_H; }
+__host__ __device__ constexpr int nn_n(int l) { return l == NN_LAYERS - 1 ? NN_OUT_PAD : NN_H; }
+__host__ __device__ constexpr int nn_woff(int l) { return l == 0 ? 0 : nn_woff(l - 1) + nn_k(l - 1) * nn_n(l - 1); }
+__host__ __device__ constexpr int nn_boff(int l) { return l == 0 ? 0 : nn_boff(l - 1) + nn_n(l - 1); }
+constexpr int NN_W = nn_woff(NN_LAYERS);  // 49152 weights
+constexpr int NN_B = nn_boff(NN_LAYERS);  // 416 biases
+// clamps of the proposal parameters (the density is that of the clamped values)
+constexpr float NN_LS_MIN = -6.2f;   // log sigma: 0.002 .. 0.6
+constexpr float NN_LS_MAX = -0.5f;
+constexpr float NN_LK_MIN = -3.0f;   // log kappa: 0.05 .. 3000
+constexpr float NN_LK_MAX = 8.0f;
+
+// Device state of one network: FP32 master weights and Adam moments, and the
+// BF16 operand fragments of the MFMA kernels (forward: W; backward: W^T).
+struct NetDev {
+	float *w, *b;
+	float *mw, *vw, *mb, *vb;
+	float *gw, *gb;
+	__bf16 *frag, *fragT;
+};
+
+constexpr int NN_PROP_W = 16;  // doubles of one drawn proposal
+
+// Per-context state of the learned passes (scratch sized for the largest list).
+struct LearnDev {
+	int on;
+	float frac;          // share of colour passes that use the learned proposal
+	double theta_max;    // rotation angle of a learned move, uniform in [0, theta_max)
+	__bf16 *xf, *xr;     // [total_act][NN_IN_PAD] descriptors at the current / proposed state
+	float *yf, *yr;      // [total_act][NN_OUT_PAD] network outputs
+	double *prop;        // [total_act][NN_PROP_W] p'[3], O'[9], log q forward, a[3] of the proposal
+	int *slot;           // [total_act] moving slot (-1: no move)
+};
 
 // device error bits
 enum Err : int { ERR_CELL_CAP = 1, ERR_BOX = 2, ERR_SCAN = 8, ERR_BARRIER = 16 };
@@ -114,6 +164,7 @@ struct DevState {
 	int32_t *out_bonds;
 	uint8_t *out_ovl;
 	int trials_per_cell;  // trials of a non-empty cell in one pass; 0: as many as it holds
+	LearnDev learn;
 	long long *trace;  // [R][trace_cap] bonded pairs at the start of each sweep (optional)
 	int trace_cap;
 	uint64_t seed;
@@ -126,6 +177,21 @@ void launch_fill_u64(unsigned long long *p, unsigned long long v, long long n, h
 // `sweep` and records the energy trace entry trace_pos (if >= 0)
 void launch_rebuild(const DevState &d, int mode, uint64_t sweep, long long trace_pos, hipStream_t s);
 void launch_pass(const DevState &d, uint64_t sweep, int pass, int T, int block, hipStream_t s);
+// learned proposal (pp_learn.hip)
+void launch_learned_pass(const DevState &d, const NetDev &net, uint64_t sweep, int pass, hipStream_t s);
+void launch_net_forward(const NetDev &net, const __bf16 *x, int m, const int *m_dev, float *y, __bf16 *h,
+                        hipStream_t s);
+void launch_net_pack(const NetDev &net, hipStream_t s);
+// m rows (padded to a multiple of 32, rows >= m of x zero): forward with the
+// hidden layers kept, per-row -log q (loss[m_pad]), backward, gradients of
+// the mean loss into net.gw / net.gb
+void launch_net_loss_grad(const NetDev &net, const __bf16 *x, const float *target, int m, __bf16 *h, float *y,
+                          __bf16 *dy, __bf16 *dh, float *loss, hipStream_t s);
+void launch_net_adam(const NetDev &net, int step, float lr, hipStream_t s);
+void launch_descriptor(int n, const double *r, const double *O, const Grid &g, int m, const int *idx, __bf16 *x,
+                       hipStream_t s);
+void launch_train_samples(const DevState &d, uint64_t seed, uint64_t step, int m, double pos_noise, double theta_max,
+                          __bf16 *x, float *target, hipStream_t s);
 // the 8 passes of a sweep in one cooperative launch of `blocks` workgroups
 hipError_t launch_sweep_passes(const DevState &d, uint64_t sweep, int T, int block, int blocks, hipStream_t s);
 int sweep_blocks_max(int T, int block, int cus);  // co-resident workgroups of that launch

@@ -8,7 +8,7 @@ import ctypes as C
 
 import numpy as np
 
-from ._lib import Counters, Options, Params, _arr, _ptr, check, lib
+from ._lib import NET_BIASES, NET_INPUTS, NET_OUTPUTS, NET_WEIGHTS, Counters, Options, Params, _arr, _ptr, check, lib
 
 NVT, NPT = 0, 5  # defs.h:21-25
 
@@ -64,6 +64,8 @@ class System:
                                     np.ascontiguousarray(O_all), np.ascontiguousarray(b_all), C.byref(h)))
         self._h = h
         self.R = R
+        self._theta_max0 = float(P[0].theta_max) if R else 0.1
+        self._net = None
 
     def close(self):
         if self._h is not None:
@@ -172,3 +174,83 @@ class System:
         ms = C.c_double(0)
         check(lib().pp_get_timing(self._h, None if name is None else name.encode(), C.byref(nl), C.byref(ms)))
         return nl.value, ms.value
+
+    # ---- learned proposal ------------------------------------------------
+
+    def attach(self, net, learned_fraction=0.8, theta_max=None):
+        """Use ``net`` (a :class:`ProposalNet`) for a share ``learned_fraction``
+        of the colour passes of later sweeps; ``net=None`` detaches.  The
+        rotation angle of a learned move is uniform in [0, theta_max)
+        (default: the replicas' own theta_max, of replica 0)."""
+        if theta_max is None:
+            theta_max = self._theta_max0
+        check(lib().pp_net_attach(self._h, None if net is None else net._h, float(learned_fraction),
+                                  float(theta_max)))
+        self._net = net  # keep it alive while attached
+
+    def train(self, net, steps, batch=4096, lr=1e-3, pos_noise=0.05, theta_max=None, sweeps_between=1):
+        """Train ``net`` by denoising moves drawn from this context's replicas
+        (one Adam step per batch, ``sweeps_between`` sweeps between steps);
+        returns the mean loss of the first and last batch."""
+        if theta_max is None:
+            theta_max = self._theta_max0
+        a, b = C.c_double(0), C.c_double(0)
+        check(lib().pp_net_train(self._h, net._h, int(steps), int(batch), float(lr), float(pos_noise),
+                                 float(theta_max), int(sweeps_between), C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def learn_counters(self, rep=0):
+        """(tried, accepted) learned moves of replica ``rep`` since the last reset."""
+        t, a = C.c_longlong(0), C.c_longlong(0)
+        check(lib().pp_get_learn_counters(self._h, rep, C.byref(t), C.byref(a)))
+        return t.value, a.value
+
+
+class ProposalNet:
+    """The proposal network of the learned move (device weights; FP32 master
+    copy, BF16 operands).  Weights are laid out as include/pp.h describes."""
+
+    def __init__(self, seed=1, device=0):
+        self._h = C.c_void_p()
+        check(lib().pp_net_create(int(device), int(seed), C.byref(self._h)))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().pp_net_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def get(self):
+        w = np.zeros(NET_WEIGHTS, np.float32)
+        b = np.zeros(NET_BIASES, np.float32)
+        check(lib().pp_net_get(self._h, _ptr(w), _ptr(b)))
+        return w, b
+
+    def set(self, w, b):
+        check(lib().pp_net_set(self._h, np.ascontiguousarray(w, np.float32), np.ascontiguousarray(b, np.float32)))
+
+    def forward(self, x):
+        x = np.ascontiguousarray(x, np.float32).reshape(-1, NET_INPUTS)
+        y = np.zeros((x.shape[0], NET_OUTPUTS), np.float32)
+        check(lib().pp_net_forward(self._h, x.shape[0], _ptr(x), _ptr(y)))
+        return y
+
+    def loss_grad(self, x, target):
+        x = np.ascontiguousarray(x, np.float32).reshape(-1, NET_INPUTS)
+        t = np.ascontiguousarray(target, np.float32).reshape(-1, 6)
+        gw = np.zeros(NET_WEIGHTS, np.float32)
+        gb = np.zeros(NET_BIASES, np.float32)
+        loss = C.c_double(0)
+        check(lib().pp_net_loss_grad(self._h, x.shape[0], x, t, C.byref(loss), _ptr(gw), _ptr(gb)))
+        return loss.value, gw, gb
