@@ -92,6 +92,201 @@ def measured_traffic(path=os.path.join(ROOT, "profiles", "k_pass_n32768_rho0.5_p
     return None
 
 
+def learned_leg(args, pp, tau_int, r_eq, O_eq, box_eq, opts):
+    """The learned proposal against the plain move on the same schedule (one
+    trial per non-empty cell and pass), from the equilibrated state: a
+    ProposalNet trained on --train-steps snapshots of the plain sampler (one
+    Adam step per snapshot, --train-batch moves each), then --learned-steps
+    timed steps with --learned-fraction of the colour passes learned and the
+    rest plain; the plain run times the same steps without the network.
+    Decorrelated samples/s = sweeps/s / (2 tau_int[E/N])."""
+    def timed(net):
+        with pp.System(r_eq, O_eq, box_eq, seed=SEED + 2, trials_per_cell=1, **opts) as s:
+            if net is not None:
+                s.attach(net, learned_fraction=args.learned_fraction, theta_max=THETA)
+            s.do_ensemble(50)
+            s.reset_counters()
+            s.set_timing(args.time_every)
+            s.set_trace(args.learned_steps)
+            t0 = time.perf_counter()
+            s.do_ensemble(args.learned_steps)
+            el = time.perf_counter() - t0
+            c = s.counters()
+            e = s.trace() / N
+            nl, ml = s.timing("learned")
+            tl, al = s.learn_counters(0)
+            b, ov, tot, anyo = s.energy()
+            assert not anyo and tot == c["bonds"], "tracked energy disagrees with a full recompute"
+        sw = c["tries"][0] / N / el
+        tau = tau_int(e) / (N * args.learned_steps / c["tries"][0])
+        out = {"sweeps_per_s": round(sw, 2), "tau_int_sweeps": round(tau, 2),
+               "decorrelated_samples_per_s": round(sw / (2 * tau), 4), "E_per_N": round(float(e.mean()), 5),
+               "acceptance": round(c["accepted"][0] / max(c["tries"][0], 1), 5)}
+        if net is not None:
+            out["learned_acceptance"] = round(al / max(tl, 1), 5)
+            out["learned_pass_us"] = round(ml / max(nl, 1) * 1e3, 3)
+        return out
+
+    print("# learned proposal: training ...", file=sys.stderr, flush=True)
+    net = pp.ProposalNet(seed=SEED)
+    with pp.System(r_eq, O_eq, box_eq, seed=SEED + 3, trials_per_cell=args.trials_per_cell, **opts) as s:
+        t0 = time.perf_counter()
+        l0, l1 = s.train(net, steps=args.train_steps, batch=args.train_batch, lr=1e-3, pos_noise=0.03,
+                         theta_max=THETA, sweeps_between=1)
+        t_train = time.perf_counter() - t0
+    print(f"# trained in {t_train:.1f} s (loss {l0:.3f} -> {l1:.3f}); timing learned and plain ...", file=sys.stderr,
+          flush=True)
+    learned = timed(net)
+    plain = timed(None)
+    net.close()
+    learned["train"] = {"steps": args.train_steps, "batch": args.train_batch, "seconds": round(t_train, 2),
+                        "loss_first": round(l0, 4), "loss_last": round(l1, 4), "pos_noise": 0.03}
+    learned["learned_fraction"] = args.learned_fraction
+    return learned, plain
+
+
+def _jumps(s, sweeps):
+    """Mean squared displacement and mean 1 - cos(angle) per attempted move of
+    replica 0, from states one sweep apart."""
+    import numpy as np
+    tr = rot = 0.0
+    att = 0
+    for _ in range(sweeps):
+        r0, O0, box = s.state(0)
+        c0 = s.counters()["tries"][0]
+        s.do_ensemble(1)
+        r1, O1, _ = s.state(0)
+        d = r1 - r0
+        d -= box * np.rint(d / box)
+        tr += float((d * d).sum())
+        rot += float((1 - (np.einsum("nij,nij->n", O1, O0) - 1) / 2).sum())
+        att += s.counters()["tries"][0] - c0
+    return tr / att, rot / att
+
+
+def bench_learned(args, pp, tau_int):
+    """The learned proposal at the gel state point: N=65536, rho=0.3, T=0.09.
+    Plain equilibration (--warmup steps of 8 trials per cell), a network
+    trained on --train-steps snapshots of the plain sampler, then --steps
+    timed steps each of the learned form (--learned-fraction of the passes
+    learned, the rest plain; one trial per non-empty cell and pass) and of
+    the plain move on the same schedule.  value = sweeps/s of the learned
+    form; jumps per attempt and tau_int[E/N] for both (tau_int is only a
+    lower bound when the run is shorter than a few tau)."""
+    n, rho, temp = 65536, 0.3, 0.09
+    r, O, box = pp.generate(n, rho, SEED)
+    opts = dict(T=temp, disp_max=DISP, theta_max=THETA, kf_delta=DELTA, kf_cosmax=COSMAX)
+    with pp.System(r, O, box, seed=SEED, trials_per_cell=8, **opts) as s:
+        s.do_ensemble(args.warmup)
+        st = s.state(0)
+        print(f"# warmup {args.warmup} steps, E/N={-s.counters()['bonds'] / n:.4f}", file=sys.stderr, flush=True)
+        net = pp.ProposalNet(seed=SEED)
+        t0 = time.perf_counter()
+        l0, l1 = s.train(net, steps=args.train_steps, batch=args.train_batch, lr=1e-3, pos_noise=0.03,
+                         theta_max=THETA, sweeps_between=1)
+        t_train = time.perf_counter() - t0
+    print(f"# trained in {t_train:.1f} s (loss {l0:.3f} -> {l1:.3f})", file=sys.stderr, flush=True)
+    res = {}
+    for name in ("learned", "plain"):
+        with pp.System(*st, seed=SEED + 2, trials_per_cell=1, **opts) as s:
+            if name == "learned":
+                s.attach(net, learned_fraction=args.learned_fraction, theta_max=THETA)
+            s.do_ensemble(20)
+            jt, jr = _jumps(s, 10)
+            s.reset_counters()
+            s.set_timing(args.time_every)
+            s.set_trace(args.steps)
+            t0 = time.perf_counter()
+            s.do_ensemble(args.steps)
+            el = time.perf_counter() - t0
+            c = s.counters()
+            e = s.trace() / n
+            nl, ml = s.timing("learned")
+            b, ov, tot, anyo = s.energy()
+            assert not anyo and tot == c["bonds"], "tracked energy disagrees with a full recompute"
+        sw = c["tries"][0] / n / el
+        tau = tau_int(e) / (n * args.steps / c["tries"][0])
+        res[name] = {"sweeps_per_s": round(sw, 2), "acceptance": round(c["accepted"][0] / c["tries"][0], 5),
+                     "jump_sq_per_attempt": round(jt, 8), "one_minus_cos_per_attempt": round(jr, 8),
+                     "tau_int_sweeps": round(tau, 2), "decorrelated_samples_per_s": round(sw / (2 * tau), 4),
+                     "E_per_N": round(float(e.mean()), 5), "run_sweeps": round(c["tries"][0] / n, 1)}
+        if name == "learned":
+            res[name]["learned_pass_us"] = round(ml / max(nl, 1) * 1e3, 3)
+        print(f"# {name}: {res[name]}", file=sys.stderr, flush=True)
+    net.close()
+    out = {
+        "metric": "particle sweeps/s + decorrelated samples/s (E/N), learned vs plain, 1 GPU",
+        "value": res["learned"]["sweeps_per_s"], "unit": "sweeps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": None, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64 geometry, bf16/f32 network", "data": "synthetic (seeded random-insertion start)",
+        "config": {"workload": f"learned-proposal NVT, N={n}, rho={rho}, T={temp}, learned fraction "
+                               f"{args.learned_fraction}, one trial per non-empty cell and pass",
+                   "train": {"steps": args.train_steps, "batch": args.train_batch, "seconds": round(t_train, 2),
+                             "loss_first": round(l0, 4), "loss_last": round(l1, 4)}},
+        "decorrelated_samples_per_s": {"learned": res["learned"]["decorrelated_samples_per_s"],
+                                       "plain": res["plain"]["decorrelated_samples_per_s"]},
+        "learned": res["learned"], "plain": res["plain"],
+    }
+    print(json.dumps(out), flush=True)
+
+
+def bench_npt(args, pp):
+    """16 NPT replicas of N=131072 at P=0.02, T=0.11 from a dilute start
+    (rho=0.05), rescale_factor_max 0.01, one volume attempt per N particle
+    attempts; with --learned-fraction > 0 a network trained on the batch
+    proposes that share of the colour passes.  value = sum over replicas of
+    attempts / N per second."""
+    import numpy as np
+    n, rho, temp, P, R = 131072, 0.05, 0.11, 0.02, 16
+    rs, Os, bs = [], [], []
+    for k in range(R):
+        r, O, box = pp.generate(n, rho, SEED + k)
+        rs.append(r)
+        Os.append(O)
+        bs.append(box)
+    prm = [dict(T=temp, disp_max=DISP, theta_max=THETA, kf_delta=DELTA, kf_cosmax=COSMAX, ensemble=pp.NPT, P=P,
+                rescale_factor_max=0.01)] * R
+    s = pp.System.batch(rs, Os, bs, prm, seed=SEED, trials_per_cell=args.trials_per_cell)
+    print(f"# {R} replicas of N={n} created", file=sys.stderr, flush=True)
+    for k in range(0, args.warmup, 50):
+        s.do_ensemble(min(50, args.warmup - k))
+        print(f"# warmup {k + 50} steps, density {float((n / s.counters_all()['volume']).mean()):.4f}",
+              file=sys.stderr, flush=True)
+    net = None
+    if args.learned_fraction > 0:
+        net = pp.ProposalNet(seed=SEED)
+        steps = min(args.train_steps, 1000)  # an NPT sweep of the batch between steps is the slow part
+        for k in range(0, steps, 100):
+            l0, l1 = s.train(net, steps=min(100, steps - k), batch=args.train_batch, lr=1e-3, pos_noise=0.03,
+                             theta_max=THETA, sweeps_between=1)
+            print(f"# train {k + 100} steps, loss {l1:.3f}", file=sys.stderr, flush=True)
+        s.attach(net, learned_fraction=args.learned_fraction, theta_max=THETA)
+    s.reset_counters()
+    t0 = time.perf_counter()
+    for k in range(0, args.steps, 50):
+        s.do_ensemble(min(50, args.steps - k))
+        print(f"# timed {min(k + 50, args.steps)} steps", file=sys.stderr, flush=True)
+    el = time.perf_counter() - t0
+    c = s.counters_all()
+    s.close()
+    if net is not None:
+        net.close()
+    rep_sweeps = float((c["tries"][:, 0] / n).sum())
+    dens = n / c["volume"]
+    out = {
+        "metric": "particle sweeps/s + decorrelated samples/s (E/N), learned vs plain, 1 GPU",
+        "value": round(rep_sweeps / el, 2), "unit": "replica sweeps/s (sum over replicas of attempts / N per second)",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(el / args.steps * 1e3, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded random-insertion starts)",
+        "config": {"workload": f"{R} NPT replicas of N={n}, P={P}, T={temp}, start rho={rho}, rescale_factor_max 0.01",
+                   "trials_per_cell": args.trials_per_cell, "learned_fraction": args.learned_fraction,
+                   "density_mean": round(float(dens.mean()), 5), "E_per_N_mean": round(float((-c["bonds"] / n).mean()), 5),
+                   "volume_acceptance": round(float(c["accepted"][:, 1].sum() / max(c["tries"][:, 1].sum(), 1)), 4)},
+    }
+    print(json.dumps(out), flush=True)
+
+
 def bench_batch(args, pp):
     """256 independent NVT replicas in one context: N log-uniform in [500,
     8000], T uniform in [0.08, 0.20], rho uniform in [0.05, 0.60], random
@@ -148,8 +343,15 @@ def main():
     ap.add_argument("--no-timing", action="store_true", help="no HIP events in the timed region")
     ap.add_argument("--time-every", type=int, default=16,
                     help="HIP events bracket the kernels of every k-th sweep of the timed region")
-    ap.add_argument("--workload", choices=("single", "batch"), default="single",
-                    help="single: the N=32768 system; batch: 256 independent replicas of uneven N, T and rho")
+    ap.add_argument("--learned-steps", type=int, default=30000,
+                    help="timed steps of the learned-vs-plain comparison (0: skip it)")
+    ap.add_argument("--train-steps", type=int, default=10000, help="Adam steps (one sampler snapshot each)")
+    ap.add_argument("--train-batch", type=int, default=4096)
+    ap.add_argument("--learned-fraction", type=float, default=0.8)
+    ap.add_argument("--workload", choices=("single", "batch", "learned", "npt"), default="single",
+                    help="single: the N=32768 system; batch: 256 independent replicas of uneven N, T and rho; "
+                         "learned: the learned proposal at N=65536, rho=0.3, T=0.09; npt: 16 NPT replicas of "
+                         "N=131072")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only copy a MIB-sized device buffer 5 times (the reference point that scales "
                          "the FETCH_SIZE / WRITE_SIZE counters of a profile)")
@@ -175,6 +377,10 @@ def main():
 
     if args.workload == "batch":
         return bench_batch(args, pp)
+    if args.workload == "learned":
+        return bench_learned(args, pp, tau_int)
+    if args.workload == "npt":
+        return bench_npt(args, pp)
 
     r, O, box = pp.generate(N, RHO, SEED)
     opts = dict(T=T, disp_max=DISP, theta_max=THETA, kf_delta=DELTA, kf_cosmax=COSMAX, threads_per_cell=args.tpc,
@@ -230,6 +436,10 @@ def main():
         per_member = {"trials_per_cell": 0, "sweeps_per_s": round(c0["tries"][0] / N / (tb - ta), 2),
                       "steps": args.per_member_steps}
 
+    learned = plain_same = None
+    if args.learned_steps > 0:
+        learned, plain_same = learned_leg(args, pp, tau_int, r_eq, O_eq, box_eq, opts)
+
     cpu = None
     if not args.no_cpu:
         print("# cpu baseline ...", file=sys.stderr, flush=True)
@@ -258,8 +468,13 @@ def main():
             "acceptance": round(c["accepted"][0] / max(attempts, 1), 5),
             "E_per_N": round(float(-c["bonds"] / N), 6),
         },
-        "decorrelated_samples_per_s": {"plain": round(decorrelated, 3), "tau_int_sweeps": round(tau, 2),
-                                       "learned": None},
+        "decorrelated_samples_per_s": {
+            "plain": round(decorrelated, 3), "tau_int_sweeps": round(tau, 2),
+            "learned": learned["decorrelated_samples_per_s"] if learned else None,
+            "learned_over_plain": round(learned["decorrelated_samples_per_s"] / decorrelated, 4) if learned else None,
+            "learned_run": learned,
+            "plain_one_trial_per_cell": plain_same,
+        },
         "one_trial_per_member": per_member,
         "roofline": {
             "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -228,3 +228,52 @@ def test_training_lowers_the_loss(pp, oracle):
         got = net.forward(x).astype(np.float64)
         want = no.forward(w, bb, x.astype(np.float64))
         assert np.linalg.norm(got - want) < 1e-2 * np.linalg.norm(want)
+
+
+def _jumps(s, sweeps):
+    """Mean squared displacement and mean 1 - cos(rotation angle) per
+    attempted move, from states one sweep apart (the expected squared jump of
+    a move; a particle rarely moves twice in one sweep of one trial per cell)."""
+    tr = rot = 0.0
+    att = 0
+    for _ in range(sweeps):
+        r0, O0, box = s.state(0)
+        c0 = s.counters()["tries"][0]
+        s.do_ensemble(1)
+        r1, O1, _ = s.state(0)
+        d = r1 - r0
+        d -= box * np.rint(d / box)
+        tr += float((d * d).sum())
+        rot += float((1 - (np.einsum("nij,nij->n", O1, O0) - 1) / 2).sum())
+        att += s.counters()["tries"][0] - c0
+    return tr / att, rot / att
+
+
+def test_trained_network_gains_over_plain_moves(pp, oracle):
+    """What training buys, per attempted move at rho = 0.5, T = 0.12 with one
+    trial per non-empty cell and pass: the trained proposal's squared jump
+    beats the plain uniform move's (Disp_max 0.05) and the untrained
+    network's.  Measured: about 1.7x plain and 3x untrained in translation;
+    rotations stay below the plain move's (the learned axis density
+    concentrates), and the energy's autocorrelation time per attempt is
+    unchanged (bench.py reports both)."""
+    r, O, box = oracle.generate(4000, 0.5, seed=7)
+    with pp.System(r, O, box, T=0.12, seed=5, trials_per_cell=8) as s:
+        s.do_ensemble(1500)
+        st = s.state(0)
+    with pp.System(*st, T=0.12, seed=9, trials_per_cell=1) as s:
+        plain = _jumps(s, 20)
+    res = {}
+    for name, steps in (("untrained", 0), ("trained", 1500)):
+        with pp.ProposalNet(seed=3) as net:
+            if steps:
+                with pp.System(*st, T=0.12, seed=13, trials_per_cell=8) as s:
+                    s.train(net, steps=steps, batch=4096, lr=1e-3, pos_noise=0.03, theta_max=0.1)
+            with pp.System(*st, T=0.12, seed=9, trials_per_cell=1) as s:
+                s.attach(net, 1.0, 0.1)
+                res[name] = _jumps(s, 20)
+                tried, acc = s.learn_counters(0)
+                assert tried > 0
+    assert res["trained"][0] > 1.3 * plain[0], (res, plain)
+    assert res["trained"][0] > 2.0 * res["untrained"][0], (res, plain)
+    assert res["trained"][1] > 1.5 * res["untrained"][1], (res, plain)

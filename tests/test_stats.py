@@ -1,0 +1,46 @@
+"""The time-series tools behind the decorrelated-samples metric, on synthetic
+AR(1) series whose integrated autocorrelation time is known in closed form:
+x_t = phi x_{t-1} + e_t has rho(t) = phi^t, so tau_int = 1/2 + phi / (1 - phi)."""
+import math
+
+import numpy as np
+import pytest
+
+from patchyparticles_amd.stats import autocorr, block_mean, tau_int
+
+
+def _ar1(phi, n, seed):
+    rng = np.random.default_rng(seed)
+    e = rng.normal(size=n)
+    x = np.empty(n)
+    x[0] = e[0] / math.sqrt(1 - phi * phi)
+    for t in range(1, n):
+        x[t] = phi * x[t - 1] + e[t]
+    return x
+
+
+@pytest.mark.parametrize("phi", [0.0, 0.5, 0.9, 0.98])
+def test_tau_int_of_ar1(phi):
+    x = _ar1(phi, 400000, seed=int(phi * 100))
+    want = 0.5 + phi / (1 - phi)
+    got = tau_int(x)
+    # the estimator's relative error is about sqrt(2 (2W + 1) / n) with W ~ 5 tau
+    assert abs(got - want) < 0.06 * want + 0.02, (phi, got, want)
+    rho = autocorr(x)
+    assert rho[0] == pytest.approx(1.0)
+    assert abs(rho[1] - phi) < 0.01
+
+
+def test_block_standard_error_of_ar1():
+    phi, n = 0.9, 400000
+    x = _ar1(phi, n, seed=3)
+    tau = 0.5 + phi / (1 - phi)
+    var = 1 / (1 - phi * phi)
+    m, se = block_mean(x, nblocks=20)
+    want = math.sqrt(2 * tau * var / n)
+    assert abs(se - want) < 0.4 * want
+    assert abs(m) < 5 * want
+
+
+def test_constant_series():
+    assert tau_int(np.ones(100)) == 0.5
