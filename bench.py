@@ -410,7 +410,8 @@ def main():
     sweeps_per_s = attempts / N / elapsed  # a sweep is N attempted moves (MC.c:21)
     nside, side = sysm.grid()
     bpm = algorithmic_bytes_per_move(RHO, side[0])
-    bytes_per_launch = bpm * N / 8  # the data of the N/8 particles one colour pass covers
+    moves_per_launch = attempts / args.steps / 8  # attempted moves one colour pass makes
+    bytes_per_launch = bpm * moves_per_launch
     avg_pass_s = ms_pass / max(n_pass, 1) / 1e3
     achieved = bytes_per_launch / avg_pass_s / 1e9 if avg_pass_s > 0 else 0.0
     steps_per_sweep = N * args.steps / attempts
@@ -480,6 +481,7 @@ def main():
             "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": measured_traffic(),
             "kernel": "k_pass (colour pass)", "bytes_per_move": round(bpm, 1),
+            "moves_per_launch": round(moves_per_launch, 1),
             "algorithmic_bytes_per_launch": round(bytes_per_launch), "avg_launch_us": round(avg_pass_s * 1e6, 3),
             "rebuild_us_per_sweep": round(ms_reb / max(n_reb, 1) * 1e3, 3),
             "timed_sweeps": n_reb,

@@ -39,6 +39,8 @@
  *   pp_get_grid                   no reference counterpart (launch-shape
  *                                 tuning, per-sweep energy series, kernel
  *                                 timing, grid introspection)
+ *   pp_checkpoint_save/_load     .rrr restart (output.c:37-46, system.c:213-245),
+ *                                 exact and with the random-number counter
  *   pp_net_*, pp_get_learn_counters
  *                                 no reference counterpart: the learned
  *                                 proposal of the rototranslation move, a
@@ -187,6 +189,15 @@ pp_status pp_get_timing(pp_ctx *ctx, const char *name, long long *launches, doub
  * shape, so tuning only advances the simulation.  Outputs may be NULL.
  * Per-pass launch form only. */
 pp_status pp_autotune(pp_ctx *ctx, int sweeps, int *threads_per_cell, int *block_size, double *us_per_sweep);
+
+/* Checkpoint (no reference counterpart; the reference restarts from .rrr
+ * text, output.c:37-46 and system.c:213-245, which rounds the state): the
+ * exact device state of every replica (FP64 positions and orientations, box,
+ * tracked volume, move counters) and the sweep index, which is the Philox
+ * counter.  Loading into a context created with the same parameters, sizes,
+ * seed and trials_per_cell resumes the run bit for bit. */
+pp_status pp_checkpoint_save(pp_ctx *ctx, const char *path);
+pp_status pp_checkpoint_load(pp_ctx *ctx, const char *path);
 
 /* Grid of replica rep: cells per side and cell side lengths. */
 pp_status pp_get_grid(pp_ctx *ctx, int rep, int n_side[3], double side[3]);

@@ -15,7 +15,11 @@
  *
  * Keys of this program only (optional): GPU_device (HIP ordinal, 0),
  * Threads_per_cell (32 or 64), Trials_per_cell (0: a sweep is N attempts;
- * s > 0: s trials per non-empty sweep cell and colour pass).
+ * s > 0: s trials per non-empty sweep cell and colour pass), Checkpoint_file
+ * (written with every last.rrr: the exact state and the random-number
+ * counter), Resume (1: continue bit for bit from Checkpoint_file; the
+ * Initial_conditions_file then only gives N, normally the last.rrr written
+ * with the checkpoint).
  */
 #include <signal.h>
 #include <stdio.h>
@@ -45,6 +49,7 @@ typedef struct run {
 	int n;
 	int npt;
 	double *r, *O;
+	char chk[1024];  /* checkpoint path ("" : none) */
 } run;
 
 static void check(run *u, pp_status st, const char *what) {
@@ -56,6 +61,13 @@ static void save_conf(run *u, long long step, const char *path) {
 	double box[3];
 	check(u, pp_download(u->ctx, 0, u->r, u->O, box), "pp_download");
 	if(pprrr_write(path, step, u->n, box, u->r, u->O) != 0) ppo_exit(&u->out, "File '%s' is not writable\n", path);
+}
+
+/* the checkpoint: the state at the top of an output step, before its energy
+ * check (whose Gram-Schmidt changes the orientations); a resumed run redoes
+ * that step's check, as the run that wrote it did */
+static void save_checkpoint(run *u) {
+	if(u->chk[0]) check(u, pp_checkpoint_save(u->ctx, u->chk), "pp_checkpoint_save");
 }
 
 static void print_step(run *u, long long step) {
@@ -145,6 +157,15 @@ int main(int argc, char *argv[]) {
 	u.O = malloc(sizeof(double) * 9 * (size_t) (u.n > 0 ? u.n : 1));
 	if(!u.r || !u.O) ppo_exit(&u.out, "out of memory\n");
 
+	int resume = 0;
+	u.chk[0] = '\0';
+	ppi_string(in, "Checkpoint_file", u.chk, sizeof u.chk, 0);
+	ppi_int(in, "Resume", &resume, 0);
+	if(resume) {
+		if(!u.chk[0]) ppo_exit(&u.out, "Resume = 1 needs Checkpoint_file\n");
+		check(&u, pp_checkpoint_load(u.ctx, u.chk), "pp_checkpoint_load");
+	}
+
 	pp_counters cnt;
 	check(&u, pp_get_counters(u.ctx, 0, &cnt), "pp_get_counters");
 	ppo_log(&u.out, "Initial energy: %lf\n", u.n > 0 ? -(double) cnt.bonds / u.n : 0.);
@@ -155,15 +176,18 @@ int main(int argc, char *argv[]) {
 
 	long long step = u.out.start_from;
 	while(step < steps && !g_stop) {
-		if(step % u.out.print_every == 0) {
+		const int print = step % u.out.print_every == 0, save = step > 0 && step % u.out.save_every == 0;
+		if(print) {
 			print_step(&u, step);
+			save_checkpoint(&u);
 			if(step != 0) check_energy(&u);
 		}
-		if(step > 0 && step % u.out.save_every == 0) {
+		if(save) {
 			char name[1100];
 			snprintf(name, sizeof name, "%s/conf_%lld.rrr", u.out.folder, step);
 			save_conf(&u, step, name);
 			save_conf(&u, step, u.out.last);
+			if(!print) save_checkpoint(&u);
 		}
 		long long until = next_multiple(step, u.out.print_every);
 		const long long s2 = next_multiple(step, u.out.save_every);
@@ -175,6 +199,7 @@ int main(int argc, char *argv[]) {
 	}
 
 	save_conf(&u, step, u.out.last);
+	save_checkpoint(&u);
 	check(&u, pp_get_counters(u.ctx, 0, &cnt), "pp_get_counters");
 	const double E = -(double) cnt.bonds;
 	printf("%lld %lf %lf", step, u.n > 0 ? E / u.n : 0., E);

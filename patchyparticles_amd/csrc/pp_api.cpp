@@ -869,6 +869,131 @@ pp_status pp_get_timing(pp_ctx *c, const char *name, long long *launches, double
 	return PP_OK;
 }
 
+namespace {
+
+// checkpoint file: header, then per replica its size, parameters, box,
+// tracked volume, move counters, positions and orientations (FP64, bytes as
+// held on the device)
+constexpr char CHK_MAGIC[8] = {'P', 'P', 'C', 'H', 'K', '1', 0, 0};
+constexpr int CHK_ROWS[6] = {C_TRY_ROTO, C_ACC_ROTO, C_TRY_VOL, C_ACC_VOL, C_TRY_LEARN, C_ACC_LEARN};
+
+struct ChkHead {
+	char magic[8];
+	int32_t version, R;
+	int64_t sweep;
+	uint64_t seed;
+	int32_t trials_per_cell, pad;
+};
+
+struct ChkRep {
+	int32_t n, ensemble;
+	double T, disp_max, theta_max, rc2, cosmax, P, rfm;
+	double box[3], V;
+	int64_t ctr[6];
+};
+
+}  // namespace
+
+pp_status pp_checkpoint_save(pp_ctx *c, const char *path) {
+	if(!c || !path) return fail(PP_ERR_ARG, "NULL argument");
+	HIPCHECK(hipSetDevice(c->device));
+	TRY(check_err(c));
+	TRY(pull_reps(c));
+	std::vector<unsigned long long> hc;
+	TRY(pull_ctr(c, hc));
+	const std::string tmp = std::string(path) + ".tmp";
+	FILE *f = fopen(tmp.c_str(), "wb");
+	if(!f) return fail(PP_ERR_ARG, "cannot open '%s' for writing", tmp.c_str());
+	ChkHead h{};
+	std::memcpy(h.magic, CHK_MAGIC, 8);
+	h.version = 1;
+	h.R = c->R;
+	h.sweep = c->sweep;
+	h.seed = c->d.seed;
+	h.trials_per_cell = c->d.trials_per_cell;
+	bool ok = fwrite(&h, sizeof h, 1, f) == 1;
+	std::vector<double> r, O;
+	for(int k = 0; k < c->R && ok; k++) {
+		const Rep &rp = c->reps[k];
+		ChkRep q{};
+		q.n = rp.np;
+		q.ensemble = rp.ensemble;
+		q.T = rp.T;
+		q.disp_max = rp.disp_max;
+		q.theta_max = rp.theta_max;
+		q.rc2 = rp.rc2;
+		q.cosmax = rp.cosmax;
+		q.P = rp.P;
+		q.rfm = rp.rfm;
+		for(int a = 0; a < 3; a++) q.box[a] = rp.g.box[a];
+		q.V = rp.V;
+		for(int t = 0; t < 6; t++) q.ctr[t] = ctr_sum(hc, k, CHK_ROWS[t]);
+		r.assign(3 * (size_t) std::max(rp.np, 1), 0.);
+		O.assign(9 * (size_t) std::max(rp.np, 1), 0.);
+		pp_status st = pp_download(c, k, r.data(), O.data(), nullptr);
+		if(st != PP_OK) {
+			fclose(f);
+			return st;
+		}
+		ok = fwrite(&q, sizeof q, 1, f) == 1 && fwrite(r.data(), sizeof(double), 3 * (size_t) rp.np, f) == 3 * (size_t) rp.np &&
+		     fwrite(O.data(), sizeof(double), 9 * (size_t) rp.np, f) == 9 * (size_t) rp.np;
+	}
+	ok = (fclose(f) == 0) && ok;
+	if(!ok || std::rename(tmp.c_str(), path) != 0) return fail(PP_ERR_ARG, "writing '%s' failed", path);
+	return PP_OK;
+}
+
+pp_status pp_checkpoint_load(pp_ctx *c, const char *path) {
+	if(!c || !path) return fail(PP_ERR_ARG, "NULL argument");
+	HIPCHECK(hipSetDevice(c->device));
+	FILE *f = fopen(path, "rb");
+	if(!f) return fail(PP_ERR_ARG, "cannot open '%s'", path);
+	ChkHead h{};
+	std::vector<ChkRep> q(c->R);
+	std::vector<std::vector<double>> r(c->R), O(c->R);
+	pp_status st = PP_OK;
+	if(fread(&h, sizeof h, 1, f) != 1 || std::memcmp(h.magic, CHK_MAGIC, 8) != 0 || h.version != 1)
+		st = fail(PP_ERR_ARG, "'%s' is not a checkpoint of this library", path);
+	else if(h.R != c->R)
+		st = fail(PP_ERR_ARG, "checkpoint holds %d replicas, the context %d", h.R, c->R);
+	else if(h.seed != c->d.seed || h.trials_per_cell != c->d.trials_per_cell)
+		st = fail(PP_ERR_ARG, "checkpoint written with another seed or trials_per_cell");
+	TRY(pull_reps(c));
+	for(int k = 0; k < c->R && st == PP_OK; k++) {
+		const Rep &rp = c->reps[k];
+		ChkRep &x = q[k];
+		if(fread(&x, sizeof x, 1, f) != 1) {
+			st = fail(PP_ERR_ARG, "checkpoint '%s' is truncated", path);
+			break;
+		}
+		if(x.n != rp.np || x.ensemble != rp.ensemble || x.T != rp.T || x.disp_max != rp.disp_max ||
+		   x.theta_max != rp.theta_max || x.rc2 != rp.rc2 || x.cosmax != rp.cosmax || x.P != rp.P || x.rfm != rp.rfm) {
+			st = fail(PP_ERR_ARG, "replica %d of the checkpoint has other parameters or size than the context", k);
+			break;
+		}
+		r[k].resize(3 * (size_t) x.n);
+		O[k].resize(9 * (size_t) x.n);
+		if(fread(r[k].data(), sizeof(double), r[k].size(), f) != r[k].size() ||
+		   fread(O[k].data(), sizeof(double), O[k].size(), f) != O[k].size())
+			st = fail(PP_ERR_ARG, "checkpoint '%s' is truncated", path);
+	}
+	fclose(f);
+	if(st != PP_OK) return st;
+	for(int k = 0; k < c->R; k++) TRY(pp_upload(c, k, r[k].data(), O[k].data(), q[k].box));
+	TRY(pull_reps(c));
+	for(int k = 0; k < c->R; k++) c->reps[k].V = q[k].V;
+	TRY(push_reps(c));
+	std::vector<unsigned long long> hc;
+	TRY(pull_ctr(c, hc));
+	for(int k = 0; k < c->R; k++)
+		for(int t = 0; t < 6; t++)
+			for(int sl = 0; sl < NSLOT; sl++)
+				hc[((size_t) k * NCTR + CHK_ROWS[t]) * NSLOT + sl] = sl == 0 ? (unsigned long long) q[k].ctr[t] : 0ull;
+	HIPCHECK(hipMemcpy(c->d.ctr, hc.data(), hc.size() * sizeof(unsigned long long), hipMemcpyHostToDevice));
+	c->sweep = h.sweep;
+	return PP_OK;
+}
+
 pp_status pp_get_learn_counters(pp_ctx *c, int rep, long long *tries, long long *accepted) {
 	if(!c) return fail(PP_ERR_ARG, "ctx is NULL");
 	if(rep < 0 || rep >= c->R) return fail(PP_ERR_ARG, "replica %d out of range", rep);

@@ -175,6 +175,14 @@ class System:
         check(lib().pp_get_timing(self._h, None if name is None else name.encode(), C.byref(nl), C.byref(ms)))
         return nl.value, ms.value
 
+    def checkpoint_save(self, path):
+        """Exact state, counters and sweep index of every replica (bit-exact resume)."""
+        check(lib().pp_checkpoint_save(self._h, str(path).encode()))
+
+    def checkpoint_load(self, path):
+        """Resume from pp_checkpoint_save's file (same parameters, sizes, seed)."""
+        check(lib().pp_checkpoint_load(self._h, str(path).encode()))
+
     # ---- learned proposal ------------------------------------------------
 
     def attach(self, net, learned_fraction=0.8, theta_max=None):

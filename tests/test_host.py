@@ -215,3 +215,38 @@ def test_driver_run_matches_reference_outputs(host_built, oracle, tmp_path, ense
         assert not ov.any() and abs(-tot / 1000 - e_printed) <= 3e-3
     s, box, r, O = oracle.rrr_load(str(tmp_path / "last.rrr"))
     assert s == 300
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("ensemble", ["nvt", "npt"])
+def test_driver_resume_from_checkpoint_is_bit_exact(host_built, tmp_path, ensemble):
+    """Checkpoint_file / Resume: 300 steps in one run equal 150 steps, then a
+    second run started from that run's last.rrr (Restart_step_counter = 0)
+    and its checkpoint: the same last.rrr bytes and conf_200.rrr, and the
+    same final line."""
+    g = golden("host_runs.json")[ensemble]
+    subprocess.run([GENERATOR, "500", "0.2"], cwd=tmp_path, check=True, capture_output=True)
+    base = dict(g["keys"], Print_every=50, Save_every=100, Configuration_folder="confs",
+                Checkpoint_file="run.ppchk")
+    outs = {}
+    for name, steps in (("full", 300), ("first", 150)):
+        d = tmp_path / name
+        (d / "confs").mkdir(parents=True)
+        (d / "generated.rrr").write_bytes((tmp_path / "generated.rrr").read_bytes())
+        _write_input(d, dict(base, Steps=steps))
+        res = subprocess.run([DRIVER, "input"], cwd=d, capture_output=True, text=True, timeout=120)
+        assert res.returncode == 0, res.stderr
+        outs[name] = res.stdout.splitlines()
+    d = tmp_path / "first"
+    (d / "start.rrr").write_bytes((d / "last.rrr").read_bytes())
+    _write_input(d, dict(base, Steps=150, Initial_conditions_file="start.rrr", Restart_step_counter=0, Resume=1))
+    res = subprocess.run([DRIVER, "input"], cwd=d, capture_output=True, text=True, timeout=120)
+    assert res.returncode == 0, res.stderr
+    full = tmp_path / "full"
+    assert (d / "last.rrr").read_bytes() == (full / "last.rrr").read_bytes()
+    assert (d / "confs" / "conf_200.rrr").read_bytes() == (full / "confs" / "conf_200.rrr").read_bytes()
+    assert res.stdout.splitlines()[-1] == outs["full"][-1]
+    # Resume without a checkpoint file is refused
+    _write_input(d, dict(g["keys"], Resume=1))
+    res = subprocess.run([DRIVER, "input"], cwd=d, capture_output=True, text=True, timeout=60)
+    assert res.returncode == 1 and "Resume = 1 needs Checkpoint_file" in res.stderr
