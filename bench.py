@@ -331,7 +331,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=4000, help="timed steps (a step: rebuild + 8 colour passes)")
     ap.add_argument("--warmup", type=int, default=6000, help="untimed steps first (equilibration)")
-    ap.add_argument("--trials-per-cell", type=int, default=8,
+    ap.add_argument("--trials-per-cell", type=int, default=16,
                     help="trials per non-empty cell and pass (0: one per member, a step of exactly N attempts)")
     ap.add_argument("--per-member-steps", type=int, default=2000,
                     help="steps of the one-trial-per-member schedule timed after the main run (0: skip)")

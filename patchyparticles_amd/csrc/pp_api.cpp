@@ -375,6 +375,7 @@ pp_status create_impl(int R, const pp_params *prm, const pp_options *opts, const
 	   (st = dev_alloc(c, (int **) &d.act_off, R + 1)) || (st = dev_alloc(c, &d.count, d.C_total + 1)) ||
 	   (st = dev_alloc(c, &d.start, d.C_total + 1)) || (st = dev_alloc(c, &d.slot, d.stride)) ||
 	   (st = dev_alloc(c, &d.cellid, d.stride)) || (st = dev_alloc(c, &d.scan_status, std::max(d.ntiles, 1))) ||
+	   (st = dev_alloc(c, &d.tile_sum, std::max(d.ntiles, 1))) ||
 	   (st = dev_alloc(c, &d.scan_ticket, 2)) ||
 	   (st = dev_alloc(c, &d.clist, d.C_total + 1)) || (st = dev_alloc(c, &d.ccount, (size_t) R * 8)) ||
 	   (st = dev_alloc(c, &d.pfx, (size_t) 8 * (R + 1))) ||
@@ -386,7 +387,7 @@ pp_status create_impl(int R, const pp_params *prm, const pp_options *opts, const
 	{
 		// every buffer a kernel may touch must exist before the first launch
 		const void *need[] = {d.f[0], d.f[1], d.pid[0], d.pid[1], d.reps, d.prep, d.act_off, d.count, d.start,
-		                      d.slot, d.cellid, d.scan_status, d.clist, d.ccount, d.pfx, d.rec, d.scan_ticket,
+		                      d.slot, d.cellid, d.scan_status, d.tile_sum, d.clist, d.ccount, d.pfx, d.rec, d.scan_ticket,
 		                      d.ctr, d.err, d.out_bonds, d.out_ovl, d.bar};
 		for(const void *q : need)
 			if(!q) return bail(fail(PP_ERR_NOMEM, "a device buffer was not allocated"));

@@ -77,6 +77,11 @@ enum Err : int { ERR_CELL_CAP = 1, ERR_BOX = 2, ERR_SCAN = 8, ERR_BARRIER = 16 }
 // rebuild modes
 enum Reb : int { REB_KEEP = 0, REB_SHIFT = 1, REB_TRIAL = 2 };
 
+// grids of more scan tiles than this are scanned in three steps (tile sums,
+// one scan of the sums, the tiles) instead of one pass with a look-back,
+// whose chains grow with the number of tiles in flight
+constexpr int SCAN_LOOKBACK_MAX_TILES = 64;
+
 // spin bound of the scan's look-back (a stuck predecessor is reported, not waited on forever)
 constexpr int SCAN_SPIN_MAX = 1 << 22;
 // polls of a grid barrier before it reports ERR_BARRIER and lets the workgroup go on
@@ -158,6 +163,7 @@ struct DevState {
 	int *rec;          // [8][total_act][REC_W] records of the cells of each pass, in list order
 	unsigned *bar;     // grid barrier of the cooperative sweep kernel: arrivals, generation
 	int *scan_ticket;  // [0]: tiles taken, [1]: tiles done
+	int *tile_sum;     // [ntiles] per-tile sums, then their exclusive prefixes (grids of many tiles)
 	int ntiles;
 	unsigned long long *ctr;  // [R][NCTR][NSLOT]
 	int *err;

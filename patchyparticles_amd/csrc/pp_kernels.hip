@@ -129,7 +129,47 @@ constexpr int SCAN_KEYS = 512;
 // smaller ticket, which are already running.  start[c] = first slot of cell
 // c; count is cleared for the next rebuild; a flipping scan also switches
 // every replica to the buffer the scatter fills.
-__global__ __launch_bounds__(256) void k_scan(DevState d, int flip, int lists) {
+// Sum of the cell counts of each scan tile.
+__global__ __launch_bounds__(256) void k_tile_sums(DevState d) {
+	__shared__ int part[4];
+	const int t = threadIdx.x, base = blockIdx.x * SCAN_TILE + t * 8;
+	int tot = 0;
+#pragma unroll
+	for(int e = 0; e < 8; e++)
+		if(base + e < d.C_total) tot += d.count[base + e];
+#pragma unroll
+	for(int o = 32; o > 0; o >>= 1) tot += __shfl_xor(tot, o, 64);
+	if((t & 63) == 0) part[t >> 6] = tot;
+	__syncthreads();
+	if(t == 0) d.tile_sum[blockIdx.x] = part[0] + part[1] + part[2] + part[3];
+}
+
+// Exclusive prefix of the tile sums, in place (one workgroup).
+__global__ __launch_bounds__(1024) void k_tile_scan(DevState d) {
+	__shared__ int part[1024];
+	const int t = threadIdx.x, n = d.ntiles;
+	const int per = (n + 1023) / 1024, lo = t * per, hi = min(n, lo + per);
+	int s = 0;
+	for(int i = lo; i < hi; i++) s += d.tile_sum[i];
+	part[t] = s;
+	__syncthreads();
+	for(int o = 1; o < 1024; o <<= 1) {
+		const int v = t >= o ? part[t - o] : 0;
+		__syncthreads();
+		part[t] += v;
+		__syncthreads();
+	}
+	int run = part[t] - s;
+	for(int i = lo; i < hi; i++) {
+		const int v = d.tile_sum[i];
+		d.tile_sum[i] = run;
+		run += v;
+	}
+}
+
+// pre = 1: the tiles' exclusive prefixes are in d.tile_sum (k_tile_sums,
+// k_tile_scan); pre = 0: a decoupled look-back finds them.
+__global__ __launch_bounds__(256) void k_scan(DevState d, int flip, int lists, int pre) {
 	__shared__ int wsum[4];
 	__shared__ int sh_tile, sh_excl, sh_r0, sh_last;
 	__shared__ int lcnt[SCAN_KEYS], lbase[SCAN_KEYS];
@@ -166,7 +206,10 @@ __global__ __launch_bounds__(256) void k_scan(DevState d, int flip, int lists) {
 	}
 	if(lane == 63) wsum[wid] = inc;
 	__syncthreads();
-	if(wid == 0) {
+	if(pre) {
+		if(t == 0) sh_excl = d.tile_sum[tile];
+	}
+	else if(wid == 0) {
 		// wavefront 0 publishes the tile's sum and looks back over up to 64
 		// predecessors at once: the nearest one with an inclusive prefix ends
 		// the walk; one that has published nothing yet is polled again
@@ -372,7 +415,12 @@ void launch_rebuild(const DevState &d, int mode, uint64_t sweep, long long trace
 	if(gp == 0) gp = 1;
 	const int flip = mode != REB_TRIAL;
 	hipLaunchKernelGGL(k_count, dim3(gp), dim3(B), 0, s, d, mode, sweep, trace_pos);
-	hipLaunchKernelGGL(k_scan, dim3(d.ntiles), dim3(256), 0, s, d, flip, mode == REB_SHIFT ? 1 : 0);
+	const int pre = d.ntiles > SCAN_LOOKBACK_MAX_TILES;
+	if(pre) {
+		hipLaunchKernelGGL(k_tile_sums, dim3(d.ntiles), dim3(256), 0, s, d);
+		hipLaunchKernelGGL(k_tile_scan, dim3(1), dim3(1024), 0, s, d);
+	}
+	hipLaunchKernelGGL(k_scan, dim3(d.ntiles), dim3(256), 0, s, d, flip, mode == REB_SHIFT ? 1 : 0, pre);
 	unsigned gs = gp;
 	if(mode == REB_SHIFT) gs = std::max(gp, (unsigned) ((d.C_total + B - 1) / B));
 	hipLaunchKernelGGL(k_scatter, dim3(gs), dim3(B), 0, s, d, mode, sweep);
@@ -910,11 +958,19 @@ __global__ __launch_bounds__(256) void k_energy(DevState d, int trial, int per_p
 	}
 }
 
+// Lanes per particle: 32 where the 27 cells hold many candidates, 8 in dilute
+// systems (fewer than 8 candidates on average over the cell capacity), where
+// most of a 32-lane team would idle.  The result does not depend on it.
 void launch_energy(const DevState &d, int trial, int per_particle, hipStream_t s) {
 	hipLaunchKernelGGL(k_zero_energy_rows, dim3((d.R * 2 * NSLOT + 255) / 256), dim3(256), 0, s, d.ctr, d.R);
 	const int block = 256;
-	hipLaunchKernelGGL(k_energy<32>, dim3((d.Ntot + block / 32 - 1) / (block / 32)), dim3(block), 0, s, d, trial,
-	                   per_particle);
+	const double cand = 27. * (double) d.Ntot / (double) (d.C_total > 0 ? d.C_total : 1);
+	if(cand < 8.)
+		hipLaunchKernelGGL(k_energy<8>, dim3((d.Ntot + block / 8 - 1) / (block / 8)), dim3(block), 0, s, d, trial,
+		                   per_particle);
+	else
+		hipLaunchKernelGGL(k_energy<32>, dim3((d.Ntot + block / 32 - 1) / (block / 32)), dim3(block), 0, s, d, trial,
+		                   per_particle);
 }
 
 // ------------------------------------------------------------------------

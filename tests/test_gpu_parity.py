@@ -272,3 +272,30 @@ def test_uneven_batch(pp, oracle, trials_per_cell):
             ob, oov, otot = oracle.energy(rr, OO, bb)
             assert np.array_equal(b, ob) and otot == tot, k
         assert (c["tries"][:, 0] > 0).all()
+
+
+@pytest.mark.parametrize("ensemble", ["nvt", "npt"])
+def test_large_grid_scan(pp, oracle, ensemble):
+    """A dilute system whose cell grid spans more scan tiles than the
+    look-back handles (N=20000 at rho=0.05: 64^3 cells, 128 tiles), so every
+    rebuild takes the three-step scan: the tracked energy equals a full
+    recompute, the cell populations check, and the energies of the evolved
+    snapshot equal the oracle's bit for bit (NPT: through trial rebuilds and
+    accepted volume moves)."""
+    r, O, box = _start(oracle, 20000, 0.05, 8)
+    prm = dict(T=0.11)
+    if ensemble == "npt":
+        prm = dict(T=0.11, ensemble=pp.NPT, P=0.02, rescale_factor_max=0.002)
+    with pp.System(r, O, box, seed=5, trials_per_cell=4, **prm) as s:
+        nside, _ = s.grid()
+        assert np.prod(nside) > 64 * 2048
+        s.do_ensemble(60)
+        c = s.counters()
+        b, ov, tot, anyo = s.energy()
+        assert not anyo and c["bonds"] == tot
+        rr, OO, bb = s.state()
+        ob, oov, otot = oracle.energy(rr, OO, bb)
+        assert np.array_equal(b, ob) and not oov.any()
+        s.check_energy()
+        if ensemble == "npt":
+            assert c["accepted"][1] > 0

@@ -63,6 +63,7 @@ def _mean_se(x):
 
 @pytest.mark.parametrize("name,trials_per_cell", [("nvt_rho03_T020", 0), ("nvt_rho03_T020", 4),
                                                   ("nvt_rho05_T020", 0), ("nvt_rho05_T020", 8),
+                                                  ("nvt_rho05_T020", 16),
                                                   ("npt_T020_P02", 0), ("npt_T020_P02", 4)])
 def test_equilibrium_matches_reference_runs(pp, tmp_path, name, trials_per_cell):
     ref = golden("stats_ref.json")[name]
