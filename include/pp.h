@@ -39,6 +39,7 @@
  *   pp_get_grid                   no reference counterpart (launch-shape
  *                                 tuning, per-sweep energy series, kernel
  *                                 timing, grid introspection)
+ *   pp_bond_list                  output_print_bonds (output.c:213-268)
  *   pp_checkpoint_save/_load     .rrr restart (output.c:37-46, system.c:213-245),
  *                                 exact and with the random-number counter
  *   pp_net_*, pp_get_learn_counters
@@ -189,6 +190,13 @@ pp_status pp_get_timing(pp_ctx *ctx, const char *name, long long *launches, doub
  * shape, so tuning only advances the simulation.  Outputs may be NULL.
  * Per-pass launch form only. */
 pp_status pp_autotune(pp_ctx *ctx, int sweeps, int *threads_per_cell, int *block_size, double *us_per_sweep);
+
+/* Bonded partners of every particle of replica rep, the pairs of
+ * output_print_bonds (output.c:213-268, MC_interact's test):
+ * count[i] = number of particles bonded to particle i, and their indices at
+ * partners[i * max_per + k] for k < min(count[i], max_per), in no particular
+ * order (the host program sorts them into the reference's cell order). */
+pp_status pp_bond_list(pp_ctx *ctx, int rep, int max_per, int32_t *count, int32_t *partners);
 
 /* Checkpoint (no reference counterpart; the reference restarts from .rrr
  * text, output.c:37-46 and system.c:213-245, which rounds the state): the

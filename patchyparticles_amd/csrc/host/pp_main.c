@@ -50,7 +50,11 @@ typedef struct run {
 	int npt;
 	double *r, *O;
 	char chk[1024];  /* checkpoint path ("" : none) */
+	double kf_delta;
+	int *bcount, *bpart; /* bond list of Print_bonds */
 } run;
+
+#define MAX_BONDS 16 /* bonded partners listed per particle */
 
 static void check(run *u, pp_status st, const char *what) {
 	if(st == PP_OK) return;
@@ -76,6 +80,14 @@ static void print_step(run *u, long long step) {
 	ppo_print(&u->out, step, u->n, -(double) c.bonds, c.volume, c.accepted[0], c.tries[0], c.accepted[1], c.tries[1]);
 	check(u, pp_reset_counters(u->ctx), "pp_reset_counters");
 	save_conf(u, step, u->out.last);
+	if(u->out.print_bonds) {
+		double box[3];
+		check(u, pp_download(u->ctx, 0, u->r, NULL, box), "pp_download");
+		check(u, pp_bond_list(u->ctx, 0, MAX_BONDS, u->bcount, u->bpart), "pp_bond_list");
+		for(int i = 0; i < u->n; i++)
+			if(u->bcount[i] > MAX_BONDS) ppo_exit(&u->out, "particle %d has more than %d bonded partners\n", i, MAX_BONDS);
+		ppo_print_bonds(&u->out, step, u->n, box, u->r, u->kf_delta, u->bcount, u->bpart, MAX_BONDS);
+	}
 }
 
 /* MC_check_energy + cells_check (MC.c:766-784, cells.c:65-76) */
@@ -155,7 +167,10 @@ int main(int argc, char *argv[]) {
 	pprrr_free(&c);
 	u.r = malloc(sizeof(double) * 3 * (size_t) (u.n > 0 ? u.n : 1));
 	u.O = malloc(sizeof(double) * 9 * (size_t) (u.n > 0 ? u.n : 1));
-	if(!u.r || !u.O) ppo_exit(&u.out, "out of memory\n");
+	u.kf_delta = prm.kf_delta;
+	u.bcount = malloc(sizeof(int) * (size_t) (u.n > 0 ? u.n : 1));
+	u.bpart = malloc(sizeof(int) * (size_t) MAX_BONDS * (size_t) (u.n > 0 ? u.n : 1));
+	if(!u.r || !u.O || !u.bcount || !u.bpart) ppo_exit(&u.out, "out of memory\n");
 
 	int resume = 0;
 	u.chk[0] = '\0';
@@ -209,6 +224,8 @@ int main(int argc, char *argv[]) {
 	pp_destroy(u.ctx);
 	free(u.r);
 	free(u.O);
+	free(u.bcount);
+	free(u.bpart);
 	ppo_free(&u.out);
 	ppi_free(in);
 	return 0;

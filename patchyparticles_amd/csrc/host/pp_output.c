@@ -4,6 +4,9 @@
  */
 #include "pp_output.h"
 
+#include <float.h>
+#include <math.h>
+
 #include <stdarg.h>
 #include <stdlib.h>
 #include <string.h>
@@ -93,9 +96,76 @@ void ppo_init(ppo *o, ppi_file *in, int ensemble, const char *initial_conf) {
 		ppi_string(in, "Density_file", name, sizeof name, 0);
 		o->density = open_or_die(o, "Density file", name, mode);
 	}
-	int bonds = 0;
-	if(ppi_int(in, "Print_bonds", &bonds, 0) == PPI_FOUND && bonds)
-		ppo_log(o, "WARNING: Print_bonds is not supported by this program\n");
+	o->print_bonds = 0;
+	ppi_int(in, "Print_bonds", &o->print_bonds, 0);
+	if(o->print_bonds) {
+		if(o->npt) ppo_exit(o, "The printing of bonds is unavailable in non-canonical ensembles");
+		ppi_string(in, "Bonds_folder", o->bonds_folder, sizeof o->bonds_folder, 1);
+		if(access(o->bonds_folder, W_OK) != 0)
+			ppo_exit(o, "Cannot create files in directory '%s': please make sure that the directory exists and it is "
+			            "writable\n",
+			         o->bonds_folder);
+	}
+}
+
+/* the reference's cell of a position (cells_fill_and_get_idx_from_vector) */
+static void ref_cell(const double r[3], const double box[3], const int ns[3], int idx[3]) {
+	for(int k = 0; k < 3; k++) idx[k] = (int) ((r[k] / box[k] - floor(r[k] / box[k])) * (1. - DBL_EPSILON) * ns[k]);
+}
+
+/* loop offset (-1, 0, 1) of cell b from cell a along an axis of ns cells */
+static int cell_offset(int a, int b, int ns) {
+	for(int j = -1; j < 2; j++)
+		if((a + j + ns) % ns == b) return j;
+	return 2; /* not a neighbour cell: listed last */
+}
+
+void ppo_print_bonds(ppo *o, long long step, int n, const double box[3], const double *r, double kf_delta,
+                     const int *count, const int *partners, int max_per) {
+	char name[1100];
+	snprintf(name, sizeof name, "%s/bonds_%lld.dat", o->bonds_folder, step);
+	FILE *out = fopen(name, "w");
+	if(!out) ppo_exit(o, "File '%s' is not writable\n", name);
+	const double rcut = 1. + kf_delta;
+	int ns[3];
+	for(int k = 0; k < 3; k++) {
+		ns[k] = (int) floor(box[k] / rcut);
+		if(ns[k] < 3) ns[k] = 3;
+	}
+	fprintf(out, "%d\n", n);
+	long long *key = malloc(sizeof(long long) * (size_t) (max_per > 0 ? max_per : 1));
+	int *lst = malloc(sizeof(int) * (size_t) (max_per > 0 ? max_per : 1));
+	if(!key || !lst) ppo_exit(o, "out of memory\n");
+	for(int i = 0; i < n; i++) {
+		const int m = count[i] < max_per ? count[i] : max_per;
+		int ci[3];
+		ref_cell(r + 3 * i, box, ns, ci);
+		for(int k = 0; k < m; k++) {
+			const int q = partners[(size_t) i * max_per + k];
+			int cq[3];
+			ref_cell(r + 3 * q, box, ns, cq);
+			const int o0 = cell_offset(ci[0], cq[0], ns[0]), o1 = cell_offset(ci[1], cq[1], ns[1]),
+			          o2 = cell_offset(ci[2], cq[2], ns[2]);
+			/* loop order over the 27 cells, then decreasing index */
+			key[k] = ((long long) ((o0 + 1) * 16 + (o1 + 1) * 4 + (o2 + 1)) << 32) | (unsigned) (0x7fffffff - q);
+			lst[k] = q;
+		}
+		for(int a = 1; a < m; a++) /* insertion sort: a handful of partners */
+			for(int b = a; b > 0 && key[b - 1] > key[b]; b--) {
+				const long long tk = key[b];
+				key[b] = key[b - 1];
+				key[b - 1] = tk;
+				const int tq = lst[b];
+				lst[b] = lst[b - 1];
+				lst[b - 1] = tq;
+			}
+		fprintf(out, "%d %d\n", i, count[i]);
+		for(int k = 0; k < m; k++) fprintf(out, "%d ", lst[k]);
+		fprintf(out, "\n");
+	}
+	free(key);
+	free(lst);
+	fclose(out);
 }
 
 void ppo_free(ppo *o) {

@@ -22,6 +22,8 @@ typedef struct ppo {
 	char folder[512];       /* Configuration_folder */
 	char last[512];         /* Configuration_last (last.rrr) */
 	int npt;
+	int print_bonds;        /* Print_bonds (NVT only) */
+	char bonds_folder[512]; /* Bonds_folder */
 } ppo;
 
 /* Reads the output keys and opens the files; exits with a message on error. */
@@ -39,5 +41,15 @@ void ppo_exit(ppo *o, const char *fmt, ...);
  * since the previous print. */
 void ppo_print(ppo *o, long long step, int n, double energy, double volume, long long acc_roto, long long try_roto,
                long long acc_vol, long long try_vol);
+
+/* <Bonds_folder>/bonds_<step>.dat (output.c:205-268): N, then per particle
+ * "index count" and the line of its bonded partners, each followed by a
+ * space.  count[i] partners of particle i are at partners[i * max_per + k];
+ * they are listed in the reference's order for a freshly filled cell list:
+ * the 27 cells around i on the reference grid (side >= 1 + kf_delta, at least
+ * 3 per side, cells.c:17-49) in loop order, within a cell by decreasing index
+ * (cells_fill pushes to the front, cells.c:51-62). */
+void ppo_print_bonds(ppo *o, long long step, int n, const double box[3], const double *r, double kf_delta,
+                     const int *count, const int *partners, int max_per);
 
 #endif

@@ -89,11 +89,11 @@ struct pp_ctx {
 
 	~pp_ctx() {
 		for(auto &p : pending) {
-			hipEventDestroy(p.second.first);
-			hipEventDestroy(p.second.second);
+			(void) hipEventDestroy(p.second.first);
+			(void) hipEventDestroy(p.second.second);
 		}
-		for(void *p : allocs) hipFree(p);
-		if(stream) hipStreamDestroy(stream);
+		for(void *p : allocs) (void) hipFree(p);
+		if(stream) (void) hipStreamDestroy(stream);
 	}
 };
 
@@ -160,18 +160,30 @@ pp_status check_err(pp_ctx *c) {
 	return PP_OK;
 }
 
+// Timing is best effort: a launch whose events cannot be created or recorded
+// is simply not timed.
 void tic(pp_ctx *c, int kind, hipEvent_t *a) {
-	if(!c->timing) return;
-	hipEventCreate(a);
-	hipEventRecord(*a, c->stream);
 	(void) kind;
+	*a = nullptr;
+	if(!c->timing) return;
+	if(hipEventCreate(a) != hipSuccess) {
+		*a = nullptr;
+		return;
+	}
+	if(hipEventRecord(*a, c->stream) != hipSuccess) {
+		(void) hipEventDestroy(*a);
+		*a = nullptr;
+	}
 }
 
 void toc(pp_ctx *c, int kind, hipEvent_t a) {
-	if(!c->timing) return;
-	hipEvent_t b;
-	hipEventCreate(&b);
-	hipEventRecord(b, c->stream);
+	if(!c->timing || !a) return;
+	hipEvent_t b = nullptr;
+	if(hipEventCreate(&b) != hipSuccess || hipEventRecord(b, c->stream) != hipSuccess) {
+		if(b) (void) hipEventDestroy(b);
+		(void) hipEventDestroy(a);
+		return;
+	}
 	c->pending.push_back({kind, {a, b}});
 }
 
@@ -180,11 +192,12 @@ pp_status collect_timing(pp_ctx *c) {
 	HIPCHECK(hipStreamSynchronize(c->stream));
 	for(auto &p : c->pending) {
 		float ms = 0.f;
-		hipEventElapsedTime(&ms, p.second.first, p.second.second);
-		c->t_ms[p.first] += ms;
-		c->t_launch[p.first] += 1;
-		hipEventDestroy(p.second.first);
-		hipEventDestroy(p.second.second);
+		if(hipEventElapsedTime(&ms, p.second.first, p.second.second) == hipSuccess) {
+			c->t_ms[p.first] += ms;
+			c->t_launch[p.first] += 1;
+		}
+		(void) hipEventDestroy(p.second.first);
+		(void) hipEventDestroy(p.second.second);
 	}
 	c->pending.clear();
 	return PP_OK;
@@ -359,8 +372,8 @@ pp_status create_impl(int R, const pp_params *prm, const pp_options *opts, const
 		// the passes of a sweep as one cooperative launch: as many workgroups
 		// as the lists of one pass can use, at most as many as stay resident
 		int coop = 0, cus = 0;
-		hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, o.device);
-		hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, o.device);
+		if(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, o.device) != hipSuccess) coop = 0;
+		if(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, o.device) != hipSuccess) cus = 0;
 		const int tpb = o.block_size / Tv;
 		const int need = std::max(1, (d.total_act + tpb - 1) / tpb);
 		const int fit = (coop && cus > 0) ? sweep_blocks_max(Tv, o.block_size, cus) : 0;
@@ -995,6 +1008,32 @@ pp_status pp_checkpoint_load(pp_ctx *c, const char *path) {
 	return PP_OK;
 }
 
+pp_status pp_bond_list(pp_ctx *c, int rep, int max_per, int32_t *count, int32_t *partners) {
+	if(!c || !count || (max_per > 0 && !partners) || max_per < 0) return fail(PP_ERR_ARG, "bad arguments");
+	if(rep < 0 || rep >= c->R) return fail(PP_ERR_ARG, "replica %d out of range", rep);
+	HIPCHECK(hipSetDevice(c->device));
+	TRY(pull_reps(c));
+	const int n = c->np[rep], off = c->p_off[rep];
+	int *dc = nullptr, *dp = nullptr;
+	const size_t np_all = (size_t) std::max(c->d.Ntot, 1);
+	if(hipMalloc(&dc, sizeof(int) * np_all) != hipSuccess ||
+	   hipMalloc(&dp, sizeof(int) * np_all * (size_t) std::max(max_per, 1)) != hipSuccess) {
+		if(dc) (void) hipFree(dc);
+		return fail(PP_ERR_NOMEM, "hipMalloc of the bond list failed");
+	}
+	rebuild_now(c);  // every particle in the cell of its position
+	launch_bond_list(c->d, max_per, dc, dp, c->stream);
+	hipError_t e = hipStreamSynchronize(c->stream);
+	if(e == hipSuccess) e = hipGetLastError();
+	if(e == hipSuccess && n > 0) e = hipMemcpy(count, dc + off, sizeof(int) * n, hipMemcpyDeviceToHost);
+	if(e == hipSuccess && n > 0 && max_per > 0)
+		e = hipMemcpy(partners, dp + (size_t) off * max_per, sizeof(int) * (size_t) n * max_per, hipMemcpyDeviceToHost);
+	(void) hipFree(dc);
+	(void) hipFree(dp);
+	if(e != hipSuccess) return fail(PP_ERR_HIP, "bond list: %s", hipGetErrorString(e));
+	return check_err(c);
+}
+
 pp_status pp_get_learn_counters(pp_ctx *c, int rep, long long *tries, long long *accepted) {
 	if(!c) return fail(PP_ERR_ARG, "ctx is NULL");
 	if(rep < 0 || rep >= c->R) return fail(PP_ERR_ARG, "replica %d out of range", rep);
@@ -1092,8 +1131,8 @@ pp_status pp_get_grid(pp_ctx *c, int rep, int n_side[3], double side[3]) {
 
 void pp_destroy(pp_ctx *c) {
 	if(!c) return;
-	hipSetDevice(c->device);
-	if(c->stream) hipStreamSynchronize(c->stream);
+	(void) hipSetDevice(c->device);
+	if(c->stream) (void) hipStreamSynchronize(c->stream);
 	delete c;
 }
 

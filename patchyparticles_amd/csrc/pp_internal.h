@@ -204,6 +204,7 @@ int sweep_blocks_max(int T, int block, int cus);  // co-resident workgroups of t
 size_t pass_lds_bytes(int T, int block);  // dynamic LDS of one pass workgroup
 constexpr size_t MAX_LDS_BYTES = 65536;
 void launch_energy(const DevState &d, int trial, int per_particle, hipStream_t s);
+void launch_bond_list(const DevState &d, int max_per, int *count, int *partners, hipStream_t s);
 // volume-move attempt k of sweep `sweep` (NPT replicas only)
 void launch_vol_propose(const DevState &d, uint64_t sweep, int k, hipStream_t s);
 void launch_vol_decide(const DevState &d, uint64_t sweep, int k, hipStream_t s);

@@ -958,6 +958,58 @@ __global__ __launch_bounds__(256) void k_energy(DevState d, int trial, int per_p
 	}
 }
 
+// Bonded partners of every particle (the pairs output_print_bonds lists,
+// output.c:213-268, with MC_interact's test): slot j's partners' particle
+// indices at partners[(p_off + pid) * max_per + k], k < max_per, and their
+// number at count[p_off + pid] (which may exceed max_per).  One thread per
+// slot; an output path, not a hot one.
+__global__ void k_bond_list(DevState d, int max_per, int *count, int *partners) {
+	const int j = blockIdx.x * blockDim.x + threadIdx.x;
+	if(j >= d.Ntot) return;
+	const int r = d.prep[j];
+	const Rep &rp = d.reps[r];
+	const Grid &G = rp.g;
+	const double *F = d.f[rp.sel];
+	const int *PID = d.pid[rp.sel];
+	const int S = d.stride;
+	const double px = F[j], py = F[S + j], pz = F[2 * S + j];
+	double O[9];
+	for(int c = 0; c < 9; c++) O[c] = F[(3 + c) * S + j];
+	double P[4][3];
+	kf_patches(O, P);
+	const int n0 = G.n[0], n1 = G.n[1], n2 = G.n[2];
+	const int ix = cell_coord(px, G.shift[0], G.inv_box[0], n0);
+	const int iy = cell_coord(py, G.shift[1], G.inv_box[1], n1);
+	const int iz = cell_coord(pz, G.shift[2], G.inv_box[2], n2);
+	const size_t me = (size_t) rp.p_off + PID[j];
+	int nb = 0;
+	for(int c = 0; c < 27; c++) {
+		const int cx = wrapi(ix + c / 9 - 1, n0), cy = wrapi(iy + (c / 3) % 3 - 1, n1), cz = wrapi(iz + c % 3 - 1, n2);
+		const int cc = rp.cell_off + (cx * n1 + cy) * n2 + cz;
+		for(int q = d.start[cc]; q < d.start[cc + 1]; q++) {
+			if(q == j) continue;
+			double dx = kf_min_image(F[q] - px, G.box[0], G.inv_box[0]);
+			double dy = kf_min_image(F[S + q] - py, G.box[1], G.inv_box[1]);
+			double dz = kf_min_image(F[2 * S + q] - pz, G.box[2], G.inv_box[2]);
+			const double d2 = dx * dx + dy * dy + dz * dz;
+			if(d2 < 1. || !(d2 < rp.rc2)) continue;
+			kf_unit(dx, dy, dz, d2);
+			if(!kf_any_p(dx, dy, dz, &P[0][0], rp.cosmax)) continue;
+			double Oq[9];
+			for(int e = 0; e < 9; e++) Oq[e] = F[(3 + e) * S + q];
+			if(!kf_any_q(dx, dy, dz, Oq, rp.cosmax)) continue;
+			if(nb < max_per) partners[me * max_per + nb] = PID[q];
+			nb++;
+		}
+	}
+	count[me] = nb;
+}
+
+void launch_bond_list(const DevState &d, int max_per, int *count, int *partners, hipStream_t s) {
+	if(d.Ntot > 0)
+		hipLaunchKernelGGL(k_bond_list, dim3((d.Ntot + 255) / 256), dim3(256), 0, s, d, max_per, count, partners);
+}
+
 // Lanes per particle: 32 where the 27 cells hold many candidates, 8 in dilute
 // systems (fewer than 8 candidates on average over the cell capacity), where
 // most of a 32-lane team would idle.  The result does not depend on it.

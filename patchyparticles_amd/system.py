@@ -175,6 +175,16 @@ class System:
         check(lib().pp_get_timing(self._h, None if name is None else name.encode(), C.byref(nl), C.byref(ms)))
         return nl.value, ms.value
 
+    def bond_list(self, rep=0, max_per=16):
+        """Bonded partners of every particle: a list of sorted index arrays."""
+        n = self.n[rep]
+        cnt = np.zeros(max(n, 1), np.int32)
+        part = np.zeros(max(n, 1) * max_per, np.int32)
+        check(lib().pp_bond_list(self._h, rep, max_per, _ptr(cnt), _ptr(part)))
+        if (cnt[:n] > max_per).any():
+            raise RuntimeError("more bonded partners than max_per")
+        return [np.sort(part[i * max_per:i * max_per + cnt[i]]) for i in range(n)]
+
     def checkpoint_save(self, path):
         """Exact state, counters and sweep index of every replica (bit-exact resume)."""
         check(lib().pp_checkpoint_save(self._h, str(path).encode()))

@@ -17,16 +17,21 @@ executables.  Outputs are data only (inputs and expected outputs):
   host_runs.json    the lines the reference program prints at step 0
                     (stdout, energy.dat, acceptance.dat, density.dat) on the
                     reference generator's N=1000 rho=0.1 start, NVT and NPT
+  bonds_start.rrr,  the reference program's Print_bonds file at step 0 of a
+  bonds_0_ref.dat,  run from the snap_rho05_T01 liquid (N=500) and the keys
+  bonds_keys.json   it ran with
 
     python tests/golden/make_golden.py            (everything)
     python tests/golden/make_golden.py host       (host_runs.json only)
     python tests/golden/make_golden.py stats      (stats_ref.json only)
+    python tests/golden/make_golden.py bonds      (the bond-list files only)
 """
 from __future__ import annotations
 
 import hashlib
 import json
 import os
+import shutil
 import subprocess
 import sys
 import tempfile
@@ -275,6 +280,34 @@ def host_runs():
     print("host_runs:", json.dumps(out)[:300])
 
 
+def bonds_ref():
+    """The reference program's bond list (Print_bonds = 1) at step 0 of a run
+    from a bonded liquid (snap_rho05_T01: N=500, rho=0.5), written by
+    output_print_bonds right after the cells were filled: bonds_start.rrr
+    (the input) and bonds_0_ref.dat (its output)."""
+    build_ref()
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    z = np.load(os.path.join(HERE, "snap_rho05_T01.npz"))
+    d = tempfile.mkdtemp(prefix="bonds_")
+    start = os.path.join(HERE, "bonds_start.rrr")
+    pyoracle.rrr_save(start, 0, z["box"], z["r"], z["O"])
+    shutil.copy(start, os.path.join(d, "start.rrr"))
+    os.mkdir(os.path.join(d, "bonds"))
+    keys = {"Dynamics": 0, "Ensemble": 0, "Seed": 3, "Temperature": 0.1, "Disp_max": 0.05, "Theta_max": 0.1,
+            "KF_delta": 0.12, "KF_cosmax": 0.92, "Initial_conditions_file": "start.rrr", "Steps": 1,
+            "Print_every": 1, "Save_every": 1000, "Configuration_folder": ".", "Print_bonds": 1,
+            "Bonds_folder": "bonds"}
+    with open(os.path.join(d, "input"), "w") as f:
+        f.write("".join(f"{k} = {v}\n" for k, v in keys.items()))
+    subprocess.run([os.path.join(ROOT, "oracle", "_ref", "PatchyParticles"), "input"], cwd=d, check=True,
+                   capture_output=True)
+    shutil.copy(os.path.join(d, "bonds", "bonds_0.dat"), os.path.join(HERE, "bonds_0_ref.dat"))
+    with open(os.path.join(HERE, "bonds_keys.json"), "w") as f:
+        json.dump(keys, f, indent=1)
+    print("bonds: reference bond list of", len(z["r"]), "particles")
+
+
 # state points of the statistical comparison: the reference program's own
 # output series (energy.dat, density.dat) from its generator's start
 STAT_POINTS = {
@@ -359,7 +392,10 @@ if __name__ == "__main__":
         host_runs()
     elif sys.argv[1:] == ["stats"]:
         stats_ref()
+    elif sys.argv[1:] == ["bonds"]:
+        bonds_ref()
     else:
         main()
         host_runs()
+        bonds_ref()
         stats_ref()

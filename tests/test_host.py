@@ -250,3 +250,26 @@ def test_driver_resume_from_checkpoint_is_bit_exact(host_built, tmp_path, ensemb
     _write_input(d, dict(g["keys"], Resume=1))
     res = subprocess.run([DRIVER, "input"], cwd=d, capture_output=True, text=True, timeout=60)
     assert res.returncode == 1 and "Resume = 1 needs Checkpoint_file" in res.stderr
+
+
+@pytest.mark.gpu
+def test_driver_bond_list_matches_reference_file(host_built, tmp_path):
+    """Print_bonds = 1: bonds_0.dat equals the reference program's file byte
+    for byte (bonded partners from the GPU, listed in the reference's cell
+    order)."""
+    keys = golden("bonds_keys.json")
+    (tmp_path / "start.rrr").write_bytes(open(os.path.join(ROOT, "tests", "golden", "bonds_start.rrr"), "rb").read())
+    (tmp_path / "bonds").mkdir()
+    _write_input(tmp_path, keys)
+    res = subprocess.run([DRIVER, "input"], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert res.returncode == 0, res.stderr
+    want = open(os.path.join(ROOT, "tests", "golden", "bonds_0_ref.dat"), "rb").read()
+    assert (tmp_path / "bonds" / "bonds_0.dat").read_bytes() == want
+
+
+def test_driver_refuses_bonds_in_npt(host_built, tmp_path):
+    keys = dict(golden("host_runs.json")["npt"]["keys"], Print_bonds=1, Bonds_folder=".")
+    subprocess.run([GENERATOR, "100", "0.1"], cwd=tmp_path, check=True, capture_output=True)
+    _write_input(tmp_path, keys)
+    res = subprocess.run([DRIVER, "input"], cwd=tmp_path, capture_output=True, text=True)
+    assert res.returncode == 1 and "The printing of bonds is unavailable in non-canonical ensembles" in res.stderr
