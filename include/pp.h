@@ -40,6 +40,8 @@
  *                                 tuning, per-sweep energy series, kernel
  *                                 timing, grid introspection)
  *   pp_bond_list                  output_print_bonds (output.c:213-268)
+ *   pp_set_lx_move                the Lx_move keys and MC_change_Lx
+ *                                 (system.c:170-196, MC.c:515-585)
  *   pp_checkpoint_save/_load     .rrr restart (output.c:37-46, system.c:213-245),
  *                                 exact and with the random-number counter
  *   pp_net_*, pp_get_learn_counters
@@ -190,6 +192,15 @@ pp_status pp_get_timing(pp_ctx *ctx, const char *name, long long *launches, doub
  * shape, so tuning only advances the simulation.  Outputs may be NULL.
  * Per-pass launch form only. */
 pp_status pp_autotune(pp_ctx *ctx, int sweeps, int *threads_per_cell, int *block_size, double *us_per_sweep);
+
+/* Box-shape moves (MC_change_Lx, MC.c:515-585; keys Lx_move, Lx_change_max,
+ * Lyz_min, Lyz_max, system.c:170-196) for replica rep (-1: all), NVT only:
+ * about once per N particle attempts, Lx += U(-1/2, 1/2) lx_change_max with
+ * Ly = Lz = sqrt(V / Lx), rejected outside [lyz_min, lyz_max], accepted with
+ * min(1, exp(-dE/T)).  Their tries / acceptances are counted in tries[1] /
+ * accepted[1] of pp_counters.  lx_change_max <= 0 turns them off.  Errors
+ * as the reference's: Ly != Lz, lyz_min >= Ly, lyz_max <= Ly. */
+pp_status pp_set_lx_move(pp_ctx *ctx, int rep, double lx_change_max, double lyz_min, double lyz_max);
 
 /* Bonded partners of every particle of replica rep, the pairs of
  * output_print_bonds (output.c:213-268, MC_interact's test):

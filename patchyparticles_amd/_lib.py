@@ -21,7 +21,7 @@ EXPORTS = (
     "pp_strerror", "pp_last_error", "pp_autotune",
     "pp_net_create", "pp_net_set", "pp_net_get", "pp_net_forward", "pp_net_loss_grad", "pp_net_destroy",
     "pp_net_attach", "pp_net_train", "pp_get_learn_counters", "pp_net_descriptor",
-    "pp_checkpoint_save", "pp_checkpoint_load", "pp_bond_list",
+    "pp_checkpoint_save", "pp_checkpoint_load", "pp_bond_list", "pp_set_lx_move",
 )
 
 # sizes of the proposal network (PP_NET_* in include/pp.h)
@@ -167,6 +167,8 @@ def lib():
     L.pp_checkpoint_load.restype = st
     L.pp_bond_list.argtypes = [_vp, C.c_int, C.c_int, _vp, _vp]
     L.pp_bond_list.restype = st
+    L.pp_set_lx_move.argtypes = [_vp, C.c_int, C.c_double, C.c_double, C.c_double]
+    L.pp_set_lx_move.restype = st
     L.pp_destroy.argtypes = [_vp]
     L.pp_destroy.restype = None
     L.pp_strerror.argtypes = [st]

@@ -19,7 +19,9 @@
  * (written with every last.rrr: the exact state and the random-number
  * counter), Resume (1: continue bit for bit from Checkpoint_file; the
  * Initial_conditions_file then only gives N, normally the last.rrr written
- * with the checkpoint).
+ * with the checkpoint).  Reference keys beyond the ones above: Print_bonds /
+ * Bonds_folder (bond lists) and Lx_move / Lx_change_max / Lyz_min / Lyz_max
+ * (box-shape moves, boxshape.dat).
  */
 #include <signal.h>
 #include <stdio.h>
@@ -77,7 +79,8 @@ static void save_checkpoint(run *u) {
 static void print_step(run *u, long long step) {
 	pp_counters c;
 	check(u, pp_get_counters(u->ctx, 0, &c), "pp_get_counters");
-	ppo_print(&u->out, step, u->n, -(double) c.bonds, c.volume, c.accepted[0], c.tries[0], c.accepted[1], c.tries[1]);
+	ppo_print(&u->out, step, u->n, -(double) c.bonds, c.volume, c.box, c.accepted[0], c.tries[0], c.accepted[1],
+	          c.tries[1]);
 	check(u, pp_reset_counters(u->ctx), "pp_reset_counters");
 	save_conf(u, step, u->out.last);
 	if(u->out.print_bonds) {
@@ -149,10 +152,22 @@ int main(int argc, char *argv[]) {
 		ppi_double(in, "rescale_factor_max", &prm.rescale_factor_max, 1);
 		ppi_double(in, "P", &prm.pressure, 1);
 	}
-	else {
-		int lx = 0;
-		if(ppi_int(in, "Lx_move", &lx, 0) == PPI_FOUND && lx) ppo_exit(&u.out, "Lx_move is not supported by this program\n");
+	/* box-shape moves (system.c:170-196): NVT only */
+	int lx = 0;
+	double lx_change_max = 0., lyz_min = 0., lyz_max = 0.;
+	if(prm.ensemble != PP_NPT) {
+		ppi_int(in, "Lx_move", &lx, 0);
+		if(lx) {
+			if(c.box[1] != c.box[2]) ppo_exit(&u.out, "Lx_move = 1 requires Ly = Lz\n");
+			ppi_double(in, "Lx_change_max", &lx_change_max, 1);
+			ppi_double(in, "Lyz_min", &lyz_min, 1);
+			ppi_double(in, "Lyz_max", &lyz_max, 1);
+			if(lyz_min >= c.box[1]) ppo_exit(&u.out, "Lyz_min should be smaller than box size\n");
+			if(lyz_max <= c.box[1]) ppo_exit(&u.out, "Lyz_max should be larger than box size\n");
+			ppo_log(&u.out, "Ly and Lz are allowed to vary between %lf and %lf\n", lyz_min, lyz_max);
+		}
 	}
+	u.out.lx = lx != 0;
 
 	pp_options opt;
 	memset(&opt, 0, sizeof opt);
@@ -164,6 +179,7 @@ int main(int argc, char *argv[]) {
 	const pp_status st = pp_create(&prm, &opt, c.n, c.r, c.O, c.box, &u.ctx);
 	if(st == PP_ERR_OVERLAP) ppo_exit(&u.out, "Initial configuration contains an overlap, aborting\n");
 	if(st != PP_OK) ppo_exit(&u.out, "pp_create failed: %s (%s)\n", pp_strerror(st), pp_last_error());
+	if(lx) check(&u, pp_set_lx_move(u.ctx, 0, lx_change_max, lyz_min, lyz_max), "pp_set_lx_move");
 	pprrr_free(&c);
 	u.r = malloc(sizeof(double) * 3 * (size_t) (u.n > 0 ? u.n : 1));
 	u.O = malloc(sizeof(double) * 9 * (size_t) (u.n > 0 ? u.n : 1));

@@ -106,6 +106,12 @@ void ppo_init(ppo *o, ppi_file *in, int ensemble, const char *initial_conf) {
 			            "writable\n",
 			         o->bonds_folder);
 	}
+	int shape = 0;
+	ppi_int(in, "Lx_move", &shape, 0);
+	if(shape) {
+		o->boxshape = fopen("boxshape.dat", mode);
+		if(!o->boxshape) ppo_exit(o, "boxshape.dat is not writable\n");
+	}
 }
 
 /* the reference's cell of a position (cells_fill_and_get_idx_from_vector) */
@@ -172,12 +178,13 @@ void ppo_free(ppo *o) {
 	if(o->energy) fclose(o->energy);
 	if(o->acc) fclose(o->acc);
 	if(o->density) fclose(o->density);
+	if(o->boxshape) fclose(o->boxshape);
 	if(o->log && o->log != stderr) fclose(o->log);
 	memset(o, 0, sizeof(*o));
 }
 
-void ppo_print(ppo *o, long long step, int n, double energy, double volume, long long acc_roto, long long try_roto,
-               long long acc_vol, long long try_vol) {
+void ppo_print(ppo *o, long long step, int n, double energy, double volume, const double box[3], long long acc_roto,
+               long long try_roto, long long acc_vol, long long try_vol) {
 	const double e = n > 0 ? energy / n : 0.;
 	fprintf(o->energy, "%lld %lf\n", step, e);
 	fflush(o->energy);
@@ -185,9 +192,13 @@ void ppo_print(ppo *o, long long step, int n, double energy, double volume, long
 		fprintf(o->density, "%lld %lf %d\n", step, n / volume, n);
 		fflush(o->density);
 	}
+	if(o->lx && o->boxshape) {
+		fprintf(o->boxshape, "%lld %lf %lf %lf\n", step, box[0], box[1], box[2]);
+		fflush(o->boxshape);
+	}
 	fprintf(o->acc, "%lld", step);
 	fprintf(o->acc, " %e", acc_roto / (double) try_roto);
-	if(o->npt) fprintf(o->acc, " %e", acc_vol / (double) try_vol);
+	if(o->npt || o->lx) fprintf(o->acc, " %e", acc_vol / (double) try_vol);
 	fprintf(o->acc, "\n");
 	fflush(o->acc);
 	printf("%lld %lf %lf", step, e, energy);

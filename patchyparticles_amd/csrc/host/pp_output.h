@@ -23,6 +23,8 @@ typedef struct ppo {
 	char last[512];         /* Configuration_last (last.rrr) */
 	int npt;
 	int print_bonds;        /* Print_bonds (NVT only) */
+	int lx;                 /* box-shape moves (set by the driver, NVT only) */
+	FILE *boxshape;         /* boxshape.dat, opened when Lx_move is set (output.c:101-108) */
 	char bonds_folder[512]; /* Bonds_folder */
 } ppo;
 
@@ -36,11 +38,13 @@ void ppo_log(ppo *o, const char *fmt, ...);
 void ppo_exit(ppo *o, const char *fmt, ...);
 
 /* The print-step lines: energy.dat "step E/N", density.dat "step N/V N"
- * (NPT), acceptance.dat "step acc_roto [acc_volume]", stdout
- * "step E/N E [N/V]" (output.c:119-201).  Acceptances are accepted / tries
- * since the previous print. */
-void ppo_print(ppo *o, long long step, int n, double energy, double volume, long long acc_roto, long long try_roto,
-               long long acc_vol, long long try_vol);
+ * (NPT), boxshape.dat "step Lx Ly Lz" (box-shape moves), acceptance.dat
+ * "step acc_roto [acc_volume | acc_Lx]", stdout "step E/N E [N/V]"
+ * (output.c:119-201).  Acceptances are accepted / tries since the previous
+ * print; acc_vol / try_vol count the volume moves (NPT) or the box-shape
+ * moves (NVT). */
+void ppo_print(ppo *o, long long step, int n, double energy, double volume, const double box[3], long long acc_roto,
+               long long try_roto, long long acc_vol, long long try_vol);
 
 /* <Bonds_folder>/bonds_<step>.dat (output.c:205-268): N, then per particle
  * "index count" and the line of its bonded partners, each followed by a

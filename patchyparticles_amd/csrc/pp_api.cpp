@@ -73,6 +73,7 @@ struct pp_ctx {
 	int coop_blocks = 0;  // > 0: the passes of a sweep run as one cooperative launch of that many workgroups
 	long long sweep = 0;
 	bool any_npt = false;
+	bool any_lx = false;  // a replica makes box-shape moves
 	long long trace_len = -1;  // entries written to d.trace; -1 = tracing off
 	long long trace_from = 0;  // sweep index at which tracing began
 	// timing
@@ -511,10 +512,11 @@ pp_status enqueue_sweep(pp_ctx *c) {
 				toc(c, TK_PASS, b);
 			}
 		}
-	if(c->any_npt) {
-		// one volume attempt per N particle attempts on average, as do_NPT
-		// (MC.c:60-68): with s trials per non-empty cell a step makes
-		// about s x (non-empty cells) <= s N attempts, and s volume attempts
+	if(c->any_npt || c->any_lx) {
+		// one volume (or box-shape) attempt per N particle attempts on
+		// average, as do_NPT and do_NVT (MC.c:19-29, 60-68): with s trials per
+		// non-empty cell a step makes about s x (non-empty cells) <= s N
+		// attempts, and s volume attempts
 		const int nvol = c->d.trials_per_cell > 0 ? c->d.trials_per_cell : 1;
 		for(int k = 0; k < nvol; k++) {
 			hipEvent_t v{};
@@ -1006,6 +1008,32 @@ pp_status pp_checkpoint_load(pp_ctx *c, const char *path) {
 	HIPCHECK(hipMemcpy(c->d.ctr, hc.data(), hc.size() * sizeof(unsigned long long), hipMemcpyHostToDevice));
 	c->sweep = h.sweep;
 	return PP_OK;
+}
+
+pp_status pp_set_lx_move(pp_ctx *c, int rep, double lx_change_max, double lyz_min, double lyz_max) {
+	if(!c) return fail(PP_ERR_ARG, "ctx is NULL");
+	if(rep < -1 || rep >= c->R) return fail(PP_ERR_ARG, "replica %d out of range", rep);
+	HIPCHECK(hipSetDevice(c->device));
+	TRY(pull_reps(c));
+	const int lo = rep < 0 ? 0 : rep, hi = rep < 0 ? c->R : rep + 1;
+	for(int k = lo; k < hi; k++) {
+		const Rep &rp = c->reps[k];
+		if(lx_change_max <= 0.) continue;  // off
+		if(rp.ensemble != PP_NVT) return fail(PP_ERR_UNSUPPORTED, "box-shape moves need the NVT ensemble (replica %d)", k);
+		if(rp.g.box[1] != rp.g.box[2]) return fail(PP_ERR_ARG, "Lx_move = 1 requires Ly = Lz");
+		if(!(lyz_min < rp.g.box[1])) return fail(PP_ERR_ARG, "Lyz_min should be smaller than box size");
+		if(!(lyz_max > rp.g.box[1])) return fail(PP_ERR_ARG, "Lyz_max should be larger than box size");
+	}
+	for(int k = lo; k < hi; k++) {
+		Rep &rp = c->reps[k];
+		rp.lx_move = lx_change_max > 0. ? 1 : 0;
+		rp.lx_change_max = lx_change_max;
+		rp.lyz_min = lyz_min;
+		rp.lyz_max = lyz_max;
+	}
+	c->any_lx = false;
+	for(const Rep &rp : c->reps) c->any_lx = c->any_lx || rp.lx_move;
+	return push_reps(c);
 }
 
 pp_status pp_bond_list(pp_ctx *c, int rep, int max_per, int32_t *count, int32_t *partners) {

@@ -117,6 +117,10 @@ struct Rep {
 	double V;          // the volume the reference tracks (MC.c:500)
 	long long bonds_base;  // bonded pairs = bonds_base + sum of C_DBONDS slots
 	double trial_f, trial_V, trial_V0;
+	double trial_s[3];        // scale of each position component in the trial (1: unchanged)
+	int trial_kind;           // 0: volume move (NPT), 1: Lx move (NVT with box-shape moves)
+	int lx_move;              // box-shape moves on (MC_change_Lx, MC.c:515-585)
+	double lx_change_max, lyz_min, lyz_max;
 	double acc_tab[ACC_TAB];  // exp(-k / T): the acceptance factor of an energy rise of k bonds
 };
 

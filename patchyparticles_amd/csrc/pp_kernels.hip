@@ -110,7 +110,8 @@ __global__ void k_count(DevState d, int mode, uint64_t sweep, long long trace_po
 	const double *F = d.f[rp.sel];
 	const int S = d.stride;
 	double p[3] = {F[j], F[S + j], F[2 * S + j]};
-	if(tr) p[rp.trial_dir] *= rp.trial_f;
+	if(tr)
+		for(int k = 0; k < 3; k++) p[k] *= rp.trial_s[k];
 	int ix = cell_coord(p[0], sh[0], G.inv_box[0], G.n[0]);
 	int iy = cell_coord(p[1], sh[1], G.inv_box[1], G.n[1]);
 	int iz = cell_coord(p[2], sh[2], G.inv_box[2], G.n[2]);
@@ -403,7 +404,7 @@ __global__ void k_scatter(DevState d, int mode, uint64_t sweep) {
 #pragma unroll
 	for(int c = 0; c < NCOMP; c++) {
 		double v = F[c * S + j];
-		if(tr && c == rp.trial_dir) v *= rp.trial_f;
+		if(tr && c < 3) v *= rp.trial_s[c];
 		G[c * S + dst] = v;
 	}
 	d.pid[sb ^ 1][dst] = d.pid[sb][j];
@@ -1033,8 +1034,31 @@ __global__ void k_vol_propose(DevState d, uint64_t sweep, int k) {
 	if(r >= d.R) return;
 	Rep &rp = d.reps[r];
 	rp.trial_active = 0;
-	if(rp.ensemble != 5) return;
+	if(rp.ensemble != 5 && !rp.lx_move) return;
 	u32x4 v = rng(d.seed, r, sweep, (uint32_t) k, TAG_VOLUME, 0, 0);  // attempt k of the sweep
+	if(rp.ensemble != 5) {
+		// box-shape move at constant volume (MC_change_Lx, MC.c:515-531):
+		// Lx += U(-1/2, 1/2) Lx_change_max, Ly = Lz = sqrt(V / Lx), rejected
+		// early outside [Lyz_min, Lyz_max] (or, here, without a valid grid)
+		const double Lx = rp.g.box[0] + (u01(v.x) - 0.5) * rp.lx_change_max;
+		const double Lyz = Lx > 0. ? sqrt(rp.V / Lx) : -1.;
+		double tb[3] = {Lx, Lyz, Lyz};
+		int tn[3];
+		if(!(Lyz >= rp.lyz_min && Lyz <= rp.lyz_max) ||
+		   !make_grid(tb, rp.side_min, rp.min_n, rp.even, rp.cap_n, tn)) {
+			atomicAdd(ctr_at(d.ctr, r, C_TRY_VOL, 0), 1ull);
+			return;
+		}
+		for(int a = 0; a < 3; a++) rp.trial_s[a] = tb[a] / rp.g.box[a];
+		set_grid_box(rp.tg, tb);
+		for(int a = 0; a < 3; a++) {
+			rp.tg.n[a] = tn[a];
+			rp.tg.shift[a] = rp.g.shift[a] * rp.trial_s[a];
+		}
+		rp.trial_kind = 1;
+		rp.trial_active = 1;
+		return;
+	}
 	const int dir = (int) (u01(v.x) * 3.);
 	const double V0 = rp.g.box[0] * rp.g.box[1] * rp.g.box[2];
 	const double lnV1 = log(V0) + (u01(v.y) - 0.5) * rp.rfm;
@@ -1059,8 +1083,10 @@ __global__ void k_vol_propose(DevState d, uint64_t sweep, int k) {
 	}
 	rp.trial_dir = dir;
 	rp.trial_f = f;
+	for(int a = 0; a < 3; a++) rp.trial_s[a] = a == dir ? f : 1.;
 	rp.trial_V = V1;
 	rp.trial_V0 = V0;
+	rp.trial_kind = 0;
 	rp.trial_active = 1;
 }
 
@@ -1082,14 +1108,17 @@ __global__ void k_vol_decide(DevState d, uint64_t sweep, int k) {
 	const long long B1 = (long long) eb / 2;
 	const long long B0 = rp.bonds_base + (long long) db;
 	const double dE = (double) (B0 - B1);  // E' - E with E = -bonds
-	const double dV = rp.trial_V - rp.trial_V0;
-	const double arg = -(rp.P * dV + dE) / rp.T + (double) (rp.np + 1) * log(rp.trial_f);
+	// volume: MC.c:497-499; box shape: exp(-dE/T) at constant volume (the
+	// Jacobian of the rescaling is 1)
+	const double arg = rp.trial_kind == 1
+	                       ? -dE / rp.T
+	                       : -(rp.P * (rp.trial_V - rp.trial_V0) + dE) / rp.T + (double) (rp.np + 1) * log(rp.trial_f);
 	u32x4 v = rng(d.seed, r, sweep, (uint32_t) k, TAG_VOLUME, 0, 0);  // attempt k of the sweep
 	atomicAdd(ctr_at(d.ctr, r, C_TRY_VOL, 0), 1ull);
 	if(ov == 0 && u01(v.z) < exp(arg)) {
 		rp.sel ^= 1;
 		rp.g = rp.tg;
-		rp.V = rp.trial_V;
+		if(rp.trial_kind == 0) rp.V = rp.trial_V;
 		rp.bonds_base = B1 - (long long) db;
 		atomicAdd(ctr_at(d.ctr, r, C_ACC_VOL, 0), 1ull);
 	}

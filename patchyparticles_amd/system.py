@@ -175,6 +175,11 @@ class System:
         check(lib().pp_get_timing(self._h, None if name is None else name.encode(), C.byref(nl), C.byref(ms)))
         return nl.value, ms.value
 
+    def set_lx_move(self, lx_change_max, lyz_min, lyz_max, rep=-1):
+        """Box-shape moves at constant volume (NVT): Lx += U(-1/2, 1/2)
+        lx_change_max, Ly = Lz = sqrt(V / Lx) within [lyz_min, lyz_max]."""
+        check(lib().pp_set_lx_move(self._h, int(rep), float(lx_change_max), float(lyz_min), float(lyz_max)))
+
     def bond_list(self, rep=0, max_per=16):
         """Bonded partners of every particle: a list of sorted index arrays."""
         n = self.n[rep]

@@ -20,11 +20,14 @@ executables.  Outputs are data only (inputs and expected outputs):
   bonds_start.rrr,  the reference program's Print_bonds file at step 0 of a
   bonds_0_ref.dat,  run from the snap_rho05_T01 liquid (N=500) and the keys
   bonds_keys.json   it ran with
+  batch_ref.json    per-replica <E/N> of the reference program for 8 uneven
+                    state points (N, T, rho), 8 runs each
 
     python tests/golden/make_golden.py            (everything)
     python tests/golden/make_golden.py host       (host_runs.json only)
     python tests/golden/make_golden.py stats      (stats_ref.json only)
     python tests/golden/make_golden.py bonds      (the bond-list files only)
+    python tests/golden/make_golden.py batch      (batch_ref.json only)
 """
 from __future__ import annotations
 
@@ -308,6 +311,66 @@ def bonds_ref():
     print("bonds: reference bond list of", len(z["r"]), "particles")
 
 
+BATCH_REPLICAS, BATCH_RUNS, BATCH_SWEEPS = 8, 8, 40000
+
+
+def batch_points():
+    """8 uneven replicas in the manner of the batched workload, restricted to
+    state points that equilibrate well within the runs: N log-uniform in
+    [500, 1500], T uniform in [0.19, 0.23], rho uniform in [0.2, 0.5]."""
+    rng = np.random.default_rng(20261203)
+    n = np.exp(rng.uniform(np.log(500), np.log(1500), BATCH_REPLICAS)).astype(int)
+    T = rng.uniform(0.19, 0.23, BATCH_REPLICAS)
+    rho = rng.uniform(0.2, 0.5, BATCH_REPLICAS)
+    return [dict(n=int(n[k]), T=round(float(T[k]), 4), rho=round(float(rho[k]), 4)) for k in range(BATCH_REPLICAS)]
+
+
+def batch_ref():
+    """Per-replica <E/N> of the reference program for batch_points(): 8
+    independent runs each from the reference generator's start, mean over
+    the second half of each run, then mean and standard error over runs."""
+    build_ref()
+    exe = os.path.join(ROOT, "oracle", "_ref", "PatchyParticles_fast")
+    pts = batch_points()
+    jobs = []
+    for k, sp in enumerate(pts):
+        d0 = tempfile.mkdtemp(prefix=f"batch{k}_")
+        subprocess.run([os.path.join(ROOT, "oracle", "_ref", "generator"), str(sp["n"]), str(sp["rho"])], cwd=d0,
+                       check=True, capture_output=True)
+        for j in range(BATCH_RUNS):
+            d = os.path.join(d0, f"run{j}")
+            os.makedirs(d)
+            keys = {"Dynamics": 0, "Ensemble": 0, "Seed": 2000 + j, "Temperature": sp["T"], "Disp_max": 0.05,
+                    "Theta_max": 0.1, "KF_delta": 0.12, "KF_cosmax": 0.92,
+                    "Initial_conditions_file": os.path.join(d0, "generated.rrr"), "Steps": BATCH_SWEEPS,
+                    "Print_every": STAT_PRINT, "Save_every": BATCH_SWEEPS, "Configuration_folder": "."}
+            with open(os.path.join(d, "input"), "w") as f:
+                f.write("".join(f"{a} = {b}\n" for a, b in keys.items()))
+            jobs.append((k, d))
+    running = []
+    for k, d in jobs:  # at most os.cpu_count() reference runs at once
+        while len(running) >= (os.cpu_count() or 4):
+            running.pop(0).wait()
+        running.append(subprocess.Popen([exe, "input"], cwd=d, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL))
+    for p in running:
+        assert p.wait() == 0
+    out = []
+    for k, sp in enumerate(pts):
+        e_runs = []
+        for kk, d in jobs:
+            if kk != k:
+                continue
+            e = np.loadtxt(os.path.join(d, "energy.dat"))[:, 1]
+            e_runs.append(e[int(STAT_DISCARD * len(e)):].mean())
+        e_runs = np.array(e_runs)
+        out.append(dict(sp, E_per_N=float(e_runs.mean()), E_per_N_se=float(e_runs.std(ddof=1) / np.sqrt(len(e_runs))),
+                        E_runs=e_runs.tolist()))
+        print("batch", k, out[-1]["n"], out[-1]["T"], out[-1]["rho"], out[-1]["E_per_N"], out[-1]["E_per_N_se"])
+    with open(os.path.join(HERE, "batch_ref.json"), "w") as f:
+        json.dump(dict(sweeps=BATCH_SWEEPS, print_every=STAT_PRINT, discard=STAT_DISCARD, runs=BATCH_RUNS,
+                       replicas=out), f, indent=1)
+
+
 # state points of the statistical comparison: the reference program's own
 # output series (energy.dat, density.dat) from its generator's start
 STAT_POINTS = {
@@ -394,8 +457,11 @@ if __name__ == "__main__":
         stats_ref()
     elif sys.argv[1:] == ["bonds"]:
         bonds_ref()
+    elif sys.argv[1:] == ["batch"]:
+        batch_ref()
     else:
         main()
         host_runs()
         bonds_ref()
         stats_ref()
+        batch_ref()

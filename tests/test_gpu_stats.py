@@ -109,3 +109,92 @@ def test_npt_hard_spheres_follow_carnahan_starling(pp, tmp_path):
     _, rho_rep = _replica_means(pp, r, O, box, prm, 32, 12000, 50, 0.4)
     m, se = _mean_se(rho_rep)
     assert abs(m - rho_cs) < 0.015 * rho_cs + 3 * se, (m, se, rho_cs)
+
+
+def test_uneven_batch_matches_reference_runs(pp, tmp_path):
+    """Uneven replicas in one batched context, in the manner of the batched
+    workload (N, T and rho differ per replica): each replica's <E/N>, from 16
+    independent copies of the reference generator's start, against 8 runs of
+    the reference program at that state point (tests/golden/batch_ref.json),
+    within 3 combined standard errors."""
+    ref = golden("batch_ref.json")
+    copies = 16
+    rs, Os, bs, ps, ns = [], [], [], [], []
+    for k, sp in enumerate(ref["replicas"]):
+        d = tmp_path / f"start{k}"
+        d.mkdir()
+        r, O, box = _reference_start(pp, d, sp["n"], sp["rho"])
+        for _ in range(copies):
+            rs.append(r)
+            Os.append(O)
+            bs.append(box)
+            ps.append(dict(T=sp["T"]))
+            ns.append(sp["n"])
+    ns = np.array(ns)
+    with pp.System.batch(rs, Os, bs, ps, seed=31) as s:
+        e = []
+        for _ in range(ref["sweeps"] // ref["print_every"]):
+            s.do_ensemble(ref["print_every"])
+            e.append(-s.counters_all()["bonds"] / ns)
+        for k in range(s.R):
+            assert not s.energy(k)[3]
+    e = np.array(e)
+    means = e[int(ref["discard"] * len(e)):].mean(axis=0)
+    for k, sp in enumerate(ref["replicas"]):
+        m, se = _mean_se(means[k * copies:(k + 1) * copies])
+        tol = 3 * math.hypot(se, sp["E_per_N_se"])
+        assert abs(m - sp["E_per_N"]) < tol, (k, sp, m, se)
+
+
+def test_box_shape_moves_sample_a_uniform_lx(pp, oracle):
+    """Box-shape moves (Lx_move) at constant volume on a dilute hard-sphere
+    gas (patches that never bond): the rescaling has unit Jacobian and no
+    energy, so Lx is uniform on [V / Lyz_max^2, V / Lyz_min^2].  Mean and
+    quartile occupancies over 64 replicas."""
+    r, O, box = oracle.generate(256, 0.05, seed=4)
+    V = float(np.prod(box))
+    lyz_min, lyz_max = 15.5, 19.0
+    lo, hi = V / lyz_max ** 2, V / lyz_min ** 2
+    R = 64
+    prm = dict(T=1.0, kf_cosmax=1.0, disp_max=0.2, theta_max=0.5)
+    with pp.System.batch([r] * R, [O] * R, [box] * R, [prm] * R, seed=9) as s:
+        s.set_lx_move(1.0, lyz_min, lyz_max)
+        s.do_ensemble(2000)
+        lx = []
+        for _ in range(400):
+            s.do_ensemble(25)
+            c = s.counters_all()
+            lx.append(c["box"][:, 0])
+            assert np.allclose(c["box"][:, 1], c["box"][:, 2])
+            assert np.allclose(c["box"].prod(axis=1), V, rtol=1e-12)
+        acc = c["accepted"][:, 1].sum() / c["tries"][:, 1].sum()
+    lx = np.array(lx)
+    assert lx.min() >= lo - 1e-9 and lx.max() <= hi + 1e-9
+    assert 0.2 < acc < 1.0
+    per_rep = lx.mean(axis=0)
+    m, se = _mean_se(per_rep)
+    assert abs(m - 0.5 * (lo + hi)) < 4 * se + 0.01 * (hi - lo), (m, se, lo, hi)
+    q = np.histogram(lx.ravel(), bins=4, range=(lo, hi))[0] / lx.size
+    assert np.all(np.abs(q - 0.25) < 0.03), q
+
+
+def test_box_shape_moves_keep_the_reference_energy(pp, tmp_path):
+    """nvt rho = 0.5, T = 0.2 with box-shape moves (Ly = Lz within +-10 %):
+    <E/N> still matches the reference program's fixed-box runs within 3
+    combined standard errors."""
+    ref = golden("stats_ref.json")["nvt_rho05_T020"]
+    r, O, box = _reference_start(pp, tmp_path, ref["n"], ref["rho0"])
+    R = 32
+    with pp.System.batch([r] * R, [O] * R, [box] * R, [dict(T=0.2)] * R, seed=2025) as s:
+        s.set_lx_move(0.3, 0.9 * box[1], 1.1 * box[1])
+        e = []
+        for _ in range(ref["sweeps"] // ref["print_every"]):
+            s.do_ensemble(ref["print_every"])
+            e.append(-s.counters_all()["bonds"] / ref["n"])
+        c = s.counters_all()
+        assert c["tries"][:, 1].sum() > 0 and c["accepted"][:, 1].sum() > 0
+        assert np.any(np.abs(c["box"][:, 0] - box[0]) > 1e-6)
+    e = np.array(e)
+    m, se = _mean_se(e[int(ref["discard"] * len(e)):].mean(axis=0))
+    tol = 3 * math.hypot(se, ref["E_per_N_se"])
+    assert abs(m - ref["E_per_N"]) < tol, (m, se, ref["E_per_N"], ref["E_per_N_se"])

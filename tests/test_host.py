@@ -273,3 +273,44 @@ def test_driver_refuses_bonds_in_npt(host_built, tmp_path):
     _write_input(tmp_path, keys)
     res = subprocess.run([DRIVER, "input"], cwd=tmp_path, capture_output=True, text=True)
     assert res.returncode == 1 and "The printing of bonds is unavailable in non-canonical ensembles" in res.stderr
+
+
+def test_driver_lx_move_key_errors(host_built, tmp_path):
+    keys = dict(golden("host_runs.json")["nvt"]["keys"], Lx_move=1, Lx_change_max=0.5, Lyz_min=20.0, Lyz_max=23.0)
+    subprocess.run([GENERATOR, "1000", "0.1"], cwd=tmp_path, check=True, capture_output=True)
+    for bad, msg in ((dict(Lyz_min=22.0), "Lyz_min should be smaller than box size"),
+                     (dict(Lyz_max=21.0), "Lyz_max should be larger than box size")):
+        _write_input(tmp_path, dict(keys, **bad))
+        res = subprocess.run([DRIVER, "input"], cwd=tmp_path, capture_output=True, text=True)
+        assert res.returncode == 1 and msg in res.stderr, res.stderr
+    _write_input(tmp_path, {k: v for k, v in keys.items() if k != "Lyz_max"})
+    res = subprocess.run([DRIVER, "input"], cwd=tmp_path, capture_output=True, text=True)
+    assert res.returncode == 255 and "Mandatory key 'Lyz_max' not found" in res.stderr
+    # a box with Ly != Lz
+    lines = (tmp_path / "generated.rrr").read_text().splitlines()
+    head = lines[0].split()
+    head[3] = str(float(head[3]) * 1.01)
+    (tmp_path / "flat.rrr").write_text("\n".join([" ".join(head)] + lines[1:]) + "\n")
+    _write_input(tmp_path, dict(keys, Initial_conditions_file="flat.rrr"))
+    res = subprocess.run([DRIVER, "input"], cwd=tmp_path, capture_output=True, text=True)
+    assert res.returncode == 1 and "Lx_move = 1 requires Ly = Lz" in res.stderr, res.stderr
+
+
+@pytest.mark.gpu
+def test_driver_lx_move_outputs(host_built, tmp_path):
+    """Lx_move = 1: boxshape.dat "step Lx Ly Lz" at every print step with
+    Ly = Lz and the volume kept, and a third acceptance column."""
+    keys = dict(golden("host_runs.json")["nvt"]["keys"], Lx_move=1, Lx_change_max=0.5, Lyz_min=20.0, Lyz_max=23.0,
+                Steps=300, Print_every=100, Temperature=0.2)
+    subprocess.run([GENERATOR, "1000", "0.1"], cwd=tmp_path, check=True, capture_output=True)
+    _write_input(tmp_path, keys)
+    res = subprocess.run([DRIVER, "input"], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert res.returncode == 0, res.stderr
+    shape = [list(map(float, ln.split())) for ln in (tmp_path / "boxshape.dat").read_text().splitlines()]
+    assert [int(x[0]) for x in shape] == [0, 100, 200]
+    V = shape[0][1] * shape[0][2] * shape[0][3]
+    for x in shape:
+        assert x[2] == x[3] and abs(x[1] * x[2] * x[3] - V) < 1e-3 * V
+    assert shape[-1][1] != shape[0][1]
+    acc = (tmp_path / "acceptance.dat").read_text().splitlines()
+    assert all(len(ln.split()) == 3 for ln in acc[1:])
