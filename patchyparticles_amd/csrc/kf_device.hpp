@@ -209,6 +209,29 @@ __device__ __forceinline__ void kf_fast_q2(double ox, double oy, double oz, doub
 	}
 }
 
+// q side on one separation: kf_fast_q2's classification of d alone
+__device__ __forceinline__ unsigned kf_fast_q1(double dx, double dy, double dz, double c2r2, double er2,
+                                               const double Oq[9]) {
+	double m[9];
+#pragma unroll
+	for(int e = 0; e < 9; e++) m[e] = Oq[e] * KF_C;
+	unsigned f = 0;
+#pragma unroll
+	for(int b = 0; b < 4; b++) {
+		const bool n0 = (KF_NEG >> (3 * b)) & 1u, n1 = (KF_NEG >> (3 * b + 1)) & 1u, n2 = (KF_NEG >> (3 * b + 2)) & 1u;
+		double q[3];
+#pragma unroll
+		for(int k = 0; k < 3; k++) {
+			double a0 = n0 ? -m[3 * k] : m[3 * k];
+			double a1 = n1 ? -m[3 * k + 1] : m[3 * k + 1];
+			double a2 = n2 ? -m[3 * k + 2] : m[3 * k + 2];
+			q[k] = a0 + a1 + a2;
+		}
+		f |= kf_fast_cos(-kf_dot(dx, dy, dz, q[0], q[1], q[2]), c2r2, er2);
+	}
+	return f;
+}
+
 // Cell coordinate of x on a grid of n cells over [s, s+L) (periodic).
 // Deterministic: the rebuild and the in-cell test of a trial move call the
 // same function on the same doubles.

@@ -179,4 +179,42 @@ __device__ __forceinline__ void bonds_old_new_exact(const Trial &t, const double
 	}
 }
 
+// Bonds of neighbour q with the new position only: the filtered decision
+// (a pair inside the filter band sets `band`) and the reference's arithmetic
+// (MC.c:242-262).  For neighbours whose bonds with the old position are known.
+template <class LoadO>
+__device__ __forceinline__ void bonds_new(const Trial &t, const double *Pn, double qx, double qy, double qz,
+                                          LoadO load_o, int &bn, bool &band) {
+	const double wx = kf_min_image(qx - t.nx, t.Lx, t.iLx);
+	const double wy = kf_min_image(qy - t.ny, t.Ly, t.iLy);
+	const double wz = kf_min_image(qz - t.nz, t.Lz, t.iLz);
+	const double w2 = wx * wx + wy * wy + wz * wz;
+	const double nc = t.c2 * w2, ne = KF_EPS * w2;
+	const unsigned fn = w2 < t.rc2 ? kf_fast_p(wx, wy, wz, Pn, nc, ne) : 0u;
+	unsigned unsure = (fn >> 1) & 1u;
+	if(fn & 1u) {
+		double Oq[9];
+		load_o(Oq);
+		const unsigned qn = kf_fast_q1(wx, wy, wz, nc, ne, Oq);
+		unsure |= (qn >> 1) & 1u;
+		bn += (int) (qn & 1u);
+	}
+	band |= unsure != 0u;
+}
+
+template <class LoadO>
+__device__ __forceinline__ void bonds_new_exact(const Trial &t, const double *Pn, double qx, double qy, double qz,
+                                                LoadO load_o, int &bn) {
+	double wx = kf_min_image(qx - t.nx, t.Lx, t.iLx);
+	double wy = kf_min_image(qy - t.ny, t.Ly, t.iLy);
+	double wz = kf_min_image(qz - t.nz, t.Lz, t.iLz);
+	double w2 = wx * wx + wy * wy + wz * wz;
+	if(!(w2 < t.rc2)) return;
+	kf_unit(wx, wy, wz, w2);
+	if(!kf_any_p(wx, wy, wz, Pn, t.cosmax)) return;
+	double Oq[9];
+	load_o(Oq);
+	bn += kf_any_q(wx, wy, wz, Oq, t.cosmax);
+}
+
 }  // namespace ppd

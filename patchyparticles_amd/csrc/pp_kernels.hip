@@ -443,16 +443,18 @@ struct PassArgs {
 constexpr int OWN_W = NCOMP + 12;   // x, y, z, O[9], P[12]
 constexpr int LDS_OWN = CELL_CAP * OWN_W;
 constexpr int LDS_PID = (CELL_CAP + 1) / 2;  // CELL_CAP ints
+constexpr int LDS_SB = (CELL_CAP + 1) / 2;   // CELL_CAP ints: each member's bonds with the static neighbours
 constexpr int LDS_TRIAL = CELL_CAP * TRIAL_W;
 constexpr int LDS_NB = REC_W / 2;      // the cell record, REC_W ints
 constexpr int LDS_ACC = ACC_TAB;       // exp(-k/T), k < ACC_TAB
-constexpr int LDS_FIXED = LDS_OWN + LDS_PID + LDS_TRIAL + LDS_NB + LDS_ACC;
+constexpr int LDS_FIXED = LDS_OWN + LDS_PID + LDS_SB + LDS_TRIAL + LDS_NB + LDS_ACC;
 // + the orientations of the T register-resident neighbours, component-major
 template <int T>
 constexpr int team_lds() {
 	return LDS_FIXED + 9 * T;
 }
-static_assert((LDS_OWN + LDS_PID + LDS_TRIAL) % 2 == 0 && LDS_FIXED % 2 == 0, "16-byte aligned record and team blocks");
+static_assert((LDS_OWN + LDS_PID + LDS_SB + LDS_TRIAL) % 2 == 0 && LDS_FIXED % 2 == 0,
+              "16-byte aligned record and team blocks");
 
 // The same decisions through the filtered tests (no square root, no
 // divisions, no branch per patch).  A pair inside a filter band sets `band`;
@@ -492,11 +494,16 @@ __device__ __forceinline__ void bonds_old_new(const Trial &t, const double *Po, 
 // make sequential trials.  Each lane keeps up to CPL of the cell's static
 // neighbours (the 26 surrounding cells, fixed during the pass) in registers;
 // a denser neighbourhood is handled T at a time from HBM.
-template <int T, int CPL>
+// KEEP (fixed trials per cell, many per member): every member's bonds with
+// the static neighbours are counted up front and kept as it moves, so a trial
+// tests only the new position against them.  Without KEEP (about one trial
+// per member) every trial tests both positions.
+template <int T, int CPL, bool KEEP>
 __device__ __forceinline__ void pass_cell(const DevState &d, uint64_t sweep, int pass, int gi, int lane,
                                           double *own) {
 	int *own_pid = (int *) (own + LDS_OWN);
-	double *trials = own + LDS_OWN + LDS_PID;
+	int *sbond = (int *) (own + LDS_OWN + LDS_PID);  // member m's bonds with the static neighbours
+	double *trials = own + LDS_OWN + LDS_PID + LDS_SB;
 	int *nb_tab = (int *) (trials + LDS_TRIAL);  // the cell's record (build_record)
 	double *acc_tab = trials + LDS_TRIAL + LDS_NB;
 	double *ori = own + LDS_FIXED;  // orientation component c of this lane's first neighbour at c * T + lane
@@ -607,6 +614,56 @@ __device__ __forceinline__ void pass_cell(const DevState &d, uint64_t sweep, int
 				for(int c = 0; c < 3; c++) w[NCOMP + 3 * i + c] = P[i][c];
 		}
 		team_sync();
+		// this lane's static neighbours (the surrounding cells, fixed during
+		// the pass): the one in registers and, in dense neighbourhoods, the
+		// rest from HBM
+		auto each_static = [&](auto &&f) {
+#pragma unroll
+			for(int u = 0; u < CPL; u++)
+				if(cqr[u] >= 0)
+					f(cxr[u], cyr[u], czr[u], [&](double *Oq) {
+						const int q = cqr[u];
+#pragma unroll
+						for(int c = 0; c < 9; c++) Oq[c] = u == 0 ? ori[c * T + lane] : F[(3 + c) * S + q];
+					});
+			for(int b = T * CPL; b < M; b += T)
+				if(b + lane < M) {
+					const int q = nb_slot(b + lane);
+					f(F[q], F[S + q], F[2 * S + q], [&](double *Oq) {
+#pragma unroll
+						for(int c = 0; c < 9; c++) Oq[c] = F[(3 + c) * S + q];
+					});
+				}
+		};
+		team_sync();
+		// 5. KEEP: every member's bonds with the static neighbours
+		if constexpr(KEEP) {
+			for(int m = 0; m < cnt; m++) {
+				const double *wm = own + m * OWN_W;
+				Trial tm = tr;
+				tm.nx = unid<T>(wm[0]);
+				tm.ny = unid<T>(wm[1]);
+				tm.nz = unid<T>(wm[2]);
+				double Pm[12];
+#pragma unroll
+				for(int c = 0; c < 12; c++) Pm[c] = wm[NCOMP + c];
+				int b = 0;
+				bool band = tr.c2 < 0.;
+				if(!band)
+					each_static([&](double qx, double qy, double qz, auto load_o) {
+						bonds_new(tm, Pm, qx, qy, qz, load_o, b, band);
+					});
+				if(team_any<T>(band) && band) {
+					b = 0;
+					each_static([&](double qx, double qy, double qz, auto load_o) {
+						bonds_new_exact(tm, Pm, qx, qy, qz, load_o, b);
+					});
+				}
+				b = team_sum<T>(b);
+				if(lane == 0) sbond[m] = b;
+			}
+			team_sync();
+		}
 		int c_acc = 0, d_bonds = 0;
 
 		for(int t = 0; t < ntrial; t++) {
@@ -678,46 +735,59 @@ __device__ __forceinline__ void pass_cell(const DevState &d, uint64_t sweep, int
 				}
 
 			// phase 2: bonds at the old and the new position, over this lane's
-			// neighbours: the static ones in registers, one member of the cell,
-			// and (dense neighbourhoods) the rest from HBM
-			auto each_neighbour = [&](auto &&f) {
+			// neighbours: the static ones (KEEP: new position only, the old
+			// count is the member's kept one) and one member of the cell
+			auto member_o = [&](double *Oq) {
 #pragma unroll
-				for(int u = 0; u < CPL; u++)
-					if(cqr[u] >= 0)
-						f(cxr[u], cyr[u], czr[u], [&](double *Oq) {
-							const int q = cqr[u];
-#pragma unroll
-							for(int c = 0; c < 9; c++) Oq[c] = u == 0 ? ori[c * T + lane] : F[(3 + c) * S + q];
-						});
-				if(has_m)
-					f(mx, my, mz, [&](double *Oq) {
-#pragma unroll
-						for(int c = 0; c < 9; c++) Oq[c] = wl[3 + c];
-					});
-				for(int b = T * CPL; b < M; b += T)
-					if(b + lane < M) {
-						const int q = nb_slot(b + lane);
-						f(F[q], F[S + q], F[2 * S + q], [&](double *Oq) {
-#pragma unroll
-							for(int c = 0; c < 9; c++) Oq[c] = F[(3 + c) * S + q];
-						});
-					}
+				for(int c = 0; c < 9; c++) Oq[c] = wl[3 + c];
 			};
-			int bo = 0, bn = 0;
-			bool band = tr.c2 < 0.;  // no filter: every lane takes the exact form
-			if(!band)
-				each_neighbour([&](double qx, double qy, double qz, auto load_o) {
-					bonds_old_new(tr, Po, Pn, qx, qy, qz, load_o, bo, bn, band);
-				});
-			if(team_any<T>(band) && band) {
+			int bo, bn, s_new = 0;
+			if constexpr(KEEP) {
+				int bns = 0, bom = 0, bnm = 0;
+				bool band = tr.c2 < 0.;  // no filter: every lane takes the exact form
+				if(!band) {
+					each_static([&](double qx, double qy, double qz, auto load_o) {
+						bonds_new(tr, Pn, qx, qy, qz, load_o, bns, band);
+					});
+					if(has_m) bonds_old_new(tr, Po, Pn, mx, my, mz, member_o, bom, bnm, band);
+				}
+				if(team_any<T>(band) && band) {
+					bns = 0;
+					bom = 0;
+					bnm = 0;
+					each_static([&](double qx, double qy, double qz, auto load_o) {
+						bonds_new_exact(tr, Pn, qx, qy, qz, load_o, bns);
+					});
+					if(has_m) bonds_old_new_exact(tr, Po, Pn, mx, my, mz, member_o, bom, bnm);
+				}
+				// three counts below 1024 in one reduction
+				const int packed = team_sum<T>(bns | (bnm << 10) | (bom << 20));
+				s_new = packed & 1023;
+				bo = sbond[k] + (packed >> 20);
+				bn = s_new + ((packed >> 10) & 1023);
+			}
+			else {
 				bo = 0;
 				bn = 0;
-				each_neighbour([&](double qx, double qy, double qz, auto load_o) {
-					bonds_old_new_exact(tr, Po, Pn, qx, qy, qz, load_o, bo, bn);
-				});
+				bool band = tr.c2 < 0.;
+				auto each_neighbour = [&](auto &&f) {
+					each_static(f);
+					if(has_m) f(mx, my, mz, member_o);
+				};
+				if(!band)
+					each_neighbour([&](double qx, double qy, double qz, auto load_o) {
+						bonds_old_new(tr, Po, Pn, qx, qy, qz, load_o, bo, bn, band);
+					});
+				if(team_any<T>(band) && band) {
+					bo = 0;
+					bn = 0;
+					each_neighbour([&](double qx, double qy, double qz, auto load_o) {
+						bonds_old_new_exact(tr, Po, Pn, qx, qy, qz, load_o, bo, bn);
+					});
+				}
+				bo = team_sum_small<T>(bo);
+				bn = team_sum_small<T>(bn);
 			}
-			bo = team_sum_small<T>(bo);
-			bn = team_sum_small<T>(bn);
 			const int db = bn - bo;  // E = -bonds, dE = -db
 			const double dE = (double) (-db);
 			// MC.c:332: accept if dE < 0 or a uniform draw < exp(-dE/T); dE is
@@ -730,6 +800,7 @@ __device__ __forceinline__ void pass_cell(const DevState &d, uint64_t sweep, int
 				d_bonds += db;
 				// member k takes the new position, orientation and patches
 				if(lane == 0) {
+					if constexpr(KEEP) sbond[k] = s_new;
 					double *wk = own + k * OWN_W;
 					wk[0] = tr.nx;
 					wk[1] = tr.ny;
@@ -742,7 +813,7 @@ __device__ __forceinline__ void pass_cell(const DevState &d, uint64_t sweep, int
 			}
 			team_sync();
 		}
-		// 5. write the members back, ranked (the ranking permutes the cell's slots)
+		// 6. write the members back, ranked (the ranking permutes the cell's slots)
 		if(c_acc > 0 || cnt > 1) {
 			for(int e = lane; e < cnt * NCOMP; e += T) {
 				int m = e / NCOMP, c = e - m * NCOMP;
@@ -770,7 +841,7 @@ __device__ __forceinline__ void pass_cell(const DevState &d, uint64_t sweep, int
 // list is in cell order, so XCD x takes the x-th eighth of the entries
 // actually listed (a slab of the box, whose shared neighbour reads its L2
 // then serves); the surplus workgroups of each XCD leave at once.
-template <int T, int CPL>
+template <int T, int CPL, bool KEEP>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_pass(PassArgs a) {
 	extern __shared__ double lds[];
 	const int lane = threadIdx.x & (T - 1);
@@ -782,7 +853,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
 	if(i >= per_xcd) return;
 	const int gi = uni<T>(((blockIdx.x & 7) * per_xcd + i) * tpb + team);
 	if(gi >= total) return;
-	pass_cell<T, CPL>(a.d, a.sweep, a.pass, gi, lane, lds + team * team_lds<T>());
+	pass_cell<T, CPL, KEEP>(a.d, a.sweep, a.pass, gi, lane, lds + team * team_lds<T>());
 }
 
 // Grid-wide barrier of a cooperative launch (every workgroup resident).
@@ -818,7 +889,7 @@ __device__ __forceinline__ void grid_barrier(unsigned *bar, int *err) {
 
 // The 8 colour passes of a sweep in one cooperative launch, separated by
 // grid barriers.  Team g takes list entries g, g + teams, ... of each pass.
-template <int T, int CPL>
+template <int T, int CPL, bool KEEP>
 __global__ __launch_bounds__(256) void k_sweep_passes(PassArgs a) {
 	extern __shared__ double lds[];
 	const DevState &d = a.d;
@@ -830,19 +901,19 @@ __global__ __launch_bounds__(256) void k_sweep_passes(PassArgs a) {
 	double *own = lds + team * team_lds<T>();
 	for(int k = 0; k < 8; k++) {
 		const int total = uni<T>(d.pfx[k * (d.R + 1) + d.R]);
-		for(int gi = g0; gi < total; gi += nteams) pass_cell<T, CPL>(d, a.sweep, k, gi, lane, own);
+		for(int gi = g0; gi < total; gi += nteams) pass_cell<T, CPL, KEEP>(d, a.sweep, k, gi, lane, own);
 		if(k < 7) grid_barrier(d.bar, d.err);
 	}
 }
 
-template <int T, int CPL>
+template <int T, int CPL, bool KEEP>
 static void pass_T(const DevState &d, uint64_t sweep, int pass, int block, hipStream_t s) {
 	const int tpb = block / T;
 	int nb = (d.total_act + tpb - 1) / tpb;
 	nb = (nb + 7) & ~7;  // 8 XCDs x the most workgroups one XCD may need
 	size_t lds = (size_t) tpb * team_lds<T>() * sizeof(double);
 	PassArgs a{d, sweep, pass};
-	hipLaunchKernelGGL((k_pass<T, CPL>), dim3(nb), dim3(block), lds, s, a);
+	hipLaunchKernelGGL((k_pass<T, CPL, KEEP>), dim3(nb), dim3(block), lds, s, a);
 }
 
 size_t pass_lds_bytes(int T, int block) {
@@ -850,33 +921,45 @@ size_t pass_lds_bytes(int T, int block) {
 }
 
 void launch_pass(const DevState &d, uint64_t sweep, int pass, int T, int block, hipStream_t s) {
-	if(T == 32) pass_T<32, 1>(d, sweep, pass, block, s);
-	else pass_T<64, 1>(d, sweep, pass, block, s);
+	const bool keep = d.trials_per_cell > 0;
+	if(T == 32) {
+		if(keep) pass_T<32, 1, true>(d, sweep, pass, block, s);
+		else pass_T<32, 1, false>(d, sweep, pass, block, s);
+	}
+	else {
+		if(keep) pass_T<64, 1, true>(d, sweep, pass, block, s);
+		else pass_T<64, 1, false>(d, sweep, pass, block, s);
+	}
 }
 
-template <int T, int CPL>
+template <int T, int CPL, bool KEEP>
 static int sweep_blocks_T(int block, int cus) {
 	int per_cu = 0;
-	if(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *) k_sweep_passes<T, CPL>, block,
+	if(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *) k_sweep_passes<T, CPL, KEEP>, block,
 	                                                pass_lds_bytes(T, block)) != hipSuccess)
 		return 0;
 	return per_cu * cus;
 }
 
+// co-resident workgroups of either form (both must fit: the form is chosen
+// per context at launch)
 int sweep_blocks_max(int T, int block, int cus) {
-	return T == 32 ? sweep_blocks_T<32, 1>(block, cus) : sweep_blocks_T<64, 1>(block, cus);
+	if(T == 32) return std::min(sweep_blocks_T<32, 1, false>(block, cus), sweep_blocks_T<32, 1, true>(block, cus));
+	return std::min(sweep_blocks_T<64, 1, false>(block, cus), sweep_blocks_T<64, 1, true>(block, cus));
 }
 
-template <int T, int CPL>
+template <int T, int CPL, bool KEEP>
 static hipError_t sweep_T(const DevState &d, uint64_t sweep, int block, int blocks, hipStream_t s) {
 	PassArgs a{d, sweep, 0};
 	void *args[] = {&a};
-	return hipLaunchCooperativeKernel((const void *) k_sweep_passes<T, CPL>, dim3(blocks), dim3(block), args,
+	return hipLaunchCooperativeKernel((const void *) k_sweep_passes<T, CPL, KEEP>, dim3(blocks), dim3(block), args,
 	                                  (unsigned) pass_lds_bytes(T, block), s);
 }
 
 hipError_t launch_sweep_passes(const DevState &d, uint64_t sweep, int T, int block, int blocks, hipStream_t s) {
-	return T == 32 ? sweep_T<32, 1>(d, sweep, block, blocks, s) : sweep_T<64, 1>(d, sweep, block, blocks, s);
+	const bool keep = d.trials_per_cell > 0;
+	if(T == 32) return keep ? sweep_T<32, 1, true>(d, sweep, block, blocks, s) : sweep_T<32, 1, false>(d, sweep, block, blocks, s);
+	return keep ? sweep_T<64, 1, true>(d, sweep, block, blocks, s) : sweep_T<64, 1, false>(d, sweep, block, blocks, s);
 }
 
 // ------------------------------------------------------------------------
