@@ -8,7 +8,7 @@ import ctypes as C
 
 import numpy as np
 
-from ._lib import NET_BIASES, NET_INPUTS, NET_OUTPUTS, NET_WEIGHTS, Counters, Options, Params, _arr, _ptr, check, lib
+from ._lib import NET_BIASES, NET_INPUTS, NET_OUTPUTS, NET_WEIGHTS, Counters, Options, Params, PPError, _arr, _ptr, check, lib
 
 NVT, NPT = 0, 5  # defs.h:21-25
 
@@ -67,6 +67,13 @@ class System:
         self._theta_max0 = float(P[0].theta_max) if R else 0.1
         self._net = None
 
+    def _n(self, rep):
+        """Particles of replica rep; an index out of range is refused as the
+        C ABI refuses it (PP_ERR_ARG), not wrapped around."""
+        if not 0 <= int(rep) < self.R:
+            raise PPError(1, f"replica {rep} out of range")
+        return self.n[rep]
+
     def close(self):
         if self._h is not None:
             lib().pp_destroy(self._h)
@@ -92,7 +99,7 @@ class System:
     def energy(self, rep=0):
         """(bonds[N], overlap[N], bonded pairs, any overlap) recomputed from
         scratch, as MC_energy over every particle (MC.c:272-309)."""
-        n = self.n[rep]
+        n = self._n(rep)
         bonds = np.zeros(max(n, 1), np.int32)
         ov = np.zeros(max(n, 1), np.uint8)
         tot = C.c_longlong(0)
@@ -107,7 +114,7 @@ class System:
         return e
 
     def state(self, rep=0):
-        n = self.n[rep]
+        n = self._n(rep)
         r = np.zeros(3 * max(n, 1))
         O = np.zeros(9 * max(n, 1))
         box = np.zeros(3)
@@ -182,7 +189,7 @@ class System:
 
     def bond_list(self, rep=0, max_per=16):
         """Bonded partners of every particle: a list of sorted index arrays."""
-        n = self.n[rep]
+        n = self._n(rep)
         cnt = np.zeros(max(n, 1), np.int32)
         part = np.zeros(max(n, 1) * max_per, np.int32)
         check(lib().pp_bond_list(self._h, rep, max_per, _ptr(cnt), _ptr(part)))
