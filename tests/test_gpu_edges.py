@@ -86,3 +86,28 @@ def test_cell_over_capacity_is_reported(pp):
         with pp.System(g, O, box, T=0.1, cell_occupancy=40.0) as s:
             s.do_ensemble(1)
     assert e.value.status == 5  # PP_ERR_CELL_CAP
+
+
+def test_sweep_grids_keep_same_colour_cells_apart(pp, oracle):
+    """The checkerboard's invariant, on the grids the library picks for many
+    boxes (and after NPT volume moves): an even number of cells per axis, at
+    least 4, each of side >= 1 + delta, so two cells of one colour are at
+    least one cell side apart across the periodic boundary too, and no two
+    particles moved in one pass can interact."""
+    rng = np.random.default_rng(12)
+    for _ in range(20):
+        n = int(rng.integers(50, 400))
+        rho = float(rng.uniform(0.05, 0.6))
+        r, O, box = oracle.generate(n, rho, seed=int(rng.integers(1 << 30)))
+        delta = float(rng.uniform(0.05, 0.2))
+        with pp.System(r, O, box, T=0.2, kf_delta=delta) as s:
+            nside, side = s.grid()
+            assert all(k % 2 == 0 and k >= 4 for k in nside), (nside, box)
+            assert min(side) >= 1 + delta, (side, delta)
+    r, O, box = oracle.generate(300, 0.3, seed=3)
+    with pp.System(r, O, box, T=0.2, ensemble=pp.NPT, P=2.0, rescale_factor_max=0.1, seed=2) as s:
+        for _ in range(20):
+            s.do_ensemble(50)
+            nside, side = s.grid()
+            assert all(k % 2 == 0 and k >= 4 for k in nside) and min(side) >= 1.12
+        assert s.counters()["volume"] < np.prod(box)  # the box did shrink

@@ -314,3 +314,42 @@ def test_driver_lx_move_outputs(host_built, tmp_path):
     assert shape[-1][1] != shape[0][1]
     acc = (tmp_path / "acceptance.dat").read_text().splitlines()
     assert all(len(ln.split()) == 3 for ln in acc[1:])
+
+
+@pytest.mark.gpu
+def test_driver_stops_cleanly_on_a_signal(host_built, tmp_path):
+    """SIGTERM (main.c:16-24): the run ends after the current block of
+    sweeps, with last.rrr written at the step it reached."""
+    import signal
+    import time
+    keys = dict(golden("host_runs.json")["nvt"]["keys"], Steps=10 ** 9, Print_every=1000, Save_every=10 ** 8)
+    subprocess.run([GENERATOR, "1000", "0.1"], cwd=tmp_path, check=True, capture_output=True)
+    _write_input(tmp_path, keys)
+    p = subprocess.Popen([DRIVER, "input"], cwd=tmp_path, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True)
+    try:
+        t0 = time.time()
+        while not (tmp_path / "energy.dat").exists() or len((tmp_path / "energy.dat").read_text().splitlines()) < 3:
+            assert p.poll() is None and time.time() - t0 < 60
+            time.sleep(0.2)
+        p.send_signal(signal.SIGTERM)
+        out, err = p.communicate(timeout=60)
+    finally:
+        if p.poll() is None:  # the child this test started, by its handle
+            p.kill()
+            p.wait()
+    assert p.returncode == 0, err
+    step = int(open(tmp_path / "last.rrr").readline().split()[0])
+    assert step >= 2000 and int(out.splitlines()[-1].split()[0]) == step
+
+
+@pytest.mark.gpu
+def test_driver_refuses_an_overlapping_start(host_built, tmp_path):
+    lines = []
+    r = [[1.0, 1.0, 1.0], [1.5, 1.0, 1.0], [5.0, 5.0, 5.0]]
+    lines.append("0 3 10.000000000000 10.000000000000 10.000000000000")
+    for p in r:
+        lines += ["1.000000 0.000000 0.000000", "0.000000 1.000000 0.000000", " ".join(f"{x:.12f}" for x in p)]
+    (tmp_path / "ov.rrr").write_text("\n".join(lines) + "\n")
+    _write_input(tmp_path, dict(golden("host_runs.json")["nvt"]["keys"], Initial_conditions_file="ov.rrr"))
+    res = subprocess.run([DRIVER, "input"], cwd=tmp_path, capture_output=True, text=True, timeout=60)
+    assert res.returncode == 1 and "Initial configuration contains an overlap, aborting" in res.stderr
