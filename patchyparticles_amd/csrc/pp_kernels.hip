@@ -494,6 +494,10 @@ __device__ __forceinline__ void bonds_old_new(const Trial &t, const double *Po, 
 // make sequential trials.  Each lane keeps up to CPL of the cell's static
 // neighbours (the 26 surrounding cells, fixed during the pass) in registers;
 // a denser neighbourhood is handled T at a time from HBM.
+// Trials per cell from which the kept form pays: with fewer, the up-front
+// counts cost about what the trials save.
+constexpr int KEEP_MIN_TRIALS = 4;
+
 // KEEP (fixed trials per cell, many per member): every member's bonds with
 // the static neighbours are counted up front and kept as it moves, so a trial
 // tests only the new position against them.  Without KEEP (about one trial
@@ -921,7 +925,7 @@ size_t pass_lds_bytes(int T, int block) {
 }
 
 void launch_pass(const DevState &d, uint64_t sweep, int pass, int T, int block, hipStream_t s) {
-	const bool keep = d.trials_per_cell > 0;
+	const bool keep = d.trials_per_cell >= KEEP_MIN_TRIALS;
 	if(T == 32) {
 		if(keep) pass_T<32, 1, true>(d, sweep, pass, block, s);
 		else pass_T<32, 1, false>(d, sweep, pass, block, s);
@@ -957,7 +961,7 @@ static hipError_t sweep_T(const DevState &d, uint64_t sweep, int block, int bloc
 }
 
 hipError_t launch_sweep_passes(const DevState &d, uint64_t sweep, int T, int block, int blocks, hipStream_t s) {
-	const bool keep = d.trials_per_cell > 0;
+	const bool keep = d.trials_per_cell >= KEEP_MIN_TRIALS;
 	if(T == 32) return keep ? sweep_T<32, 1, true>(d, sweep, block, blocks, s) : sweep_T<32, 1, false>(d, sweep, block, blocks, s);
 	return keep ? sweep_T<64, 1, true>(d, sweep, block, blocks, s) : sweep_T<64, 1, false>(d, sweep, block, blocks, s);
 }
