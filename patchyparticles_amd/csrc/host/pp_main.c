@@ -21,7 +21,9 @@
  * Initial_conditions_file then only gives N, normally the last.rrr written
  * with the checkpoint).  Reference keys beyond the ones above: Print_bonds /
  * Bonds_folder (bond lists) and Lx_move / Lx_change_max / Lyz_min / Lyz_max
- * (box-shape moves, boxshape.dat).
+ * (box-shape moves, boxshape.dat).  The learned proposal: Learned_fraction,
+ * Learned_weights, Learned_train_steps, Learned_train_batch,
+ * Learned_pos_noise (see setup_learned).
  */
 #include <signal.h>
 #include <stdio.h>
@@ -48,6 +50,7 @@ static void on_signal(int sig) {
 typedef struct run {
 	ppo out;
 	pp_ctx *ctx;
+	pp_net *net; /* learned proposal (NULL: plain moves only) */
 	int n;
 	int npt;
 	double *r, *O;
@@ -102,6 +105,56 @@ static void check_energy(run *u) {
 }
 
 static long long next_multiple(long long x, long long m) { return (x / m + 1) * m; }
+
+/* The learned proposal (keys of this program only): Learned_fraction > 0
+ * turns it on for that share of the colour passes.  The network's weights
+ * come from Learned_weights if that file exists; otherwise a network is
+ * trained on a separate copy of the start (Learned_train_steps Adam steps of
+ * Learned_train_batch moves, one plain sweep between steps, displacement
+ * noise Learned_pos_noise), so the run's own trajectory is not advanced, and
+ * saved to Learned_weights when the key is given. */
+static void setup_learned(run *u, ppi_file *in, const pp_params *prm, const pp_options *opt, const pprrr *c) {
+	double frac = 0.;
+	ppi_double(in, "Learned_fraction", &frac, 0);
+	if(!(frac > 0.)) return;
+	int steps = 10000, batch = 4096;
+	double noise = 0.03;
+	char wfile[512] = "";
+	ppi_int(in, "Learned_train_steps", &steps, 0);
+	ppi_int(in, "Learned_train_batch", &batch, 0);
+	ppi_double(in, "Learned_pos_noise", &noise, 0);
+	ppi_string(in, "Learned_weights", wfile, sizeof wfile, 0);
+	check(u, pp_net_create(opt->device, opt->seed ^ 0x5EEDull, &u->net), "pp_net_create");
+	float *w = malloc(sizeof(float) * (PP_NET_WEIGHTS + PP_NET_BIASES));
+	if(!w) ppo_exit(&u->out, "out of memory\n");
+	FILE *f = wfile[0] ? fopen(wfile, "rb") : NULL;
+	if(f) {
+		const size_t got = fread(w, sizeof(float), PP_NET_WEIGHTS + PP_NET_BIASES, f);
+		fclose(f);
+		if(got != PP_NET_WEIGHTS + PP_NET_BIASES) ppo_exit(&u->out, "Learned_weights '%s' is not a network file\n", wfile);
+		check(u, pp_net_set(u->net, w, w + PP_NET_WEIGHTS), "pp_net_set");
+		ppo_log(&u->out, "Learned proposal: weights read from '%s'\n", wfile);
+	}
+	else {
+		pp_options o2 = *opt;
+		o2.seed = opt->seed + 1;
+		pp_ctx *tc = NULL;
+		check(u, pp_create(prm, &o2, c->n, c->r, c->O, c->box, &tc), "pp_create (training copy)");
+		double l0 = 0., l1 = 0.;
+		check(u, pp_net_train(tc, u->net, steps, batch, 1e-3, noise, prm->theta_max, 1, &l0, &l1), "pp_net_train");
+		pp_destroy(tc);
+		ppo_log(&u->out, "Learned proposal: %d training steps, loss %lf -> %lf\n", steps, l0, l1);
+		if(wfile[0]) {
+			check(u, pp_net_get(u->net, w, w + PP_NET_WEIGHTS), "pp_net_get");
+			f = fopen(wfile, "wb");
+			if(!f || fwrite(w, sizeof(float), PP_NET_WEIGHTS + PP_NET_BIASES, f) != PP_NET_WEIGHTS + PP_NET_BIASES)
+				ppo_exit(&u->out, "Learned_weights '%s' is not writable\n", wfile);
+			fclose(f);
+		}
+	}
+	free(w);
+	check(u, pp_net_attach(u->ctx, u->net, frac, prm->theta_max), "pp_net_attach");
+}
 
 int main(int argc, char *argv[]) {
 	if(argc == 1) {
@@ -180,6 +233,7 @@ int main(int argc, char *argv[]) {
 	if(st == PP_ERR_OVERLAP) ppo_exit(&u.out, "Initial configuration contains an overlap, aborting\n");
 	if(st != PP_OK) ppo_exit(&u.out, "pp_create failed: %s (%s)\n", pp_strerror(st), pp_last_error());
 	if(lx) check(&u, pp_set_lx_move(u.ctx, 0, lx_change_max, lyz_min, lyz_max), "pp_set_lx_move");
+	setup_learned(&u, in, &prm, &opt, &c);
 	pprrr_free(&c);
 	u.r = malloc(sizeof(double) * 3 * (size_t) (u.n > 0 ? u.n : 1));
 	u.O = malloc(sizeof(double) * 9 * (size_t) (u.n > 0 ? u.n : 1));
@@ -238,6 +292,7 @@ int main(int argc, char *argv[]) {
 	printf("\n");
 
 	pp_destroy(u.ctx);
+	if(u.net) pp_net_destroy(u.net);
 	free(u.r);
 	free(u.O);
 	free(u.bcount);

@@ -353,3 +353,29 @@ def test_driver_refuses_an_overlapping_start(host_built, tmp_path):
     _write_input(tmp_path, dict(golden("host_runs.json")["nvt"]["keys"], Initial_conditions_file="ov.rrr"))
     res = subprocess.run([DRIVER, "input"], cwd=tmp_path, capture_output=True, text=True, timeout=60)
     assert res.returncode == 1 and "Initial configuration contains an overlap, aborting" in res.stderr
+
+
+@pytest.mark.gpu
+def test_driver_learned_proposal_keys(host_built, oracle, tmp_path):
+    """Learned_fraction with Learned_weights: the first run trains on a
+    separate copy of the start and saves the weights, the second reads them;
+    both runs are then the same trajectory (training does not advance the
+    run), the energy checks pass, and the saved configurations agree with the
+    oracle's energy."""
+    keys = dict(golden("host_runs.json")["nvt"]["keys"], Steps=200, Print_every=100, Save_every=100,
+                Temperature=0.15, Learned_fraction=0.8, Learned_weights="net.bin", Learned_train_steps=200,
+                Learned_train_batch=1024)
+    subprocess.run([GENERATOR, "1000", "0.1"], cwd=tmp_path, check=True, capture_output=True)
+    _write_input(tmp_path, keys)
+    res = subprocess.run([DRIVER, "input"], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert res.returncode == 0, res.stderr
+    assert "Learned proposal: 200 training steps" in res.stderr
+    assert (tmp_path / "net.bin").stat().st_size == 4 * (49152 + 416)
+    first = (tmp_path / "last.rrr").read_bytes()
+    res = subprocess.run([DRIVER, "input"], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert res.returncode == 0, res.stderr
+    assert "weights read from 'net.bin'" in res.stderr
+    assert (tmp_path / "last.rrr").read_bytes() == first
+    s, box, r, O = oracle.rrr_load(str(tmp_path / "last.rrr"))
+    b, ov, tot = oracle.energy(r, O, box)
+    assert s == 200 and not ov.any()
