@@ -80,6 +80,19 @@ def cpu_baseline(r, O, box, seconds):
                 sample=f"{j['sweeps']} sweeps of the N={N} rho={RHO} T={T} start with oracle/cpu_bench")
 
 
+def host_cpu():
+    """The host the CPU baseline ran on: model name and logical CPUs."""
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"cpu_model": model, "host_logical_cpus": os.cpu_count()}
+
+
 def measured_traffic(path=os.path.join(ROOT, "profiles", "k_pass_n32768_rho0.5_pmc.txt")):
     """HBM bytes per colour-pass launch from the committed rocprofv3 PMC
     summary (FETCH_SIZE + WRITE_SIZE, scaled by a plain copy), or None."""
@@ -489,6 +502,7 @@ def main():
         "cpu_baseline": cpu,
     }
     if cpu:
+        cpu.update(host_cpu())
         out["speedup_vs_cpu"] = round(sweeps_per_s / cpu["value"], 1)
     print(json.dumps(out), flush=True)
 
