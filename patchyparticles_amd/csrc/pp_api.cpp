@@ -1099,7 +1099,6 @@ pp_status pp_net_attach(pp_ctx *c, pp_net *net, double learned_fraction, double 
 		HIPCHECK(hipStreamSynchronize(c->stream));
 	}
 	L.on = 1;
-	L.frac = (float) learned_fraction;
 	L.theta_max = theta_max;
 	c->net = net;
 	c->learned_fraction = learned_fraction;

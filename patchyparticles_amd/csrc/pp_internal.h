@@ -63,7 +63,6 @@ constexpr int NN_PROP_W = 16;  // doubles of one drawn proposal
 // Per-context state of the learned passes (scratch sized for the largest list).
 struct LearnDev {
 	int on;
-	float frac;          // share of colour passes that use the learned proposal
 	double theta_max;    // rotation angle of a learned move, uniform in [0, theta_max)
 	__bf16 *xf, *xr;     // [total_act][NN_IN_PAD] descriptors at the current / proposed state
 	float *yf, *yr;      // [total_act][NN_OUT_PAD] network outputs

@@ -97,6 +97,21 @@ pp_net::~pp_net() {
 int ppd::net_reserve(pp_net *n, int m) {
 	const int need = (m + 31) & ~31;
 	if(need <= n->cap) return PP_OK;
+	// the smaller scratch of earlier calls goes (the stream is idle: every
+	// call that used it synchronised)
+	void *old[] = {n->x, n->h, n->dy, n->dh, n->y, n->target, n->loss};
+	for(void *p : old) {
+		if(!p) continue;
+		for(size_t k = 0; k < n->allocs.size(); k++)
+			if(n->allocs[k] == p) {
+				(void) hipFree(p);
+				n->allocs.erase(n->allocs.begin() + (long) k);
+				break;
+			}
+	}
+	n->x = n->h = n->dy = n->dh = nullptr;
+	n->y = n->target = n->loss = nullptr;
+	n->cap = 0;
 	const size_t r = (size_t) need;
 	pp_status st;
 	if((st = nalloc(n, &n->x, r * NN_IN_PAD)) || (st = nalloc(n, &n->h, 3 * r * NN_H)) ||
