@@ -143,93 +143,102 @@ __device__ __forceinline__ bool kf_any_q(double ux, double uy, double uz, const 
 
 // ---------------------------------------------------------------------
 // Filtered angular tests: the decisions of kf_any_p / kf_any_q without the
-// square root and the divisions of d^ for all but a vanishing share of pairs.
+// square root and the divisions of d^, and without forming q's patches, for
+// all but a vanishing share of pairs.
 //
 // For a patch vector v (|v| = 1 to rounding) and the minimum-image d with
 // r2 = d.d as above, the reference decides (d / sqrt(r2)).v > cosmax; its
 // roundings keep that dot within 2e-15 of the exact cosine (d.v)/|d| for
-// |d| ~ 1.  Here s = d.v (rounded: error below 6e-16 |d|) and
-// S = s*s - fl(cosmax^2) r2, whose error against the exact
-// (d.v)^2 - cosmax^2 |d|^2 stays below 2e-15 r2.  Hence s > 0 with
-// S > eps r2 puts the exact cosine more than eps/2 above cosmax, and s <= 0 or
-// S < -eps r2 puts it as far below; either way the reference's rounded dot is
-// on the same side of cosmax.  Inside the band |S| <= eps r2 (one pair in
-// ~10^11) the test falls back to the reference's own arithmetic.  eps = 1e-12.
+// |d| ~ 1.  Here s ~ d.v is formed in an order of its own: on the p side
+// from p's patches by fused multiply-adds (error below 4e-16 |d|), on the q
+// side in q's body frame, e = Oq^T d and d.q_b = c (sigma_b . e) for the
+// sign pattern sigma_b of patch b (error below 2.2e-15 |d| against the
+// reference's rounded patch), so q's patches are never built.  Each side
+// takes the largest t = s|s| over the particle's four patches and compares it
+// with hi = c2 r2 + eps r2 and lo = c2 r2 - eps r2, c2 = fl(cosmax^2); the
+// error of t - c2 r2 against the exact (d.v)|d.v| - cosmax^2 |d|^2 stays below
+// 6e-15 r2.  Hence t > hi puts the exact cosine of that patch more than eps/2
+// above cosmax (and s > 0), and t < lo puts every patch's as far below (or
+// its s <= 0); either way the reference's rounded dots are on the same side
+// of cosmax.  Inside the band lo <= t <= hi (one pair in ~10^11) the test
+// falls back to the reference's own arithmetic.  eps = 1e-12.  The q side
+// scales t by 1/c^2, taken as 3 (exact to 2.3e-16, relative).
 // The filter needs cosmax >= KF_FILTER_MIN_COS; for a smaller (or negative)
 // cosmax every test takes the exact form.
 constexpr double KF_EPS = 1e-12;
 constexpr double KF_FILTER_MIN_COS = 1e-3;
+constexpr double KF_INV_C2 = 3.;
 enum : int { KF_NO = 0, KF_YES = 1, KF_UNSURE = 2 };
 
-// Branch-free classification of s = d.v (or -(d.v) for the q side) against
-// cosmax on a separation with r2 = d.d: bit 0 set if certainly above, bit 1
-// set if inside the band (undecided); c2r2 = fl(cosmax^2) r2, er2 = eps r2.
-__device__ __forceinline__ unsigned kf_fast_cos(double s, double c2r2, double er2) {
-	const double S = s * s - c2r2;
-	const bool pos = s > 0.;
-	const bool yes = pos && (S > er2);
-	const bool band = pos && !(S > er2) && !(S < -er2);
+// The band of one separation (c2 = fl(cosmax^2), r2 = d.d).
+struct KfBand {
+	double hi, lo;
+};
+
+__device__ __forceinline__ KfBand kf_band(double c2, double r2) {
+	const double c = c2 * r2, e = KF_EPS * r2;
+	return {c + e, c - e};
+}
+
+// The same band in q's units (t / c^2).
+__device__ __forceinline__ KfBand kf_band_q(const KfBand &b) { return {KF_INV_C2 * b.hi, KF_INV_C2 * b.lo}; }
+
+// Branch-free classification of the largest t of a side: bit 0 set if
+// certainly above, bit 1 set if inside the band (undecided).
+__device__ __forceinline__ unsigned kf_fast_class(double t, const KfBand &b) {
+	const bool yes = t > b.hi;
+	const bool band = !yes && !(t < b.lo);
 	return (unsigned) yes | ((unsigned) band << 1);
 }
 
-// p side on the raw separation: OR over the 4 patches of P (12 doubles,
-// patch-major) of kf_fast_cos
-__device__ __forceinline__ unsigned kf_fast_p(double dx, double dy, double dz, const double *P, double c2r2,
-                                              double er2) {
-	unsigned f = 0;
-#pragma unroll
-	for(int a = 0; a < 4; a++) f |= kf_fast_cos(kf_dot(dx, dy, dz, P[3 * a], P[3 * a + 1], P[3 * a + 2]), c2r2, er2);
-	return f;
+__device__ __forceinline__ double kf_fdot(double dx, double dy, double dz, double a, double b, double c) {
+	return __builtin_fma(dx, a, __builtin_fma(dy, b, dz * c));
 }
 
-// q side on the raw old (o) and new (n) separations against the patches of
-// Oq, built as in kf_any_q2: OR over the patches of kf_fast_cos(-(d.q_b))
-__device__ __forceinline__ void kf_fast_q2(double ox, double oy, double oz, double oc2r2, double oer2, double nx,
-                                           double ny, double nz, double nc2r2, double ner2, const double Oq[9],
-                                           unsigned &fo, unsigned &fn) {
-	double m[9];
+// p side on the raw separation: the largest (d.p_a)|d.p_a| over the patches P
+// of p (12 doubles, patch-major), classified
+__device__ __forceinline__ unsigned kf_fast_p(double dx, double dy, double dz, const double *P, const KfBand &b) {
+	double t = 0.;
 #pragma unroll
-	for(int e = 0; e < 9; e++) m[e] = Oq[e] * KF_C;
-	fo = 0;
-	fn = 0;
-#pragma unroll
-	for(int b = 0; b < 4; b++) {
-		const bool n0 = (KF_NEG >> (3 * b)) & 1u, n1 = (KF_NEG >> (3 * b + 1)) & 1u, n2 = (KF_NEG >> (3 * b + 2)) & 1u;
-		double q[3];
-#pragma unroll
-		for(int k = 0; k < 3; k++) {
-			double a0 = n0 ? -m[3 * k] : m[3 * k];
-			double a1 = n1 ? -m[3 * k + 1] : m[3 * k + 1];
-			double a2 = n2 ? -m[3 * k + 2] : m[3 * k + 2];
-			q[k] = a0 + a1 + a2;
-		}
-		fo |= kf_fast_cos(-kf_dot(ox, oy, oz, q[0], q[1], q[2]), oc2r2, oer2);
-		fn |= kf_fast_cos(-kf_dot(nx, ny, nz, q[0], q[1], q[2]), nc2r2, ner2);
-		__builtin_amdgcn_sched_barrier(0);
+	for(int a = 0; a < 4; a++) {
+		const double s = kf_fdot(dx, dy, dz, P[3 * a], P[3 * a + 1], P[3 * a + 2]);
+		const double ta = s * __builtin_fabs(s);
+		t = a == 0 ? ta : __builtin_fmax(t, ta);
 	}
+	return kf_fast_class(t, b);
 }
 
-// q side on one separation: kf_fast_q2's classification of d alone
-__device__ __forceinline__ unsigned kf_fast_q1(double dx, double dy, double dz, double c2r2, double er2,
-                                               const double Oq[9]) {
-	double m[9];
-#pragma unroll
-	for(int e = 0; e < 9; e++) m[e] = Oq[e] * KF_C;
-	unsigned f = 0;
+// q side: the largest -(g_b)|g_b| over q's patches, g_b = sigma_b . e with
+// e = Oq^T d (so d.q_b = c g_b and the condition -(d.q_b) > cosmax |d| reads
+// -(g_b)|g_b| > cosmax^2 r2 / c^2)
+__device__ __forceinline__ double kf_q_top(double dx, double dy, double dz, const double Oq[9]) {
+	const double e0 = kf_fdot(dx, dy, dz, Oq[0], Oq[3], Oq[6]);
+	const double e1 = kf_fdot(dx, dy, dz, Oq[1], Oq[4], Oq[7]);
+	const double e2 = kf_fdot(dx, dy, dz, Oq[2], Oq[5], Oq[8]);
+	double t = 0.;
 #pragma unroll
 	for(int b = 0; b < 4; b++) {
-		const bool n0 = (KF_NEG >> (3 * b)) & 1u, n1 = (KF_NEG >> (3 * b + 1)) & 1u, n2 = (KF_NEG >> (3 * b + 2)) & 1u;
-		double q[3];
-#pragma unroll
-		for(int k = 0; k < 3; k++) {
-			double a0 = n0 ? -m[3 * k] : m[3 * k];
-			double a1 = n1 ? -m[3 * k + 1] : m[3 * k + 1];
-			double a2 = n2 ? -m[3 * k + 2] : m[3 * k + 2];
-			q[k] = a0 + a1 + a2;
-		}
-		f |= kf_fast_cos(-kf_dot(dx, dy, dz, q[0], q[1], q[2]), c2r2, er2);
+		const double a0 = ((KF_NEG >> (3 * b)) & 1u) ? -e0 : e0;
+		const double a1 = ((KF_NEG >> (3 * b + 1)) & 1u) ? -e1 : e1;
+		const double a2 = ((KF_NEG >> (3 * b + 2)) & 1u) ? -e2 : e2;
+		const double g = a0 + a1 + a2;
+		const double tb = -(g * __builtin_fabs(g));
+		t = b == 0 ? tb : __builtin_fmax(t, tb);
 	}
-	return f;
+	return t;
+}
+
+// q side on one separation (bq = kf_band_q of its band)
+__device__ __forceinline__ unsigned kf_fast_q1(double dx, double dy, double dz, const KfBand &bq, const double Oq[9]) {
+	return kf_fast_class(kf_q_top(dx, dy, dz, Oq), bq);
+}
+
+// q side on the old (o) and the new (n) separation of a trial
+__device__ __forceinline__ void kf_fast_q2(double ox, double oy, double oz, const KfBand &boq, double nx, double ny,
+                                           double nz, const KfBand &bnq, const double Oq[9], unsigned &fo,
+                                           unsigned &fn) {
+	fo = kf_fast_q1(ox, oy, oz, boq, Oq);
+	fn = kf_fast_q1(nx, ny, nz, bnq, Oq);
 }
 
 // Cell coordinate of x on a grid of n cells over [s, s+L) (periodic).

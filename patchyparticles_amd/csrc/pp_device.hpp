@@ -20,10 +20,33 @@ __device__ __forceinline__ int xcd_remap(int b, int nb) {
 	return (b & 7) * per + (b >> 3);
 }
 
+// DPP controls (gfx9 encoding)
+constexpr int DPP_QUAD_1032 = 0xB1;      // quad_perm [1,0,3,2]
+constexpr int DPP_QUAD_2301 = 0x4E;      // quad_perm [2,3,0,1]
+constexpr int DPP_ROW_MIRROR = 0x140;    // lane i <- 15 - i within a 16-lane row
+constexpr int DPP_HALF_MIRROR = 0x141;   // lane i <- 7 - i within 8 lanes
+
+// Sum over a team of T = 8, 16, 32 or 64 lanes, in every lane of the team,
+// with VALU cross-lane moves only (a __shfl_xor is a ds_bpermute, an LDS
+// round trip per step): DPP butterflies within a 16-lane row, then gfx950's
+// row-pair and half-wave swaps.  Each step stays inside the team, so lanes of
+// the other teams of the wavefront may be inactive.
 template <int T>
 __device__ __forceinline__ int team_sum(int v) {
-#pragma unroll
-	for(int o = T / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, T);
+	static_assert(T == 8 || T == 16 || T == 32 || T == 64, "team of 8 to 64 lanes");
+	v += __builtin_amdgcn_update_dpp(0, v, DPP_QUAD_1032, 0xf, 0xf, false);
+	v += __builtin_amdgcn_update_dpp(0, v, DPP_QUAD_2301, 0xf, 0xf, false);
+	v += __builtin_amdgcn_update_dpp(0, v, DPP_HALF_MIRROR, 0xf, 0xf, false);
+	if constexpr(T >= 16) v += __builtin_amdgcn_update_dpp(0, v, DPP_ROW_MIRROR, 0xf, 0xf, false);
+	if constexpr(T >= 32) {
+		// rows 2j and 2j+1 trade places between two copies: copy sum = row-pair sum
+		const auto p = __builtin_amdgcn_permlane16_swap(v, v, false, false);
+		v = (int) (p[0] + p[1]);
+	}
+	if constexpr(T >= 64) {
+		const auto p = __builtin_amdgcn_permlane32_swap(v, v, false, false);
+		v = (int) (p[0] + p[1]);
+	}
 	return v;
 }
 
@@ -189,13 +212,13 @@ __device__ __forceinline__ void bonds_new(const Trial &t, const double *Pn, doub
 	const double wy = kf_min_image(qy - t.ny, t.Ly, t.iLy);
 	const double wz = kf_min_image(qz - t.nz, t.Lz, t.iLz);
 	const double w2 = wx * wx + wy * wy + wz * wz;
-	const double nc = t.c2 * w2, ne = KF_EPS * w2;
-	const unsigned fn = w2 < t.rc2 ? kf_fast_p(wx, wy, wz, Pn, nc, ne) : 0u;
+	const KfBand bd = kf_band(t.c2, w2);
+	const unsigned fn = w2 < t.rc2 ? kf_fast_p(wx, wy, wz, Pn, bd) : 0u;
 	unsigned unsure = (fn >> 1) & 1u;
 	if(fn & 1u) {
 		double Oq[9];
 		load_o(Oq);
-		const unsigned qn = kf_fast_q1(wx, wy, wz, nc, ne, Oq);
+		const unsigned qn = kf_fast_q1(wx, wy, wz, kf_band_q(bd), Oq);
 		unsure |= (qn >> 1) & 1u;
 		bn += (int) (qn & 1u);
 	}

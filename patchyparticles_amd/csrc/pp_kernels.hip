@@ -470,15 +470,15 @@ __device__ __forceinline__ void bonds_old_new(const Trial &t, const double *Po, 
 	const double wy = kf_min_image(qy - t.ny, t.Ly, t.iLy);
 	const double wz = kf_min_image(qz - t.nz, t.Lz, t.iLz);
 	const double w2 = wx * wx + wy * wy + wz * wz;
-	const double oc = t.c2 * o2, oe = KF_EPS * o2, nc = t.c2 * w2, ne = KF_EPS * w2;
-	const unsigned fo = o2 < t.rc2 ? kf_fast_p(ox, oy, oz, Po, oc, oe) : 0u;
-	const unsigned fn = w2 < t.rc2 ? kf_fast_p(wx, wy, wz, Pn, nc, ne) : 0u;
+	const KfBand bo_ = kf_band(t.c2, o2), bn_ = kf_band(t.c2, w2);
+	const unsigned fo = o2 < t.rc2 ? kf_fast_p(ox, oy, oz, Po, bo_) : 0u;
+	const unsigned fn = w2 < t.rc2 ? kf_fast_p(wx, wy, wz, Pn, bn_) : 0u;
 	unsigned unsure = ((fo | fn) >> 1) & 1u;
 	if((fo | fn) & 1u) {
 		double Oq[9];
 		load_o(Oq);
 		unsigned qo, qn;
-		kf_fast_q2(ox, oy, oz, oc, oe, wx, wy, wz, nc, ne, Oq, qo, qn);
+		kf_fast_q2(ox, oy, oz, kf_band_q(bo_), wx, wy, wz, kf_band_q(bn_), Oq, qo, qn);
 		qo = (fo & 1u) ? qo : 0u;
 		qn = (fn & 1u) ? qn : 0u;
 		unsure |= ((qo | qn) >> 1) & 1u;
