@@ -161,11 +161,19 @@ struct Trial {
 	double c2;  // cosmax^2 for the filtered angular tests, -1: exact tests only
 };
 
+// Minimum-image separation of q from the trial's new position, and its
+// square (MC.c:232-240)
+__device__ __forceinline__ double sep_new(const Trial &t, double qx, double qy, double qz, double &wx, double &wy,
+                                          double &wz) {
+	wx = kf_min_image(qx - t.nx, t.Lx, t.iLx);
+	wy = kf_min_image(qy - t.ny, t.Ly, t.iLy);
+	wz = kf_min_image(qz - t.nz, t.Lz, t.iLz);
+	return wx * wx + wy * wy + wz * wz;
+}
+
 __device__ __forceinline__ bool overlaps_new(const Trial &t, double qx, double qy, double qz) {
-	double dx = kf_min_image(qx - t.nx, t.Lx, t.iLx);
-	double dy = kf_min_image(qy - t.ny, t.Ly, t.iLy);
-	double dz = kf_min_image(qz - t.nz, t.Lz, t.iLz);
-	return (dx * dx + dy * dy + dz * dz) < 1.;
+	double dx, dy, dz;
+	return sep_new(t, qx, qy, qz, dx, dy, dz) < 1.;
 }
 
 // Bonds of neighbour q with the old and the new position (one load of q for
@@ -205,13 +213,11 @@ __device__ __forceinline__ void bonds_old_new_exact(const Trial &t, const double
 // Bonds of neighbour q with the new position only: the filtered decision
 // (a pair inside the filter band sets `band`) and the reference's arithmetic
 // (MC.c:242-262).  For neighbours whose bonds with the old position are known.
+// bonds_new_sep takes the separation (wx, wy, wz) = sep_new(q) and its
+// square w2; bonds_new forms them.
 template <class LoadO>
-__device__ __forceinline__ void bonds_new(const Trial &t, const double *Pn, double qx, double qy, double qz,
-                                          LoadO load_o, int &bn, bool &band) {
-	const double wx = kf_min_image(qx - t.nx, t.Lx, t.iLx);
-	const double wy = kf_min_image(qy - t.ny, t.Ly, t.iLy);
-	const double wz = kf_min_image(qz - t.nz, t.Lz, t.iLz);
-	const double w2 = wx * wx + wy * wy + wz * wz;
+__device__ __forceinline__ void bonds_new_sep(const Trial &t, const double *Pn, double wx, double wy, double wz,
+                                              double w2, LoadO load_o, int &bn, bool &band) {
 	const KfBand bd = kf_band(t.c2, w2);
 	const unsigned fn = w2 < t.rc2 ? kf_fast_p(wx, wy, wz, Pn, bd) : 0u;
 	unsigned unsure = (fn >> 1) & 1u;
@@ -223,6 +229,14 @@ __device__ __forceinline__ void bonds_new(const Trial &t, const double *Pn, doub
 		bn += (int) (qn & 1u);
 	}
 	band |= unsure != 0u;
+}
+
+template <class LoadO>
+__device__ __forceinline__ void bonds_new(const Trial &t, const double *Pn, double qx, double qy, double qz,
+                                          LoadO load_o, int &bn, bool &band) {
+	double wx, wy, wz;
+	const double w2 = sep_new(t, qx, qy, qz, wx, wy, wz);
+	bonds_new_sep(t, Pn, wx, wy, wz, w2, load_o, bn, band);
 }
 
 template <class LoadO>

@@ -446,8 +446,9 @@ constexpr int LDS_PID = (CELL_CAP + 1) / 2;  // CELL_CAP ints
 constexpr int LDS_SB = (CELL_CAP + 1) / 2;   // CELL_CAP ints: each member's bonds with the static neighbours
 constexpr int LDS_TRIAL = CELL_CAP * TRIAL_W;
 constexpr int LDS_NB = REC_W / 2;      // the cell record, REC_W ints
-constexpr int LDS_ACC = ACC_TAB;       // exp(-k/T), k < ACC_TAB
-constexpr int LDS_FIXED = LDS_OWN + LDS_PID + LDS_SB + LDS_TRIAL + LDS_NB + LDS_ACC;
+constexpr int LDS_STAGE = 22;  // a trial's O' (9) and patches (12), formed one value per lane
+constexpr int LDS_FIXED = LDS_OWN + LDS_PID + LDS_SB + LDS_TRIAL + LDS_NB + LDS_STAGE;
+constexpr int RANK_BITS = 5;  // bits of a member rank (< CELL_CAP)
 // + the orientations of the T register-resident neighbours, component-major
 template <int T>
 constexpr int team_lds() {
@@ -455,6 +456,8 @@ constexpr int team_lds() {
 }
 static_assert((LDS_OWN + LDS_PID + LDS_SB + LDS_TRIAL) % 2 == 0 && LDS_FIXED % 2 == 0,
               "16-byte aligned record and team blocks");
+// two 256-lane workgroups of 32-lane teams share a CU's 160 KiB
+static_assert(2 * 8 * team_lds<32>() * sizeof(double) <= 160 * 1024, "two pass workgroups per CU");
 
 // The same decisions through the filtered tests (no square root, no
 // divisions, no branch per patch).  A pair inside a filter band sets `band`;
@@ -509,7 +512,7 @@ __device__ __forceinline__ void pass_cell(const DevState &d, uint64_t sweep, int
 	int *sbond = (int *) (own + LDS_OWN + LDS_PID);  // member m's bonds with the static neighbours
 	double *trials = own + LDS_OWN + LDS_PID + LDS_SB;
 	int *nb_tab = (int *) (trials + LDS_TRIAL);  // the cell's record (build_record)
-	double *acc_tab = trials + LDS_TRIAL + LDS_NB;
+	double *stage = trials + LDS_TRIAL + LDS_NB;  // LDS_STAGE doubles
 	double *ori = own + LDS_FIXED;  // orientation component c of this lane's first neighbour at c * T + lane
 	const int S = d.stride;
 	{
@@ -573,7 +576,6 @@ __device__ __forceinline__ void pass_cell(const DevState &d, uint64_t sweep, int
 		double orv[9];
 #pragma unroll
 		for(int c = 0; c < 9; c++) orv[c] = cqr[0] >= 0 ? F[(3 + c) * S + cqr[0]] : 0.;
-		const double acc_v = lane < ACC_TAB ? rp.acc_tab[lane] : 0.;
 		Trial tr;
 		tr.Lx = unid<T>(rp.g.box[0]);
 		tr.Ly = unid<T>(rp.g.box[1]);
@@ -590,16 +592,31 @@ __device__ __forceinline__ void pass_cell(const DevState &d, uint64_t sweep, int
 		const int ix = cloc / (n1 * n2), iy = (cloc / n2) % n1, iz = cloc % n2;
 
 		// 3. the cell's trials, one lane each (Philox: no memory traffic),
-		// and the particle indices into LDS
+		// and the particle indices into LDS.  A trial's slot 0 (the member
+		// rank) is replaced by its acceptance bits: bit j is set iff the draw
+		// u < exp(-j/T), the replica's factor for a rise of j bonds
+		// (MC.c:332), so a trial decides without a table read.  The ranks go
+		// to ballot bit planes, read back without LDS at every trial.
+		int my_k = 0;
 		if(lane < ntrial) {
 			double tv[TRIAL_W];
 			const int col = ((ix & 1) << 2) | ((iy & 1) << 1) | (iz & 1);
 			make_trial(d, rp, r, sweep, cloc, col, lane, cnt, tv);
+			unsigned am = 0;
+#pragma unroll
+			for(int j = 1; j < ACC_TAB; j++) am |= (unsigned) (tv[13] < rp.acc_tab[j]) << j;
+			my_k = (int) tv[0];
+			tv[0] = (double) am;
 #pragma unroll
 			for(int e = 0; e < TRIAL_W; e++) trials[lane * TRIAL_W + e] = tv[e];
 		}
+		static_assert(CELL_CAP <= (1 << RANK_BITS) && MAX_TRIALS_PER_CELL <= 32 && CELL_CAP <= 32,
+		              "trial ranks in RANK_BITS ballot planes, one lane per trial");
+		unsigned long long kplane[RANK_BITS];
+#pragma unroll
+		for(int b = 0; b < RANK_BITS; b++) kplane[b] = __ballot((my_k >> b) & 1);
+		const int koff = (threadIdx.x & 63) & ~(T - 1);
 		if(lane < cnt) own_pid[lane] = my_pid;
-		if(lane < ACC_TAB) acc_tab[lane] = acc_v;
 #pragma unroll
 		for(int c = 0; c < 9; c++) ori[c * T + lane] = orv[c];
 		team_sync();
@@ -621,15 +638,11 @@ __device__ __forceinline__ void pass_cell(const DevState &d, uint64_t sweep, int
 		// this lane's static neighbours (the surrounding cells, fixed during
 		// the pass): the one in registers and, in dense neighbourhoods, the
 		// rest from HBM
-		auto each_static = [&](auto &&f) {
+		auto reg_o = [&](int u, double *Oq) {
 #pragma unroll
-			for(int u = 0; u < CPL; u++)
-				if(cqr[u] >= 0)
-					f(cxr[u], cyr[u], czr[u], [&](double *Oq) {
-						const int q = cqr[u];
-#pragma unroll
-						for(int c = 0; c < 9; c++) Oq[c] = u == 0 ? ori[c * T + lane] : F[(3 + c) * S + q];
-					});
+			for(int c = 0; c < 9; c++) Oq[c] = u == 0 ? ori[c * T + lane] : F[(3 + c) * S + cqr[u]];
+		};
+		auto each_far = [&](auto &&f) {
 			for(int b = T * CPL; b < M; b += T)
 				if(b + lane < M) {
 					const int q = nb_slot(b + lane);
@@ -638,6 +651,12 @@ __device__ __forceinline__ void pass_cell(const DevState &d, uint64_t sweep, int
 						for(int c = 0; c < 9; c++) Oq[c] = F[(3 + c) * S + q];
 					});
 				}
+		};
+		auto each_static = [&](auto &&f) {
+#pragma unroll
+			for(int u = 0; u < CPL; u++)
+				if(cqr[u] >= 0) f(cxr[u], cyr[u], czr[u], [&](double *Oq) { reg_o(u, Oq); });
+			each_far(f);
 		};
 		team_sync();
 		// 5. KEEP: every member's bonds with the static neighbours
@@ -669,14 +688,24 @@ __device__ __forceinline__ void pass_cell(const DevState &d, uint64_t sweep, int
 			team_sync();
 		}
 		int c_acc = 0, d_bonds = 0;
+		// this lane's share of a trial's O' and patches (see the trial loop)
+		static_assert(T >= 24, "a team forms O' and the patches one value per lane");
+		const int rot_idx = lane < 9 ? 0 : lane - 9;
+		const int rot_row = lane < 9 ? lane / 3 : (lane < 21 ? rot_idx % 3 : 0);
+		const int rot_col = lane % 3;
+		const unsigned rot_neg = lane < 21 ? (KF_NEG >> (3 * (rot_idx / 3))) & 7u : 0u;
 
 		for(int t = 0; t < ntrial; t++) {
-			// the trial (one LDS read round) and this lane's member of the cell
+			// the moving member's rank from the ballot planes; then one LDS
+			// read round: the trial, the member's state and this lane's member
+			int k = 0;
+#pragma unroll
+			for(int b = 0; b < RANK_BITS; b++) k |= (int) ((kplane[b] >> (koff + t)) & 1ull) << b;
+			k = uni<T>(k);
 			const double *tp = trials + t * TRIAL_W;
 			double tv[TRIAL_W];
 #pragma unroll
 			for(int e = 0; e < TRIAL_W; e++) tv[e] = tp[e];
-			const int k = uni<T>((int) tv[0]);
 			const bool has_m = lane < cnt && lane != k;
 			const double *wl = own + (lane < cnt ? lane : 0) * OWN_W;
 			double mx = 0., my = 0., mz = 0.;
@@ -685,11 +714,11 @@ __device__ __forceinline__ void pass_cell(const DevState &d, uint64_t sweep, int
 				my = wl[1];
 				mz = wl[2];
 			}
-			// the moving member's state (a second round)
 			const double *me = own + k * OWN_W;
-			double Om[9], Po[12];
-#pragma unroll
-			for(int c = 0; c < 9; c++) Om[c] = me[3 + c];
+			// row rot_row of the member's O (the lane's share of O' below)
+			const double *orow = me + 3 + 3 * rot_row;
+			const double o0 = orow[0], o1 = orow[1], o2 = orow[2];
+			double Po[12];
 #pragma unroll
 			for(int c = 0; c < 12; c++) Po[c] = me[NCOMP + c];
 			tr.px = unid<T>(me[0]);
@@ -699,15 +728,21 @@ __device__ __forceinline__ void pass_cell(const DevState &d, uint64_t sweep, int
 			tr.ny = unid<T>(tr.py + tv[2]);
 			tr.nz = unid<T>(tr.pz + tv[3]);
 			// a trial that leaves the cell is rejected
-			if(uni<T>(cell_coord(tr.nx, sh0, tr.iLx, n0)) != ix || uni<T>(cell_coord(tr.ny, sh1, tr.iLy, n1)) != iy ||
-			   uni<T>(cell_coord(tr.nz, sh2, tr.iLz, n2)) != iz)
-				continue;
+			const bool out = (uni<T>(cell_coord(tr.nx, sh0, tr.iLx, n0)) != ix) |
+			                 (uni<T>(cell_coord(tr.ny, sh1, tr.iLy, n1)) != iy) |
+			                 (uni<T>(cell_coord(tr.nz, sh2, tr.iLz, n2)) != iz);
+			if(out) continue;
 
 			// phase 1: hard cores at the new position, wavefront-ballot early exit
+			// (the register neighbours' separations are kept for phase 2)
 			bool ov = false;
+			double sx[CPL], sy[CPL], sz[CPL], s2[CPL];
 #pragma unroll
 			for(int u = 0; u < CPL; u++)
-				if(cqr[u] >= 0) ov |= overlaps_new(tr, cxr[u], cyr[u], czr[u]);
+				if(cqr[u] >= 0) {
+					s2[u] = sep_new(tr, cxr[u], cyr[u], czr[u], sx[u], sy[u], sz[u]);
+					ov |= s2[u] < 1.;
+				}
 			if(has_m) ov |= overlaps_new(tr, mx, my, mz);
 			for(int b = T * CPL; b < M; b += T)
 				if(b + lane < M) {
@@ -717,26 +752,28 @@ __device__ __forceinline__ void pass_cell(const DevState &d, uint64_t sweep, int
 			if(team_any<T>(ov)) continue;
 
 			// O' = O R in the body frame (utils.c:172-184, 204-209) and its
-			// patches (utils.h:17-21), formed by every lane in registers:
-			// P'[a][kk] = sum_j O'[kk][j] b_a[j] with b_a[j] = +-c, and
-			// (+-x) c = +-(x c) exactly, so the products x c are shared
-			double On[9], Pn[12], mc[9];
-#pragma unroll
-			for(int i = 0; i < 3; i++)
-#pragma unroll
-				for(int j = 0; j < 3; j++)
-					On[3 * i + j] = Om[3 * i] * tv[4 + j] + Om[3 * i + 1] * tv[4 + 3 + j] + Om[3 * i + 2] * tv[4 + 6 + j];
-#pragma unroll
-			for(int e = 0; e < 9; e++) mc[e] = On[e] * KF_C;
-#pragma unroll
-			for(int a = 0; a < 4; a++)
-#pragma unroll
-				for(int kk = 0; kk < 3; kk++) {
-					const double a0 = ((KF_NEG >> (3 * a)) & 1u) ? -mc[3 * kk] : mc[3 * kk];
-					const double a1 = ((KF_NEG >> (3 * a + 1)) & 1u) ? -mc[3 * kk + 1] : mc[3 * kk + 1];
-					const double a2 = ((KF_NEG >> (3 * a + 2)) & 1u) ? -mc[3 * kk + 2] : mc[3 * kk + 2];
-					Pn[3 * a + kk] = a0 + a1 + a2;
+			// patches (utils.h:17-21), one value per lane in the reference's
+			// order: lanes 0-8 O'[l], lanes 9-20 the patch component
+			// P'[a][kk] = sum_j O'[kk][j] b_a[j], l = 9 + 3a + kk.  Each of
+			// them forms row rot_row of O'; with b_a[j] = +-c, (+-x) c = +-(x c)
+			// exactly, so a patch component is the signed sum of that row
+			// times c.  The values meet in the stage row.
+			double rv = 0.;
+			if(lane < 21) {
+				const double r0 = o0 * tv[4] + o1 * tv[4 + 3] + o2 * tv[4 + 6];
+				const double r1 = o0 * tv[5] + o1 * tv[5 + 3] + o2 * tv[5 + 6];
+				const double r2 = o0 * tv[6] + o1 * tv[6 + 3] + o2 * tv[6 + 6];
+				if(lane < 9) rv = rot_col == 0 ? r0 : (rot_col == 1 ? r1 : r2);
+				else {
+					const double m0 = r0 * KF_C, m1 = r1 * KF_C, m2 = r2 * KF_C;
+					rv = (rot_neg & 1u ? -m0 : m0) + (rot_neg & 2u ? -m1 : m1) + (rot_neg & 4u ? -m2 : m2);
 				}
+				stage[lane] = rv;
+			}
+			team_sync();
+			double Pn[12];
+#pragma unroll
+			for(int c = 0; c < 12; c++) Pn[c] = stage[9 + c];
 
 			// phase 2: bonds at the old and the new position, over this lane's
 			// neighbours: the static ones (KEEP: new position only, the old
@@ -750,7 +787,12 @@ __device__ __forceinline__ void pass_cell(const DevState &d, uint64_t sweep, int
 				int bns = 0, bom = 0, bnm = 0;
 				bool band = tr.c2 < 0.;  // no filter: every lane takes the exact form
 				if(!band) {
-					each_static([&](double qx, double qy, double qz, auto load_o) {
+#pragma unroll
+					for(int u = 0; u < CPL; u++)
+						if(cqr[u] >= 0)
+							bonds_new_sep(tr, Pn, sx[u], sy[u], sz[u], s2[u], [&](double *Oq) { reg_o(u, Oq); }, bns,
+							              band);
+					each_far([&](double qx, double qy, double qz, auto load_o) {
 						bonds_new(tr, Pn, qx, qy, qz, load_o, bns, band);
 					});
 					if(has_m) bonds_old_new(tr, Po, Pn, mx, my, mz, member_o, bom, bnm, band);
@@ -794,26 +836,24 @@ __device__ __forceinline__ void pass_cell(const DevState &d, uint64_t sweep, int
 			}
 			const int db = bn - bo;  // E = -bonds, dE = -db
 			const double dE = (double) (-db);
-			// MC.c:332: accept if dE < 0 or a uniform draw < exp(-dE/T); dE is
-			// an integer, exp(-dE/T) comes from the replica's table
+			// MC.c:332: accept if dE < 0 or a uniform draw u < exp(-dE/T); dE
+			// is an integer: below ACC_TAB the trial's acceptance bits hold
+			// the comparison with the replica's exp(-dE/T)
 			const int idE = -db;
-			const double w = idE <= 0 ? 1. : (idE < ACC_TAB ? acc_tab[idE] : exp(-dE / temp));
-			const bool acc = (dE < 0.) || (tv[13] < w);
+			const unsigned am = (unsigned) tv[0];
+			const bool acc = idE < 0 ||
+			                 (idE == 0 ? tv[13] < 1.
+			                           : (idE < ACC_TAB ? ((am >> idE) & 1u) != 0u : tv[13] < exp(-dE / temp)));
 			if(acc) {
 				c_acc++;
 				d_bonds += db;
 				// member k takes the new position, orientation and patches
-				if(lane == 0) {
-					if constexpr(KEEP) sbond[k] = s_new;
-					double *wk = own + k * OWN_W;
-					wk[0] = tr.nx;
-					wk[1] = tr.ny;
-					wk[2] = tr.nz;
-#pragma unroll
-					for(int c = 0; c < 9; c++) wk[3 + c] = On[c];
-#pragma unroll
-					for(int c = 0; c < 12; c++) wk[NCOMP + c] = Pn[c];
-				}
+				// (O' and P' are contiguous in its record, lane l's at 3 + l)
+				double *wk = own + k * OWN_W;
+				if(lane < 21) wk[3 + lane] = rv;
+				else if(lane < 24) wk[lane - 21] = lane == 21 ? tr.nx : (lane == 22 ? tr.ny : tr.nz);
+				if constexpr(KEEP)
+					if(lane == 0) sbond[k] = s_new;
 			}
 			team_sync();
 		}
