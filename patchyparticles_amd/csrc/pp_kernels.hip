@@ -505,7 +505,7 @@ constexpr int KEEP_MIN_TRIALS = 4;
 // the static neighbours are counted up front and kept as it moves, so a trial
 // tests only the new position against them.  Without KEEP (about one trial
 // per member) every trial tests both positions.
-template <int T, int CPL, bool KEEP>
+template <int T, int CPL, bool KEEP, bool ONE>
 __device__ __forceinline__ void pass_cell(const DevState &d, uint64_t sweep, int pass, int gi, int lane,
                                           double *own) {
 	int *own_pid = (int *) (own + LDS_OWN);
@@ -513,7 +513,11 @@ __device__ __forceinline__ void pass_cell(const DevState &d, uint64_t sweep, int
 	double *trials = own + LDS_OWN + LDS_PID + LDS_SB;
 	int *nb_tab = (int *) (trials + LDS_TRIAL);  // the cell's record (build_record)
 	double *stage = trials + LDS_TRIAL + LDS_NB;  // LDS_STAGE doubles
-	double *ori = own + LDS_FIXED;  // orientation component c of this lane's first neighbour at c * T + lane
+	// the orientation of this lane's first neighbour: in registers where
+	// they allow (scalar replica constants), else in LDS, component c at
+	// c * T + lane
+	constexpr bool REG_ORI = ONE || T == 64;
+	double *ori = own + LDS_FIXED;
 	const int S = d.stride;
 	{
 		// 1. the cell's record: surrounding slot ranges, own range, replica
@@ -524,7 +528,7 @@ __device__ __forceinline__ void pass_cell(const DevState &d, uint64_t sweep, int
 		const int s0 = uni<T>(nb_tab[59]);
 		const int cnt = uni<T>(nb_tab[60]);
 		const int cloc = uni<T>(nb_tab[61]);
-		const int r = uni<T>(nb_tab[62]);
+		const int r = ONE ? 0 : uni<T>(nb_tab[62]);
 		const int sel = uni<T>(nb_tab[63]);
 		if(cnt > CELL_CAP) {
 			if(lane == 0) atomicOr(d.err, (int) ERR_CELL_CAP);
@@ -617,8 +621,10 @@ __device__ __forceinline__ void pass_cell(const DevState &d, uint64_t sweep, int
 		for(int b = 0; b < RANK_BITS; b++) kplane[b] = __ballot((my_k >> b) & 1);
 		const int koff = (threadIdx.x & 63) & ~(T - 1);
 		if(lane < cnt) own_pid[lane] = my_pid;
+		if constexpr(!REG_ORI) {
 #pragma unroll
-		for(int c = 0; c < 9; c++) ori[c * T + lane] = orv[c];
+			for(int c = 0; c < 9; c++) ori[c * T + lane] = orv[c];
+		}
 		team_sync();
 		// 4. members into LDS ranked by particle index, with their patch vectors
 		if(lane < cnt) {
@@ -640,7 +646,7 @@ __device__ __forceinline__ void pass_cell(const DevState &d, uint64_t sweep, int
 		// rest from HBM
 		auto reg_o = [&](int u, double *Oq) {
 #pragma unroll
-			for(int c = 0; c < 9; c++) Oq[c] = u == 0 ? ori[c * T + lane] : F[(3 + c) * S + cqr[u]];
+			for(int c = 0; c < 9; c++) Oq[c] = u == 0 ? (REG_ORI ? orv[c] : ori[c * T + lane]) : F[(3 + c) * S + cqr[u]];
 		};
 		auto each_far = [&](auto &&f) {
 			for(int b = T * CPL; b < M; b += T)
@@ -660,6 +666,7 @@ __device__ __forceinline__ void pass_cell(const DevState &d, uint64_t sweep, int
 		};
 		team_sync();
 		// 5. KEEP: every member's bonds with the static neighbours
+		unsigned mbits = 0;
 		if constexpr(KEEP) {
 			for(int m = 0; m < cnt; m++) {
 				const double *wm = own + m * OWN_W;
@@ -684,6 +691,29 @@ __device__ __forceinline__ void pass_cell(const DevState &d, uint64_t sweep, int
 				}
 				b = team_sum<T>(b);
 				if(lane == 0) sbond[m] = b;
+			}
+			// and the members' bonds among themselves: lane m < cnt keeps bit
+			// j of mbits set iff members m and j are bonded (exact form; the
+			// trials keep it current, so a trial tests only the new position)
+			if(lane < cnt && cnt > 1) {
+				const double *wm = own + lane * OWN_W;
+				Trial tm = tr;
+				tm.nx = wm[0];
+				tm.ny = wm[1];
+				tm.nz = wm[2];
+				double Pm[12];
+#pragma unroll
+				for(int c = 0; c < 12; c++) Pm[c] = wm[NCOMP + c];
+				for(int j = 0; j < cnt; j++) {
+					if(j == lane) continue;
+					const double *wj = own + j * OWN_W;
+					int b = 0;
+					bonds_new_exact(tm, Pm, wj[0], wj[1], wj[2], [&](double *Oq) {
+#pragma unroll
+						for(int c = 0; c < 9; c++) Oq[c] = wj[3 + c];
+					}, b);
+					mbits |= (unsigned) b << j;
+				}
 			}
 			team_sync();
 		}
@@ -718,9 +748,10 @@ __device__ __forceinline__ void pass_cell(const DevState &d, uint64_t sweep, int
 			// row rot_row of the member's O (the lane's share of O' below)
 			const double *orow = me + 3 + 3 * rot_row;
 			const double o0 = orow[0], o1 = orow[1], o2 = orow[2];
-			double Po[12];
+			double Po[12];  // the old patches (two-sided tests only)
+			if constexpr(!KEEP)
 #pragma unroll
-			for(int c = 0; c < 12; c++) Po[c] = me[NCOMP + c];
+				for(int c = 0; c < 12; c++) Po[c] = me[NCOMP + c];
 			tr.px = unid<T>(me[0]);
 			tr.py = unid<T>(me[1]);
 			tr.pz = unid<T>(me[2]);
@@ -743,7 +774,11 @@ __device__ __forceinline__ void pass_cell(const DevState &d, uint64_t sweep, int
 					s2[u] = sep_new(tr, cxr[u], cyr[u], czr[u], sx[u], sy[u], sz[u]);
 					ov |= s2[u] < 1.;
 				}
-			if(has_m) ov |= overlaps_new(tr, mx, my, mz);
+			double msx = 0., msy = 0., msz = 0., ms2 = 0.;
+			if(has_m) {
+				ms2 = sep_new(tr, mx, my, mz, msx, msy, msz);
+				ov |= ms2 < 1.;
+			}
 			for(int b = T * CPL; b < M; b += T)
 				if(b + lane < M) {
 					const int q = nb_slot(b + lane);
@@ -782,9 +817,10 @@ __device__ __forceinline__ void pass_cell(const DevState &d, uint64_t sweep, int
 #pragma unroll
 				for(int c = 0; c < 9; c++) Oq[c] = wl[3 + c];
 			};
-			int bo, bn, s_new = 0;
+			int bo, bn, s_new = 0, bnm = 0;
 			if constexpr(KEEP) {
-				int bns = 0, bom = 0, bnm = 0;
+				int bns = 0;
+				const int bom = has_m ? (int) ((mbits >> k) & 1u) : 0;
 				bool band = tr.c2 < 0.;  // no filter: every lane takes the exact form
 				if(!band) {
 #pragma unroll
@@ -795,16 +831,15 @@ __device__ __forceinline__ void pass_cell(const DevState &d, uint64_t sweep, int
 					each_far([&](double qx, double qy, double qz, auto load_o) {
 						bonds_new(tr, Pn, qx, qy, qz, load_o, bns, band);
 					});
-					if(has_m) bonds_old_new(tr, Po, Pn, mx, my, mz, member_o, bom, bnm, band);
+					if(has_m) bonds_new_sep(tr, Pn, msx, msy, msz, ms2, member_o, bnm, band);
 				}
 				if(team_any<T>(band) && band) {
 					bns = 0;
-					bom = 0;
 					bnm = 0;
 					each_static([&](double qx, double qy, double qz, auto load_o) {
 						bonds_new_exact(tr, Pn, qx, qy, qz, load_o, bns);
 					});
-					if(has_m) bonds_old_new_exact(tr, Po, Pn, mx, my, mz, member_o, bom, bnm);
+					if(has_m) bonds_new_exact(tr, Pn, mx, my, mz, member_o, bnm);
 				}
 				// three counts below 1024 in one reduction
 				const int packed = team_sum<T>(bns | (bnm << 10) | (bom << 20));
@@ -852,8 +887,12 @@ __device__ __forceinline__ void pass_cell(const DevState &d, uint64_t sweep, int
 				double *wk = own + k * OWN_W;
 				if(lane < 21) wk[3 + lane] = rv;
 				else if(lane < 24) wk[lane - 21] = lane == 21 ? tr.nx : (lane == 22 ? tr.ny : tr.nz);
-				if constexpr(KEEP)
+				if constexpr(KEEP) {
 					if(lane == 0) sbond[k] = s_new;
+					// member k's bonds with the others are now the new ones
+					const unsigned long long nb_bits = (__ballot(bnm != 0) >> koff) & 0xffffffffull;
+					if(lane < cnt) mbits = lane == k ? (unsigned) nb_bits : ((mbits & ~(1u << k)) | ((unsigned) bnm << k));
+				}
 			}
 			team_sync();
 		}
@@ -885,7 +924,7 @@ __device__ __forceinline__ void pass_cell(const DevState &d, uint64_t sweep, int
 // list is in cell order, so XCD x takes the x-th eighth of the entries
 // actually listed (a slab of the box, whose shared neighbour reads its L2
 // then serves); the surplus workgroups of each XCD leave at once.
-template <int T, int CPL, bool KEEP>
+template <int T, int CPL, bool KEEP, bool ONE>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_pass(PassArgs a) {
 	extern __shared__ double lds[];
 	const int lane = threadIdx.x & (T - 1);
@@ -897,7 +936,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
 	if(i >= per_xcd) return;
 	const int gi = uni<T>(((blockIdx.x & 7) * per_xcd + i) * tpb + team);
 	if(gi >= total) return;
-	pass_cell<T, CPL, KEEP>(a.d, a.sweep, a.pass, gi, lane, lds + team * team_lds<T>());
+	pass_cell<T, CPL, KEEP, ONE>(a.d, a.sweep, a.pass, gi, lane, lds + team * team_lds<T>());
 }
 
 // Grid-wide barrier of a cooperative launch (every workgroup resident).
@@ -945,34 +984,43 @@ __global__ __launch_bounds__(256) void k_sweep_passes(PassArgs a) {
 	double *own = lds + team * team_lds<T>();
 	for(int k = 0; k < 8; k++) {
 		const int total = uni<T>(d.pfx[k * (d.R + 1) + d.R]);
-		for(int gi = g0; gi < total; gi += nteams) pass_cell<T, CPL, KEEP>(d, a.sweep, k, gi, lane, own);
+		for(int gi = g0; gi < total; gi += nteams) pass_cell<T, CPL, KEEP, false>(d, a.sweep, k, gi, lane, own);
 		if(k < 7) grid_barrier(d.bar, d.err);
 	}
 }
 
-template <int T, int CPL, bool KEEP>
+template <int T, int CPL, bool KEEP, bool ONE>
 static void pass_T(const DevState &d, uint64_t sweep, int pass, int block, hipStream_t s) {
 	const int tpb = block / T;
 	int nb = (d.total_act + tpb - 1) / tpb;
 	nb = (nb + 7) & ~7;  // 8 XCDs x the most workgroups one XCD may need
 	size_t lds = (size_t) tpb * team_lds<T>() * sizeof(double);
 	PassArgs a{d, sweep, pass};
-	hipLaunchKernelGGL((k_pass<T, CPL, KEEP>), dim3(nb), dim3(block), lds, s, a);
+	hipLaunchKernelGGL((k_pass<T, CPL, KEEP, ONE>), dim3(nb), dim3(block), lds, s, a);
 }
 
 size_t pass_lds_bytes(int T, int block) {
 	return (size_t) (block / T) * (T == 32 ? team_lds<32>() : team_lds<64>()) * sizeof(double);
 }
 
+// The form is chosen per context: 32 or 64 lanes per cell, kept static
+// bonds or not, and one replica (its constants then load to scalar registers
+// for every team) or several.
+template <int T, bool KEEP>
+static void pass_TK(const DevState &d, uint64_t sweep, int pass, int block, hipStream_t s) {
+	if(d.R == 1) pass_T<T, 1, KEEP, true>(d, sweep, pass, block, s);
+	else pass_T<T, 1, KEEP, false>(d, sweep, pass, block, s);
+}
+
 void launch_pass(const DevState &d, uint64_t sweep, int pass, int T, int block, hipStream_t s) {
 	const bool keep = d.trials_per_cell >= KEEP_MIN_TRIALS;
 	if(T == 32) {
-		if(keep) pass_T<32, 1, true>(d, sweep, pass, block, s);
-		else pass_T<32, 1, false>(d, sweep, pass, block, s);
+		if(keep) pass_TK<32, true>(d, sweep, pass, block, s);
+		else pass_TK<32, false>(d, sweep, pass, block, s);
 	}
 	else {
-		if(keep) pass_T<64, 1, true>(d, sweep, pass, block, s);
-		else pass_T<64, 1, false>(d, sweep, pass, block, s);
+		if(keep) pass_TK<64, true>(d, sweep, pass, block, s);
+		else pass_TK<64, false>(d, sweep, pass, block, s);
 	}
 }
 
