@@ -448,14 +448,14 @@ constexpr int LDS_TRIAL = CELL_CAP * TRIAL_W;
 constexpr int LDS_NB = REC_W / 2;      // the cell record, REC_W ints
 constexpr int LDS_STAGE = 22;  // a trial's O' (9) and patches (12), formed one value per lane
 constexpr int LDS_FIXED = LDS_OWN + LDS_PID + LDS_SB + LDS_TRIAL + LDS_NB + LDS_STAGE;
-constexpr int RANK_BITS = 5;  // bits of a member rank (< CELL_CAP)
 // + the orientations of the T register-resident neighbours, component-major
 template <int T>
 constexpr int team_lds() {
 	return LDS_FIXED + 9 * T;
 }
-static_assert((LDS_OWN + LDS_PID + LDS_SB + LDS_TRIAL) % 2 == 0 && LDS_FIXED % 2 == 0,
-              "16-byte aligned record and team blocks");
+static_assert((LDS_STAGE + LDS_TRIAL + LDS_OWN + LDS_PID + LDS_SB) % 2 == 0 && LDS_FIXED % 2 == 0 &&
+                  (LDS_STAGE + LDS_TRIAL) % 2 == 0 && TRIAL_W % 2 == 0 && OWN_W % 2 == 0,
+              "16-byte aligned trials, members, record and team blocks");
 // two 256-lane workgroups of 32-lane teams share a CU's 160 KiB
 static_assert(2 * 8 * team_lds<32>() * sizeof(double) <= 160 * 1024, "two pass workgroups per CU");
 
@@ -507,17 +507,19 @@ constexpr int KEEP_MIN_TRIALS = 4;
 // per member) every trial tests both positions.
 template <int T, int CPL, bool KEEP, bool ONE>
 __device__ __forceinline__ void pass_cell(const DevState &d, uint64_t sweep, int pass, int gi, int lane,
-                                          double *own) {
+                                          double *blk) {
+	// the team's block, the per-trial data first (short LDS offsets)
+	double *stage = blk;                    // LDS_STAGE doubles
+	double *trials = blk + LDS_STAGE;       // LDS_TRIAL
+	double *own = trials + LDS_TRIAL;       // LDS_OWN
 	int *own_pid = (int *) (own + LDS_OWN);
 	int *sbond = (int *) (own + LDS_OWN + LDS_PID);  // member m's bonds with the static neighbours
-	double *trials = own + LDS_OWN + LDS_PID + LDS_SB;
-	int *nb_tab = (int *) (trials + LDS_TRIAL);  // the cell's record (build_record)
-	double *stage = trials + LDS_TRIAL + LDS_NB;  // LDS_STAGE doubles
+	int *nb_tab = (int *) (own + LDS_OWN + LDS_PID + LDS_SB);  // the cell's record (build_record)
 	// the orientation of this lane's first neighbour: in registers where
 	// they allow (scalar replica constants), else in LDS, component c at
 	// c * T + lane
 	constexpr bool REG_ORI = ONE || T == 64;
-	double *ori = own + LDS_FIXED;
+	double *ori = blk + LDS_FIXED;
 	const int S = d.stride;
 	{
 		// 1. the cell's record: surrounding slot ranges, own range, replica
@@ -599,8 +601,8 @@ __device__ __forceinline__ void pass_cell(const DevState &d, uint64_t sweep, int
 		// and the particle indices into LDS.  A trial's slot 0 (the member
 		// rank) is replaced by its acceptance bits: bit j is set iff the draw
 		// u < exp(-j/T), the replica's factor for a rise of j bonds
-		// (MC.c:332), so a trial decides without a table read.  The ranks go
-		// to ballot bit planes, read back without LDS at every trial.
+		// (MC.c:332), so a trial decides without a table read.  Lane t keeps
+		// trial t's rank.
 		int my_k = 0;
 		if(lane < ntrial) {
 			double tv[TRIAL_W];
@@ -614,11 +616,7 @@ __device__ __forceinline__ void pass_cell(const DevState &d, uint64_t sweep, int
 #pragma unroll
 			for(int e = 0; e < TRIAL_W; e++) trials[lane * TRIAL_W + e] = tv[e];
 		}
-		static_assert(CELL_CAP <= (1 << RANK_BITS) && MAX_TRIALS_PER_CELL <= 32 && CELL_CAP <= 32,
-		              "trial ranks in RANK_BITS ballot planes, one lane per trial");
-		unsigned long long kplane[RANK_BITS];
-#pragma unroll
-		for(int b = 0; b < RANK_BITS; b++) kplane[b] = __ballot((my_k >> b) & 1);
+		static_assert(MAX_TRIALS_PER_CELL <= 32 && CELL_CAP <= 32, "one lane per trial");
 		const int koff = (threadIdx.x & 63) & ~(T - 1);
 		if(lane < cnt) own_pid[lane] = my_pid;
 		if constexpr(!REG_ORI) {
@@ -723,15 +721,23 @@ __device__ __forceinline__ void pass_cell(const DevState &d, uint64_t sweep, int
 		const int rot_idx = lane < 9 ? 0 : lane - 9;
 		const int rot_row = lane < 9 ? lane / 3 : (lane < 21 ? rot_idx % 3 : 0);
 		const int rot_col = lane % 3;
+		// the signs of patch (rot_idx / 3) as sign-bit masks of the high words
 		const unsigned rot_neg = lane < 21 ? (KF_NEG >> (3 * (rot_idx / 3))) & 7u : 0u;
+		const unsigned long long rs0 = (unsigned long long) (rot_neg & 1u) << 63;
+		const unsigned long long rs1 = (unsigned long long) ((rot_neg >> 1) & 1u) << 63;
+		const unsigned long long rs2 = (unsigned long long) ((rot_neg >> 2) & 1u) << 63;
 
 		for(int t = 0; t < ntrial; t++) {
-			// the moving member's rank from the ballot planes; then one LDS
-			// read round: the trial, the member's state and this lane's member
-			int k = 0;
-#pragma unroll
-			for(int b = 0; b < RANK_BITS; b++) k |= (int) ((kplane[b] >> (koff + t)) & 1ull) << b;
-			k = uni<T>(k);
+			// the moving member's rank, read from lane t of the team; then one
+			// LDS read round: the trial, the member's state and this lane's
+			// member
+			int k;
+			static_assert(T == 32 || T == 64, "teams of 32 or 64 lanes");
+			if constexpr(T == 64) k = __builtin_amdgcn_readlane(my_k, t);
+			else {
+				const int k0 = __builtin_amdgcn_readlane(my_k, t), k1 = __builtin_amdgcn_readlane(my_k, t + 32);
+				k = koff ? k1 : k0;
+			}
 			const double *tp = trials + t * TRIAL_W;
 			double tv[TRIAL_W];
 #pragma unroll
@@ -801,7 +807,11 @@ __device__ __forceinline__ void pass_cell(const DevState &d, uint64_t sweep, int
 				if(lane < 9) rv = rot_col == 0 ? r0 : (rot_col == 1 ? r1 : r2);
 				else {
 					const double m0 = r0 * KF_C, m1 = r1 * KF_C, m2 = r2 * KF_C;
-					rv = (rot_neg & 1u ? -m0 : m0) + (rot_neg & 2u ? -m1 : m1) + (rot_neg & 4u ? -m2 : m2);
+					// -x by its sign bit: (+-x) c = +-(x c) exactly
+					const double a0 = __longlong_as_double(__double_as_longlong(m0) ^ (long long) rs0);
+					const double a1 = __longlong_as_double(__double_as_longlong(m1) ^ (long long) rs1);
+					const double a2 = __longlong_as_double(__double_as_longlong(m2) ^ (long long) rs2);
+					rv = a0 + a1 + a2;
 				}
 				stage[lane] = rv;
 			}
@@ -886,9 +896,13 @@ __device__ __forceinline__ void pass_cell(const DevState &d, uint64_t sweep, int
 				// (O' and P' are contiguous in its record, lane l's at 3 + l)
 				double *wk = own + k * OWN_W;
 				if(lane < 21) wk[3 + lane] = rv;
-				else if(lane < 24) wk[lane - 21] = lane == 21 ? tr.nx : (lane == 22 ? tr.ny : tr.nz);
+				if(lane == 0) {
+					wk[0] = tr.nx;
+					wk[1] = tr.ny;
+					wk[2] = tr.nz;
+					if constexpr(KEEP) sbond[k] = s_new;
+				}
 				if constexpr(KEEP) {
-					if(lane == 0) sbond[k] = s_new;
 					// member k's bonds with the others are now the new ones
 					const unsigned long long nb_bits = (__ballot(bnm != 0) >> koff) & 0xffffffffull;
 					if(lane < cnt) mbits = lane == k ? (unsigned) nb_bits : ((mbits & ~(1u << k)) | ((unsigned) bnm << k));
