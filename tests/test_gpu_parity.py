@@ -136,16 +136,21 @@ def test_overlapping_start_is_refused(pp, oracle):
     assert e.value.status == 3
 
 
-def test_batched_replicas_match_single(pp, oracle):
+@pytest.mark.parametrize("trials_per_cell,tpc", [(0, 0), (16, 0), (16, 64)])
+def test_batched_replicas_match_single(pp, oracle, trials_per_cell, tpc):
     """Replica 0 of a batch follows the same trajectory as the same system
-    alone (Philox streams are keyed by replica index)."""
+    alone (Philox streams are keyed by replica index).  The lone system runs
+    the single-replica form of the colour pass (scalar replica constants),
+    the batch the general one; with 16 trials per cell both keep static and
+    member bonds across trials."""
     ra, Oa, ba = _start(oracle, 1000, 0.2, 1)
     rb, Ob, bb = _start(oracle, 700, 0.5, 2)
-    with pp.System(ra, Oa, ba, T=0.1, seed=5) as s:
+    kw = dict(seed=5, trials_per_cell=trials_per_cell, threads_per_cell=tpc)
+    with pp.System(ra, Oa, ba, T=0.1, **kw) as s:
         s.do_ensemble(20)
         single = s.state()
     prm = [dict(T=0.1), dict(T=0.15)]
-    with pp.System.batch([ra, rb], [Oa, Ob], [ba, bb], prm, seed=5) as s:
+    with pp.System.batch([ra, rb], [Oa, Ob], [ba, bb], prm, **kw) as s:
         s.do_ensemble(20)
         both = s.state(0)
         for k in range(2):
