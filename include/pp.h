@@ -92,7 +92,8 @@ typedef struct pp_params {
 /* Options of the GPU engine (no reference counterpart). */
 typedef struct pp_options {
 	uint64_t seed;           /* Philox key; the whole trajectory is a function of it */
-	int threads_per_cell;    /* lanes that work on one sweep cell: 32 or 64 (0: 32) */
+	int threads_per_cell;    /* lanes that work on one sweep cell: 16, 32 or 64 (0: 32); 16 needs
+	                            at most 16 particles per sweep cell and the per-pass launch form */
 	int block_size;          /* threads per workgroup: 64/128/256 (0: 256) */
 	double cell_occupancy;   /* target particles per sweep cell; 0: smallest cells (side >= 1+delta) */
 	int device;              /* HIP device ordinal */

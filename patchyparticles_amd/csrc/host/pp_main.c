@@ -14,7 +14,7 @@
  * configuration saved (main.c:16-24).
  *
  * Keys of this program only (optional): GPU_device (HIP ordinal, 0),
- * Threads_per_cell (32 or 64), Trials_per_cell (0: a sweep is N attempts;
+ * Threads_per_cell (16, 32 or 64), Trials_per_cell (0: a sweep is N attempts;
  * s > 0: s trials per non-empty sweep cell and colour pass), Checkpoint_file
  * (written with every last.rrr: the exact state and the random-number
  * counter), Resume (1: continue bit for bit from Checkpoint_file; the

@@ -250,10 +250,12 @@ pp_status create_impl(int R, const pp_params *prm, const pp_options *opts, const
 			o.block_size /= 2;
 	}
 	const int Tv = o.threads_per_cell;
-	if(Tv != 32 && Tv != 64) return fail(PP_ERR_ARG, "threads_per_cell must be 32 or 64");
+	if(Tv != 16 && Tv != 32 && Tv != 64) return fail(PP_ERR_ARG, "threads_per_cell must be 16, 32 or 64");
 	if(o.block_size % 64 || o.block_size < 64 || o.block_size > 256)
 		return fail(PP_ERR_ARG, "block_size must be 64, 128, 192 or 256");
 	if(o.launch < PP_LAUNCH_DEFAULT || o.launch > PP_LAUNCH_COOP) return fail(PP_ERR_ARG, "launch must be 0, 1 or 2");
+	if(o.launch == PP_LAUNCH_COOP && Tv == 16)
+		return fail(PP_ERR_ARG, "the cooperative launch form takes 32 or 64 threads per cell");
 	if(o.trials_per_cell < 0 || o.trials_per_cell > MAX_TRIALS_PER_CELL)
 		return fail(PP_ERR_ARG, "trials_per_cell must be in [0, %d]", MAX_TRIALS_PER_CELL);
 	if(pass_lds_bytes(Tv, o.block_size) > MAX_LDS_BYTES)
@@ -563,8 +565,8 @@ pp_status pp_autotune(pp_ctx *c, int sweeps, int *threads_per_cell, int *block_s
 	pp_status st = PP_OK;
 	// every candidate runs real sweeps: the trajectory does not depend on the
 	// launch shape, so nothing is wasted
-	const int Ts[2] = {32, 64}, blocks[3] = {64, 128, 256};
-	for(int ti = 0; ti < 2 && st == PP_OK; ti++)
+	const int Ts[3] = {16, 32, 64}, blocks[3] = {64, 128, 256};
+	for(int ti = 0; ti < 3 && st == PP_OK; ti++)
 		for(int bi = 0; bi < 3 && st == PP_OK; bi++) {
 			if(pass_lds_bytes(Ts[ti], blocks[bi]) > MAX_LDS_BYTES || blocks[bi] < Ts[ti]) continue;
 			c->T = Ts[ti];

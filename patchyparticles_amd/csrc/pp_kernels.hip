@@ -451,7 +451,7 @@ constexpr int LDS_FIXED = LDS_OWN + LDS_PID + LDS_SB + LDS_TRIAL + LDS_NB + LDS_
 // + the orientations of the T register-resident neighbours, component-major
 template <int T>
 constexpr int team_lds() {
-	return LDS_FIXED + 9 * T;
+	return LDS_FIXED + 9 * (T < 32 ? 32 : T);
 }
 static_assert((LDS_STAGE + LDS_TRIAL + LDS_OWN + LDS_PID + LDS_SB) % 2 == 0 && LDS_FIXED % 2 == 0 &&
                   (LDS_STAGE + LDS_TRIAL) % 2 == 0 && TRIAL_W % 2 == 0 && OWN_W % 2 == 0,
@@ -497,6 +497,44 @@ __device__ __forceinline__ void bonds_old_new(const Trial &t, const double *Po, 
 // make sequential trials.  Each lane keeps up to CPL of the cell's static
 // neighbours (the 26 surrounding cells, fixed during the pass) in registers;
 // a denser neighbourhood is handled T at a time from HBM.
+// One of the 21 values of a trial's O' = O R (v < 9: O'[v]) and its patches
+// (v = 9 + 3a + kk: P'[a][kk]): the row of O it needs, the column, and for a
+// patch component the signs of patch a as sign-bit masks of the high word.
+struct RotShare {
+	int row, col;
+	bool patch;
+	unsigned long long s0, s1, s2;
+};
+
+__device__ __forceinline__ RotShare rot_share(int v) {
+	RotShare r;
+	const int idx = v < 9 ? 0 : v - 9;
+	r.patch = v >= 9;
+	r.row = v < 9 ? v / 3 : (v < 21 ? idx % 3 : 0);
+	r.col = v % 3;
+	const unsigned neg = (v >= 9 && v < 21) ? (KF_NEG >> (3 * (idx / 3))) & 7u : 0u;
+	r.s0 = (unsigned long long) (neg & 1u) << 63;
+	r.s1 = (unsigned long long) ((neg >> 1) & 1u) << 63;
+	r.s2 = (unsigned long long) ((neg >> 2) & 1u) << 63;
+	return r;
+}
+
+// The value: row `row` of O' from that row of O (o0, o1, o2) and R (the
+// trial's tv[4..12]) in the reference's order (utils.c:172-184), then either
+// its column or, with b_a[j] = +-c and (+-x) c = +-(x c) exactly, the signed
+// sum of the row times c (utils.h:17-21).
+__device__ __forceinline__ double rot_value(const RotShare &r, double o0, double o1, double o2, const double *tv) {
+	const double r0 = o0 * tv[4] + o1 * tv[4 + 3] + o2 * tv[4 + 6];
+	const double r1 = o0 * tv[5] + o1 * tv[5 + 3] + o2 * tv[5 + 6];
+	const double r2 = o0 * tv[6] + o1 * tv[6 + 3] + o2 * tv[6 + 6];
+	if(!r.patch) return r.col == 0 ? r0 : (r.col == 1 ? r1 : r2);
+	const double m0 = r0 * KF_C, m1 = r1 * KF_C, m2 = r2 * KF_C;
+	const double a0 = __longlong_as_double(__double_as_longlong(m0) ^ (long long) r.s0);
+	const double a1 = __longlong_as_double(__double_as_longlong(m1) ^ (long long) r.s1);
+	const double a2 = __longlong_as_double(__double_as_longlong(m2) ^ (long long) r.s2);
+	return a0 + a1 + a2;
+}
+
 // Trials per cell from which the kept form pays: with fewer, the up-front
 // counts cost about what the trials save.
 constexpr int KEEP_MIN_TRIALS = 4;
@@ -515,10 +553,10 @@ __device__ __forceinline__ void pass_cell(const DevState &d, uint64_t sweep, int
 	int *own_pid = (int *) (own + LDS_OWN);
 	int *sbond = (int *) (own + LDS_OWN + LDS_PID);  // member m's bonds with the static neighbours
 	int *nb_tab = (int *) (own + LDS_OWN + LDS_PID + LDS_SB);  // the cell's record (build_record)
-	// the orientation of this lane's first neighbour: in registers where
-	// they allow (scalar replica constants), else in LDS, component c at
-	// c * T + lane
-	constexpr bool REG_ORI = ONE || T == 64;
+	// the orientations of this lane's register neighbours: in registers where
+	// they allow (scalar replica constants), else in LDS, component c of
+	// neighbour u at (9 u + c) * T + lane
+	constexpr bool REG_ORI = (ONE || T == 64) && CPL == 1;
 	double *ori = blk + LDS_FIXED;
 	const int S = d.stride;
 	{
@@ -532,7 +570,8 @@ __device__ __forceinline__ void pass_cell(const DevState &d, uint64_t sweep, int
 		const int cloc = uni<T>(nb_tab[61]);
 		const int r = ONE ? 0 : uni<T>(nb_tab[62]);
 		const int sel = uni<T>(nb_tab[63]);
-		if(cnt > CELL_CAP) {
+		// (a team of T < CELL_CAP lanes holds one member per lane)
+		if(cnt > CELL_CAP || cnt > T) {
 			if(lane == 0) atomicOr(d.err, (int) ERR_CELL_CAP);
 			return;
 		}
@@ -579,9 +618,11 @@ __device__ __forceinline__ void pass_cell(const DevState &d, uint64_t sweep, int
 				czr[u] = F[2 * S + q];
 			}
 		}
-		double orv[9];
+		double orv[CPL][9];
 #pragma unroll
-		for(int c = 0; c < 9; c++) orv[c] = cqr[0] >= 0 ? F[(3 + c) * S + cqr[0]] : 0.;
+		for(int u = 0; u < CPL; u++)
+#pragma unroll
+			for(int c = 0; c < 9; c++) orv[u][c] = cqr[u] >= 0 ? F[(3 + c) * S + cqr[u]] : 0.;
 		Trial tr;
 		tr.Lx = unid<T>(rp.g.box[0]);
 		tr.Ly = unid<T>(rp.g.box[1]);
@@ -621,7 +662,9 @@ __device__ __forceinline__ void pass_cell(const DevState &d, uint64_t sweep, int
 		if(lane < cnt) own_pid[lane] = my_pid;
 		if constexpr(!REG_ORI) {
 #pragma unroll
-			for(int c = 0; c < 9; c++) ori[c * T + lane] = orv[c];
+			for(int u = 0; u < CPL; u++)
+#pragma unroll
+				for(int c = 0; c < 9; c++) ori[(9 * u + c) * T + lane] = orv[u][c];
 		}
 		team_sync();
 		// 4. members into LDS ranked by particle index, with their patch vectors
@@ -644,7 +687,7 @@ __device__ __forceinline__ void pass_cell(const DevState &d, uint64_t sweep, int
 		// rest from HBM
 		auto reg_o = [&](int u, double *Oq) {
 #pragma unroll
-			for(int c = 0; c < 9; c++) Oq[c] = u == 0 ? (REG_ORI ? orv[c] : ori[c * T + lane]) : F[(3 + c) * S + cqr[u]];
+			for(int c = 0; c < 9; c++) Oq[c] = REG_ORI ? orv[u][c] : ori[(9 * u + c) * T + lane];
 		};
 		auto each_far = [&](auto &&f) {
 			for(int b = T * CPL; b < M; b += T)
@@ -716,27 +759,26 @@ __device__ __forceinline__ void pass_cell(const DevState &d, uint64_t sweep, int
 			team_sync();
 		}
 		int c_acc = 0, d_bonds = 0;
-		// this lane's share of a trial's O' and patches (see the trial loop)
-		static_assert(T >= 24, "a team forms O' and the patches one value per lane");
-		const int rot_idx = lane < 9 ? 0 : lane - 9;
-		const int rot_row = lane < 9 ? lane / 3 : (lane < 21 ? rot_idx % 3 : 0);
-		const int rot_col = lane % 3;
-		// the signs of patch (rot_idx / 3) as sign-bit masks of the high words
-		const unsigned rot_neg = lane < 21 ? (KF_NEG >> (3 * (rot_idx / 3))) & 7u : 0u;
-		const unsigned long long rs0 = (unsigned long long) (rot_neg & 1u) << 63;
-		const unsigned long long rs1 = (unsigned long long) ((rot_neg >> 1) & 1u) << 63;
-		const unsigned long long rs2 = (unsigned long long) ((rot_neg >> 2) & 1u) << 63;
+		// this lane's share of a trial's O' and patches (see the trial loop):
+		// value lane, and with T < 21 also value lane + T
+		static_assert(T >= 16 && CELL_CAP <= 32, "a team of 16 lanes or more");
+		const RotShare rsh = rot_share(lane), rsh2 = rot_share(lane + T);
 
 		for(int t = 0; t < ntrial; t++) {
 			// the moving member's rank, read from lane t of the team; then one
 			// LDS read round: the trial, the member's state and this lane's
 			// member
 			int k;
-			static_assert(T == 32 || T == 64, "teams of 32 or 64 lanes");
+			static_assert(T == 16 || T == 32 || T == 64, "teams of 16, 32 or 64 lanes");
 			if constexpr(T == 64) k = __builtin_amdgcn_readlane(my_k, t);
-			else {
+			else if constexpr(T == 32) {
 				const int k0 = __builtin_amdgcn_readlane(my_k, t), k1 = __builtin_amdgcn_readlane(my_k, t + 32);
 				k = koff ? k1 : k0;
+			}
+			else {
+				const int k0 = __builtin_amdgcn_readlane(my_k, t), k1 = __builtin_amdgcn_readlane(my_k, t + 16);
+				const int k2 = __builtin_amdgcn_readlane(my_k, t + 32), k3 = __builtin_amdgcn_readlane(my_k, t + 48);
+				k = koff < 32 ? (koff ? k1 : k0) : (koff == 32 ? k2 : k3);
 			}
 			const double *tp = trials + t * TRIAL_W;
 			double tv[TRIAL_W];
@@ -751,9 +793,16 @@ __device__ __forceinline__ void pass_cell(const DevState &d, uint64_t sweep, int
 				mz = wl[2];
 			}
 			const double *me = own + k * OWN_W;
-			// row rot_row of the member's O (the lane's share of O' below)
-			const double *orow = me + 3 + 3 * rot_row;
+			// the rows of the member's O that the lane's share of O' needs
+			const double *orow = me + 3 + 3 * rsh.row;
 			const double o0 = orow[0], o1 = orow[1], o2 = orow[2];
+			double o3 = 0., o4 = 0., o5 = 0.;
+			if constexpr(T < 21) {
+				const double *orow2 = me + 3 + 3 * rsh2.row;
+				o3 = orow2[0];
+				o4 = orow2[1];
+				o5 = orow2[2];
+			}
 			double Po[12];  // the old patches (two-sided tests only)
 			if constexpr(!KEEP)
 #pragma unroll
@@ -793,28 +842,20 @@ __device__ __forceinline__ void pass_cell(const DevState &d, uint64_t sweep, int
 			if(team_any<T>(ov)) continue;
 
 			// O' = O R in the body frame (utils.c:172-184, 204-209) and its
-			// patches (utils.h:17-21), one value per lane in the reference's
-			// order: lanes 0-8 O'[l], lanes 9-20 the patch component
-			// P'[a][kk] = sum_j O'[kk][j] b_a[j], l = 9 + 3a + kk.  Each of
-			// them forms row rot_row of O'; with b_a[j] = +-c, (+-x) c = +-(x c)
-			// exactly, so a patch component is the signed sum of that row
-			// times c.  The values meet in the stage row.
-			double rv = 0.;
+			// patches (utils.h:17-21), one value per lane (two on some lanes of
+			// a 16-lane team) in the reference's order: value v < 9 is O'[v],
+			// v = 9 + 3a + kk the patch component P'[a][kk] (rot_value).  The
+			// values meet in the stage row.
+			double rv = 0., rv2 = 0.;
 			if(lane < 21) {
-				const double r0 = o0 * tv[4] + o1 * tv[4 + 3] + o2 * tv[4 + 6];
-				const double r1 = o0 * tv[5] + o1 * tv[5 + 3] + o2 * tv[5 + 6];
-				const double r2 = o0 * tv[6] + o1 * tv[6 + 3] + o2 * tv[6 + 6];
-				if(lane < 9) rv = rot_col == 0 ? r0 : (rot_col == 1 ? r1 : r2);
-				else {
-					const double m0 = r0 * KF_C, m1 = r1 * KF_C, m2 = r2 * KF_C;
-					// -x by its sign bit: (+-x) c = +-(x c) exactly
-					const double a0 = __longlong_as_double(__double_as_longlong(m0) ^ (long long) rs0);
-					const double a1 = __longlong_as_double(__double_as_longlong(m1) ^ (long long) rs1);
-					const double a2 = __longlong_as_double(__double_as_longlong(m2) ^ (long long) rs2);
-					rv = a0 + a1 + a2;
-				}
+				rv = rot_value(rsh, o0, o1, o2, tv);
 				stage[lane] = rv;
 			}
+			if constexpr(T < 21)
+				if(lane + T < 21) {
+					rv2 = rot_value(rsh2, o3, o4, o5, tv);
+					stage[lane + T] = rv2;
+				}
 			team_sync();
 			double Pn[12];
 #pragma unroll
@@ -896,6 +937,8 @@ __device__ __forceinline__ void pass_cell(const DevState &d, uint64_t sweep, int
 				// (O' and P' are contiguous in its record, lane l's at 3 + l)
 				double *wk = own + k * OWN_W;
 				if(lane < 21) wk[3 + lane] = rv;
+				if constexpr(T < 21)
+					if(lane + T < 21) wk[3 + lane + T] = rv2;
 				if(lane == 0) {
 					wk[0] = tr.nx;
 					wk[1] = tr.ny;
@@ -904,7 +947,8 @@ __device__ __forceinline__ void pass_cell(const DevState &d, uint64_t sweep, int
 				}
 				if constexpr(KEEP) {
 					// member k's bonds with the others are now the new ones
-					const unsigned long long nb_bits = (__ballot(bnm != 0) >> koff) & 0xffffffffull;
+					const unsigned long long nb_bits =
+					    (__ballot(bnm != 0) >> koff) & (T >= 32 ? 0xffffffffull : ((1ull << T) - 1ull));
 					if(lane < cnt) mbits = lane == k ? (unsigned) nb_bits : ((mbits & ~(1u << k)) | ((unsigned) bnm << k));
 				}
 			}
@@ -1014,7 +1058,8 @@ static void pass_T(const DevState &d, uint64_t sweep, int pass, int block, hipSt
 }
 
 size_t pass_lds_bytes(int T, int block) {
-	return (size_t) (block / T) * (T == 32 ? team_lds<32>() : team_lds<64>()) * sizeof(double);
+	return (size_t) (block / T) * (T == 16 ? team_lds<16>() : (T == 32 ? team_lds<32>() : team_lds<64>())) *
+	       sizeof(double);
 }
 
 // The form is chosen per context: 32 or 64 lanes per cell, kept static
@@ -1022,13 +1067,18 @@ size_t pass_lds_bytes(int T, int block) {
 // for every team) or several.
 template <int T, bool KEEP>
 static void pass_TK(const DevState &d, uint64_t sweep, int pass, int block, hipStream_t s) {
-	if(d.R == 1) pass_T<T, 1, KEEP, true>(d, sweep, pass, block, s);
-	else pass_T<T, 1, KEEP, false>(d, sweep, pass, block, s);
+	constexpr int CPL = T < 32 ? 32 / T : 1;  // register neighbours per lane
+	if(d.R == 1) pass_T<T, CPL, KEEP, true>(d, sweep, pass, block, s);
+	else pass_T<T, CPL, KEEP, false>(d, sweep, pass, block, s);
 }
 
 void launch_pass(const DevState &d, uint64_t sweep, int pass, int T, int block, hipStream_t s) {
 	const bool keep = d.trials_per_cell >= KEEP_MIN_TRIALS;
-	if(T == 32) {
+	if(T == 16) {
+		if(keep) pass_TK<16, true>(d, sweep, pass, block, s);
+		else pass_TK<16, false>(d, sweep, pass, block, s);
+	}
+	else if(T == 32) {
 		if(keep) pass_TK<32, true>(d, sweep, pass, block, s);
 		else pass_TK<32, false>(d, sweep, pass, block, s);
 	}

@@ -93,19 +93,23 @@ def test_check_energy_gram_schmidt_bit_exact(pp, oracle):
     assert np.array_equal(O2, want)
 
 
-@pytest.mark.parametrize("tpc,block,launch", [(32, 64, 1), (32, 128, 2), (32, 64, 2), (64, 64, 2), (64, 128, 1),
-                                              (64, 256, 2)])
-def test_trajectory_independent_of_launch_shape(pp, oracle, tpc, block, launch):
-    """Teams per cell, workgroup size and launch form (one launch per colour
-    pass, or one cooperative launch per sweep with grid barriers) leave the
-    trajectory unchanged bit for bit."""
+@pytest.mark.parametrize("tpc,block,launch,trials", [(32, 64, 1, 0), (32, 128, 2, 0), (32, 64, 2, 0), (64, 64, 2, 0),
+                                                     (64, 128, 1, 0), (64, 256, 2, 0), (16, 64, 1, 0), (16, 64, 1, 16),
+                                                     (32, 128, 1, 16)])
+def test_trajectory_independent_of_launch_shape(pp, oracle, tpc, block, launch, trials):
+    """Lanes per cell (16, 32 or 64: 4, 2 or 1 cells per wavefront),
+    workgroup size and launch form (one launch per colour pass, or one
+    cooperative launch per sweep with grid barriers) leave the trajectory
+    unchanged bit for bit, with one trial per member and with 16 trials per
+    cell (kept bonds)."""
     r, O, box = _start(oracle, 3000, 0.4, 9)
     ref = None
-    with pp.System(r, O, box, T=0.11, seed=99, threads_per_cell=64, block_size=256, launch=1) as s:
+    kw = dict(T=0.11, seed=99, trials_per_cell=trials)
+    with pp.System(r, O, box, threads_per_cell=64, block_size=256, launch=1, **kw) as s:
         s.do_ensemble(25)
         ref = s.state()
         cref = s.counters()
-    with pp.System(r, O, box, T=0.11, seed=99, threads_per_cell=tpc, block_size=block, launch=launch) as s:
+    with pp.System(r, O, box, threads_per_cell=tpc, block_size=block, launch=launch, **kw) as s:
         s.do_ensemble(25)
         got = s.state()
         c = s.counters()
@@ -200,11 +204,11 @@ def _fcc(cells, a, seed):
     return r, q, np.full(3, cells * a)
 
 
-@pytest.mark.parametrize("tpc", [32, 64])
+@pytest.mark.parametrize("tpc", [16, 32, 64])
 def test_dense_neighbourhood_beyond_registers(pp, oracle, tpc):
     """FCC at rho=0.89 in a box of 4^3 sweep cells of side 1.65: the 26 cells
     around a sweep cell hold ~104 particles, more than one team keeps in
-    registers (64), so the colour pass reads the rest from memory.  Overlaps
+    registers (32 or 64), so the colour pass reads the rest from memory.  Overlaps
     and bonds must still be seen: the run stays overlap-free, the tracked
     energy equals a full recompute, and the bonds equal the oracle's."""
     r, O, box = _fcc(4, 1.65, 4)
@@ -218,7 +222,7 @@ def test_dense_neighbourhood_beyond_registers(pp, oracle, tpc):
     assert not anyo and not oov.any()
     assert np.array_equal(b, ob) and c["bonds"] == tot == otot
     assert c["accepted"][0] > 0.2 * c["tries"][0]
-    with pp.System(r, O, box, T=0.12, seed=21, threads_per_cell=96 - tpc) as s:
+    with pp.System(r, O, box, T=0.12, seed=21, threads_per_cell=64 if tpc == 32 else 32) as s:
         s.do_ensemble(40)
         other = s.state()
     for x, y in zip((rr, OO, bb), other):
