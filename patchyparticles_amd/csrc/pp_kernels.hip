@@ -748,16 +748,43 @@ __device__ __forceinline__ void pass_cell(const DevState &d, uint64_t sweep, int
 				for(int j = 0; j < cnt; j++) {
 					if(j == lane) continue;
 					const double *wj = own + j * OWN_W;
-					int b = 0;
-					bonds_new_exact(tm, Pm, wj[0], wj[1], wj[2], [&](double *Oq) {
+					auto load_j = [&](double *Oq) {
 #pragma unroll
 						for(int c = 0; c < 9; c++) Oq[c] = wj[3 + c];
-					}, b);
+					};
+					int b = 0;
+					bool band = tr.c2 < 0.;
+					if(!band) bonds_new(tm, Pm, wj[0], wj[1], wj[2], load_j, b, band);
+					if(band) {
+						b = 0;
+						bonds_new_exact(tm, Pm, wj[0], wj[1], wj[2], load_j, b);
+					}
 					mbits |= (unsigned) b << j;
 				}
 			}
 			team_sync();
+			// The members' fractional coordinates inside the cell, u = (cell
+			// coordinate as cell_coord forms it) - (ix, iy, iz), kept in the
+			// record's patch slots (the kept form reads no old patches): a
+			// trial whose u + displacement stays 1e-9 cells clear of the
+			// faces cannot leave the cell (cell_coord's roundings are below
+			// 1e-13 cells), so it skips the exact test.
+			if(lane < cnt) {
+				double *wm = own + lane * OWN_W;
+				const double p3[3] = {wm[0], wm[1], wm[2]};
+				const double sh[3] = {sh0, sh1, sh2}, il[3] = {tr.iLx, tr.iLy, tr.iLz};
+				const int nn[3] = {n0, n1, n2}, ii[3] = {ix, iy, iz};
+#pragma unroll
+				for(int a = 0; a < 3; a++) {
+					double t = (p3[a] - sh[a]) * il[a];
+					t = t - __builtin_floor(t);
+					wm[NCOMP + a] = t * (double) nn[a] - (double) ii[a];
+				}
+			}
+			team_sync();
 		}
+		// cells per unit length along each axis (the u step of a displacement)
+		const double gnx = (double) n0 * tr.iLx, gny = (double) n1 * tr.iLy, gnz = (double) n2 * tr.iLz;
 		int c_acc = 0, d_bonds = 0;
 		// this lane's share of a trial's O' and patches (see the trial loop):
 		// value lane, and with T < 21 also value lane + T
@@ -793,6 +820,12 @@ __device__ __forceinline__ void pass_cell(const DevState &d, uint64_t sweep, int
 				mz = wl[2];
 			}
 			const double *me = own + k * OWN_W;
+			double ux = 0., uy = 0., uz = 0.;  // (kept form) the member's u
+			if constexpr(KEEP) {
+				ux = me[NCOMP];
+				uy = me[NCOMP + 1];
+				uz = me[NCOMP + 2];
+			}
 			// the rows of the member's O that the lane's share of O' needs
 			const double *orow = me + 3 + 3 * rsh.row;
 			const double o0 = orow[0], o1 = orow[1], o2 = orow[2];
@@ -814,10 +847,24 @@ __device__ __forceinline__ void pass_cell(const DevState &d, uint64_t sweep, int
 			tr.ny = unid<T>(tr.py + tv[2]);
 			tr.nz = unid<T>(tr.pz + tv[3]);
 			// a trial that leaves the cell is rejected
-			const bool out = (uni<T>(cell_coord(tr.nx, sh0, tr.iLx, n0)) != ix) |
-			                 (uni<T>(cell_coord(tr.ny, sh1, tr.iLy, n1)) != iy) |
-			                 (uni<T>(cell_coord(tr.nz, sh2, tr.iLz, n2)) != iz);
-			if(out) continue;
+			// (kept form: skipped when u + displacement is clear of the faces)
+			double cu0 = 0., cu1 = 0., cu2 = 0.;
+			bool deep = false;
+			if constexpr(KEEP) {
+				cu0 = ux + tv[1] * gnx;
+				cu1 = uy + tv[2] * gny;
+				cu2 = uz + tv[3] * gnz;
+				constexpr double CELL_EPS = 1e-9;
+				deep = __builtin_fmax(__builtin_fabs(cu0 - 0.5), __builtin_fmax(__builtin_fabs(cu1 - 0.5),
+				                                                               __builtin_fabs(cu2 - 0.5))) <
+				       0.5 - CELL_EPS;
+			}
+			if(!deep) {
+				const bool out = (uni<T>(cell_coord(tr.nx, sh0, tr.iLx, n0)) != ix) |
+				                 (uni<T>(cell_coord(tr.ny, sh1, tr.iLy, n1)) != iy) |
+				                 (uni<T>(cell_coord(tr.nz, sh2, tr.iLz, n2)) != iz);
+				if(out) continue;
+			}
 
 			// phase 1: hard cores at the new position, wavefront-ballot early exit
 			// (the register neighbours' separations are kept for phase 2)
@@ -936,13 +983,20 @@ __device__ __forceinline__ void pass_cell(const DevState &d, uint64_t sweep, int
 				// member k takes the new position, orientation and patches
 				// (O' and P' are contiguous in its record, lane l's at 3 + l)
 				double *wk = own + k * OWN_W;
-				if(lane < 21) wk[3 + lane] = rv;
-				if constexpr(T < 21)
-					if(lane + T < 21) wk[3 + lane + T] = rv2;
+				// (the kept form keeps u in the patch slots instead of P')
+				constexpr int NW = KEEP ? 9 : 21;
+				if(lane < NW) wk[3 + lane] = rv;
+				if constexpr(T < NW)
+					if(lane + T < NW) wk[3 + lane + T] = rv2;
 				if(lane == 0) {
 					wk[0] = tr.nx;
 					wk[1] = tr.ny;
 					wk[2] = tr.nz;
+					if constexpr(KEEP) {
+						wk[NCOMP] = cu0;
+						wk[NCOMP + 1] = cu1;
+						wk[NCOMP + 2] = cu2;
+					}
 					if constexpr(KEEP) sbond[k] = s_new;
 				}
 				if constexpr(KEEP) {
