@@ -261,8 +261,10 @@ struct u32x4 {
 __device__ __forceinline__ u32x4 philox(u32x4 c, uint32_t k0, uint32_t k1) {
 #pragma unroll
 	for(int i = 0; i < 10; i++) {
-		uint32_t hi0 = __umulhi(0xD2511F53u, c.x), lo0 = 0xD2511F53u * c.x;
-		uint32_t hi1 = __umulhi(0xCD9E8D57u, c.z), lo1 = 0xCD9E8D57u * c.z;
+		// each 32 x 32 -> 64 product in one v_mad_u64_u32
+		const uint64_t p0 = (uint64_t) 0xD2511F53u * c.x, p1 = (uint64_t) 0xCD9E8D57u * c.z;
+		const uint32_t hi0 = (uint32_t) (p0 >> 32), lo0 = (uint32_t) p0;
+		const uint32_t hi1 = (uint32_t) (p1 >> 32), lo1 = (uint32_t) p1;
 		c = u32x4{hi1 ^ c.y ^ k0, lo1, hi0 ^ c.w ^ k1, lo0};
 		k0 += 0x9E3779B9u;
 		k1 += 0xBB67AE85u;
