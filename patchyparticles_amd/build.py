@@ -78,9 +78,23 @@ def build_host():
     return out
 
 
+def build_checks():
+    """GPU checkers run by the tests: kf_filter_check (the colour pass's
+    filtered pair test against the reference arithmetic at the thresholds)."""
+    src = os.path.join(ROOT, "tests", "gpu_checks", "kf_filter_check.hip")
+    if not os.path.exists(src):
+        return None
+    os.makedirs(BIN, exist_ok=True)
+    out = os.path.join(BIN, "kf_filter_check")
+    flags = [f for f in HIPCC_FLAGS if f != "-fPIC"]
+    _run([hipcc(), *flags, "-I", CSRC, "-I", os.path.join(ROOT, "include"), src, "-o", out])
+    return out
+
+
 def build_all():
     build_lib()
     build_host()
+    build_checks()
 
 
 if __name__ == "__main__":
