@@ -1210,6 +1210,7 @@ __global__ __launch_bounds__(256) void k_energy(DevState d, int trial, int per_p
 	const int iz = cell_coord(pz, G.shift[2], G.inv_box[2], n2);
 	const double Lx = G.box[0], Ly = G.box[1], Lz = G.box[2];
 	const double iLx = G.inv_box[0], iLy = G.inv_box[1], iLz = G.inv_box[2];
+	const double c2 = rp.cosmax >= KF_FILTER_MIN_COS ? rp.cosmax * rp.cosmax : -1.;
 	int nbond = 0;
 	bool ov = false;
 #pragma unroll
@@ -1227,12 +1228,31 @@ __global__ __launch_bounds__(256) void k_energy(DevState d, int trial, int per_p
 			double d2 = dx * dx + dy * dy + dz * dz;
 			if(d2 < 1.) ov = true;
 			else if(d2 < rp.rc2) {
-				kf_unit(dx, dy, dz, d2);
-				if(kf_any_p(dx, dy, dz, &P[0][0], rp.cosmax)) {
-					double Oq[9];
+				// the filtered angular test (kf_fast_p / kf_fast_q1); the
+				// reference's arithmetic only inside the band or without the
+				// filter
+				bool unsure = c2 < 0.;
+				if(!unsure) {
+					const KfBand bd = kf_band(c2, d2);
+					const unsigned fp = kf_fast_p(dx, dy, dz, &P[0][0], bd);
+					unsure = (fp >> 1) != 0u;
+					if(fp & 1u) {
+						double Oq[9];
 #pragma unroll
-					for(int c = 0; c < 9; c++) Oq[c] = F[(3 + c) * S + q];
-					if(kf_any_q(dx, dy, dz, Oq, rp.cosmax)) nbond++;
+						for(int c = 0; c < 9; c++) Oq[c] = F[(3 + c) * S + q];
+						const unsigned fq = kf_fast_q1(dx, dy, dz, kf_band_q(bd), Oq);
+						unsure |= (fq >> 1) != 0u;
+						if(!unsure) nbond += (int) (fq & 1u);
+					}
+				}
+				if(unsure) {
+					kf_unit(dx, dy, dz, d2);
+					if(kf_any_p(dx, dy, dz, &P[0][0], rp.cosmax)) {
+						double Oq[9];
+#pragma unroll
+						for(int c = 0; c < 9; c++) Oq[c] = F[(3 + c) * S + q];
+						if(kf_any_q(dx, dy, dz, Oq, rp.cosmax)) nbond++;
+					}
 				}
 			}
 		}
