@@ -50,6 +50,25 @@ __device__ __forceinline__ int team_sum(int v) {
 	return v;
 }
 
+// Inclusive prefix sum over a whole wavefront (every lane active):
+// Hillis-Steele steps within each 16-lane row (row_shr 1, 2, 4, 8, the
+// lanes shifted in from outside the row reading 0), then row 0's total into
+// row 1 and row 2's into row 3 (row_bcast:15), then the first half's total
+// into rows 2 and 3 (row_bcast:31).
+constexpr int DPP_ROW_SHR = 0x110;     // + k: lane i <- i - k within a row
+constexpr int DPP_ROW_BCAST15 = 0x142;
+constexpr int DPP_ROW_BCAST31 = 0x143;
+
+__device__ __forceinline__ int wave_incl_scan(int v) {
+	v += __builtin_amdgcn_update_dpp(0, v, DPP_ROW_SHR + 1, 0xf, 0xf, true);
+	v += __builtin_amdgcn_update_dpp(0, v, DPP_ROW_SHR + 2, 0xf, 0xf, true);
+	v += __builtin_amdgcn_update_dpp(0, v, DPP_ROW_SHR + 4, 0xf, 0xf, true);
+	v += __builtin_amdgcn_update_dpp(0, v, DPP_ROW_SHR + 8, 0xf, 0xf, true);
+	v += __builtin_amdgcn_update_dpp(0, v, DPP_ROW_BCAST15, 0xa, 0xf, false);
+	v += __builtin_amdgcn_update_dpp(0, v, DPP_ROW_BCAST31, 0xc, 0xf, false);
+	return v;
+}
+
 // Sum over a team of small non-negative integers (< 8): one ballot per bit.
 template <int T>
 __device__ __forceinline__ int team_sum_small(int v) {

@@ -138,8 +138,7 @@ __global__ __launch_bounds__(256) void k_tile_sums(DevState d) {
 #pragma unroll
 	for(int e = 0; e < 8; e++)
 		if(base + e < d.C_total) tot += d.count[base + e];
-#pragma unroll
-	for(int o = 32; o > 0; o >>= 1) tot += __shfl_xor(tot, o, 64);
+	tot = team_sum<64>(tot);
 	if((t & 63) == 0) part[t >> 6] = tot;
 	__syncthreads();
 	if(t == 0) d.tile_sum[blockIdx.x] = part[0] + part[1] + part[2] + part[3];
@@ -199,12 +198,7 @@ __global__ __launch_bounds__(256) void k_scan(DevState d, int flip, int lists, i
 		tot += v[e];
 	}
 	int lane = t & 63, wid = t >> 6;
-	int inc = tot;
-#pragma unroll
-	for(int o = 1; o < 64; o <<= 1) {
-		int u = __shfl_up(inc, o, 64);
-		if(lane >= o) inc += u;
-	}
+	const int inc = wave_incl_scan(tot);
 	if(lane == 63) wsum[wid] = inc;
 	__syncthreads();
 	if(pre) {
@@ -237,9 +231,7 @@ __global__ __launch_bounds__(256) void k_scan(DevState d, int flip, int lists, i
 				continue;
 			}
 			int val = lane <= l2 ? (int) (st & 0xffffffffull) : 0;
-#pragma unroll
-			for(int o = 32; o > 0; o >>= 1) val += __shfl_xor(val, o, 64);
-			excl += val;
+			excl += team_sum<64>(val);
 			if(l2 < 64) break;
 			p -= 64;
 		}
