@@ -428,10 +428,11 @@ struct PassArgs {
 	int pass;
 };
 
-// Per-team LDS (doubles): the cell's members ranked by particle index
-// (x, y, z, O, and their 4 patch vectors), their particle indices, the
-// cell's trials, the cell's record (surrounding slot ranges) and the
-// replica's acceptance table.
+// Per-team LDS (doubles): the stage row of a trial's O' and patches, the
+// cell's trials, its members ranked by particle index (x, y, z, O and the 4
+// patch vectors; the kept form holds the member's fractional cell
+// coordinates in the first three patch slots), their particle indices, their
+// kept static bond counts and the cell's record (surrounding slot ranges).
 constexpr int OWN_W = NCOMP + 12;   // x, y, z, O[9], P[12]
 constexpr int LDS_OWN = CELL_CAP * OWN_W;
 constexpr int LDS_PID = (CELL_CAP + 1) / 2;  // CELL_CAP ints
@@ -448,7 +449,8 @@ constexpr int team_lds() {
 static_assert((LDS_STAGE + LDS_TRIAL + LDS_OWN + LDS_PID + LDS_SB) % 2 == 0 && LDS_FIXED % 2 == 0 &&
                   (LDS_STAGE + LDS_TRIAL) % 2 == 0 && TRIAL_W % 2 == 0 && OWN_W % 2 == 0,
               "16-byte aligned trials, members, record and team blocks");
-// two 256-lane workgroups of 32-lane teams share a CU's 160 KiB
+// 16 teams of 32 lanes (four 128-lane workgroups, the default shape) share a
+// CU's 160 KiB
 static_assert(2 * 8 * team_lds<32>() * sizeof(double) <= 160 * 1024, "two pass workgroups per CU");
 
 // The same decisions through the filtered tests (no square root, no
