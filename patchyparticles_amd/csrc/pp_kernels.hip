@@ -81,7 +81,6 @@ __global__ void k_count(DevState d, int mode, uint64_t sweep, long long trace_po
 	for(int t = j; t < d.R * 8; t += gridDim.x * blockDim.x) d.ccount[t] = 0;
 	if(j == 0) {
 		d.scan_ticket[0] = 0;
-		d.scan_ticket[1] = 0;
 	}
 	if(j >= d.Ntot) return;
 	const int r = d.prep[j];
@@ -171,7 +170,7 @@ __global__ __launch_bounds__(1024) void k_tile_scan(DevState d) {
 // k_tile_scan); pre = 0: a decoupled look-back finds them.
 __global__ __launch_bounds__(256) void k_scan(DevState d, int flip, int lists, int pre) {
 	__shared__ int wsum[4];
-	__shared__ int sh_tile, sh_excl, sh_r0, sh_last;
+	__shared__ int sh_tile, sh_excl, sh_r0;
 	__shared__ int lcnt[SCAN_KEYS], lbase[SCAN_KEYS];
 	const int t = threadIdx.x;
 	if(t == 0) sh_tile = atomicAdd(d.scan_ticket, 1);
@@ -298,31 +297,27 @@ __global__ __launch_bounds__(256) void k_scan(DevState d, int flip, int lists, i
 				const int cloc = base + e - rp.cell_off;
 				d.clist[rp.cell_off + (lkey[e] & 7) * rp.act_cap + lbase[lkey[e]] + lpos[e]] = cloc;
 			}
-		// the workgroup that finishes last lays out the passes' lists:
-		// pfx[k][r] = first entry of replica r in the list of pass k
-		__builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-		__syncthreads();
-		if(t == 0)
-			sh_last = __hip_atomic_fetch_add(&d.scan_ticket[1], 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) ==
-			          (int) gridDim.x - 1;
-		__syncthreads();
-		if(sh_last) {
-			__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-			if(t < 8) {
-				int acc = 0;
-				d.pfx[t * (d.R + 1)] = 0;
-				for(int r = 0; r < d.R; r++) {
-					const int col = (int) ((d.reps[r].col_perm >> (4 * t)) & 0xFu);
-					acc += d.ccount[r * 8 + col];
-					d.pfx[t * (d.R + 1) + r + 1] = acc;
-				}
-			}
-		}
 	}
 	if(tile == 0) {
 		if(t == 0) d.start[d.C_total] = d.Ntot;
 		if(flip)
 			for(int r = t; r < d.R; r += 256) d.reps[r].sel ^= 1;
+	}
+}
+
+// The layout of the passes' lists, after the scan has counted them:
+// pfx[k][r] = first entry of replica r in the list of pass k (one thread per
+// pass; the kernel boundary makes every workgroup's counts visible).
+__global__ void k_pass_prefix(DevState d) {
+	const int t = threadIdx.x;
+	if(t < 8) {
+		int acc = 0;
+		d.pfx[t * (d.R + 1)] = 0;
+		for(int r = 0; r < d.R; r++) {
+			const int col = (int) ((d.reps[r].col_perm >> (4 * t)) & 0xFu);
+			acc += d.ccount[r * 8 + col];
+			d.pfx[t * (d.R + 1) + r + 1] = acc;
+		}
 	}
 }
 
@@ -414,6 +409,7 @@ void launch_rebuild(const DevState &d, int mode, uint64_t sweep, long long trace
 		hipLaunchKernelGGL(k_tile_scan, dim3(1), dim3(1024), 0, s, d);
 	}
 	hipLaunchKernelGGL(k_scan, dim3(d.ntiles), dim3(256), 0, s, d, flip, mode == REB_SHIFT ? 1 : 0, pre);
+	if(mode == REB_SHIFT) hipLaunchKernelGGL(k_pass_prefix, dim3(1), dim3(64), 0, s, d);
 	unsigned gs = gp;
 	if(mode == REB_SHIFT) gs = std::max(gp, (unsigned) ((d.C_total + B - 1) / B));
 	hipLaunchKernelGGL(k_scatter, dim3(gs), dim3(B), 0, s, d, mode, sweep);
