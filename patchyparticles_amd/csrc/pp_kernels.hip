@@ -206,17 +206,20 @@ __global__ __launch_bounds__(256) void k_scan(DevState d, int flip, int lists, i
 	else if(wid == 0) {
 		// wavefront 0 publishes the tile's sum and looks back over up to 64
 		// predecessors at once: the nearest one with an inclusive prefix ends
-		// the walk; one that has published nothing yet is polled again
+		// the walk; one that has published nothing yet is polled again.  A
+		// status word carries its flag and its value together, so relaxed
+		// agent-scope atomics suffice (a release would write the XCD's L2
+		// back on every publication).
 		const int agg = wsum[0] + wsum[1] + wsum[2] + wsum[3];
 		int excl = 0;
 		if(lane == 0)
 			__hip_atomic_store(&d.scan_status[tile], ((tile == 0 ? 2ull : 1ull) << 32) | (unsigned) agg,
-			                   __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+			                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 		int p = tile - 1, spins = 0;
 		while(p >= 0) {
 			const int q = p - lane;
 			unsigned long long st = 2ull << 32;  // before tile 0: an inclusive prefix of 0
-			if(q >= 0) st = __hip_atomic_load(&d.scan_status[q], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+			if(q >= 0) st = __hip_atomic_load(&d.scan_status[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 			const unsigned flag = (unsigned) (st >> 32);
 			const unsigned long long m2 = __ballot(flag == 2u);
 			const int l2 = m2 ? __ffsll((long long) m2) - 1 : 64;
@@ -236,7 +239,7 @@ __global__ __launch_bounds__(256) void k_scan(DevState d, int flip, int lists, i
 		}
 		if(lane == 0) {
 			if(tile > 0)
-				__hip_atomic_store(&d.scan_status[tile], (2ull << 32) | (unsigned) (excl + agg), __ATOMIC_RELEASE,
+				__hip_atomic_store(&d.scan_status[tile], (2ull << 32) | (unsigned) (excl + agg), __ATOMIC_RELAXED,
 				                   __HIP_MEMORY_SCOPE_AGENT);
 			sh_excl = excl;
 		}
