@@ -88,6 +88,28 @@ def test_cell_over_capacity_is_reported(pp):
     assert e.value.status == 5  # PP_ERR_CELL_CAP
 
 
+def test_small_team_capacity_is_reported(pp):
+    """An anisotropic lattice (spacings 1.5, 1, 1) in a box of 4^3 sweep
+    cells of side 3 puts exactly 2 x 3 x 3 = 18 particles in every cell,
+    whatever the grid shift: 32-lane teams sweep it (the tracked energy stays
+    exact), 16-lane teams (one member per lane) refuse it with
+    PP_ERR_CELL_CAP."""
+    g = np.stack(np.meshgrid(np.arange(8) * 1.5, np.arange(12) * 1.0, np.arange(12) * 1.0, indexing="ij"),
+                 -1).reshape(-1, 3) + 0.5  # exact differences: contacts at exactly 1, no overlap
+    O = np.broadcast_to(np.eye(3), (len(g), 3, 3)).copy()
+    box = np.full(3, 12.0)
+    with pp.System(g, O, box, T=0.1, cell_occupancy=12.0, trials_per_cell=16) as s:
+        assert s.grid()[0] == [4, 4, 4]
+        s.do_ensemble(3)
+        b, ov, tot, anyo = s.energy()
+        assert s.counters()["bonds"] == tot and not anyo
+    with pytest.raises(pp.PPError) as e:
+        with pp.System(g, O, box, T=0.1, cell_occupancy=12.0, trials_per_cell=16, threads_per_cell=16,
+                       block_size=64) as s:
+            s.do_ensemble(1)
+    assert e.value.status == 5  # PP_ERR_CELL_CAP
+
+
 def test_sweep_grids_keep_same_colour_cells_apart(pp, oracle):
     """The checkerboard's invariant, on the grids the library picks for many
     boxes (and after NPT volume moves): an even number of cells per axis, at
