@@ -682,15 +682,19 @@ __device__ __forceinline__ void pass_cell(const DevState &d, uint64_t sweep, int
 #pragma unroll
 			for(int c = 0; c < 9; c++) Oq[c] = REG_ORI ? orv[u][c] : ori[(9 * u + c) * T + lane];
 		};
+		// whether any team of the wavefront has neighbours beyond the
+		// register-resident ones (a scalar branch around those loops)
+		const bool wave_far = __ballot(M > T * CPL) != 0ull;
 		auto each_far = [&](auto &&f) {
-			for(int b = T * CPL; b < M; b += T)
-				if(b + lane < M) {
-					const int q = nb_slot(b + lane);
-					f(F[q], F[S + q], F[2 * S + q], [&](double *Oq) {
+			if(wave_far)
+				for(int b = T * CPL; b < M; b += T)
+					if(b + lane < M) {
+						const int q = nb_slot(b + lane);
+						f(F[q], F[S + q], F[2 * S + q], [&](double *Oq) {
 #pragma unroll
-						for(int c = 0; c < 9; c++) Oq[c] = F[(3 + c) * S + q];
-					});
-				}
+							for(int c = 0; c < 9; c++) Oq[c] = F[(3 + c) * S + q];
+						});
+					}
 		};
 		auto each_static = [&](auto &&f) {
 #pragma unroll
@@ -717,7 +721,7 @@ __device__ __forceinline__ void pass_cell(const DevState &d, uint64_t sweep, int
 					each_static([&](double qx, double qy, double qz, auto load_o) {
 						bonds_new(tm, Pm, qx, qy, qz, load_o, b, band);
 					});
-				if(team_any<T>(band) && band) {
+				if(__ballot(band) != 0ull && team_any<T>(band) && band) {
 					b = 0;
 					each_static([&](double qx, double qy, double qz, auto load_o) {
 						bonds_new_exact(tm, Pm, qx, qy, qz, load_o, b);
@@ -874,11 +878,12 @@ __device__ __forceinline__ void pass_cell(const DevState &d, uint64_t sweep, int
 				ms2 = sep_new(tr, mx, my, mz, msx, msy, msz);
 				ov |= ms2 < 1.;
 			}
-			for(int b = T * CPL; b < M; b += T)
-				if(b + lane < M) {
-					const int q = nb_slot(b + lane);
-					ov |= overlaps_new(tr, F[q], F[S + q], F[2 * S + q]);
-				}
+			if(wave_far)
+				for(int b = T * CPL; b < M; b += T)
+					if(b + lane < M) {
+						const int q = nb_slot(b + lane);
+						ov |= overlaps_new(tr, F[q], F[S + q], F[2 * S + q]);
+					}
 			if(team_any<T>(ov)) continue;
 
 			// O' = O R in the body frame (utils.c:172-184, 204-209) and its
@@ -924,7 +929,7 @@ __device__ __forceinline__ void pass_cell(const DevState &d, uint64_t sweep, int
 					});
 					if(has_m) bonds_new_sep(tr, Pn, msx, msy, msz, ms2, member_o, bnm, band);
 				}
-				if(team_any<T>(band) && band) {
+				if(__ballot(band) != 0ull && team_any<T>(band) && band) {
 					bns = 0;
 					bnm = 0;
 					each_static([&](double qx, double qy, double qz, auto load_o) {
@@ -950,7 +955,7 @@ __device__ __forceinline__ void pass_cell(const DevState &d, uint64_t sweep, int
 					each_neighbour([&](double qx, double qy, double qz, auto load_o) {
 						bonds_old_new(tr, Po, Pn, qx, qy, qz, load_o, bo, bn, band);
 					});
-				if(team_any<T>(band) && band) {
+				if(__ballot(band) != 0ull && team_any<T>(band) && band) {
 					bo = 0;
 					bn = 0;
 					each_neighbour([&](double qx, double qy, double qz, auto load_o) {
