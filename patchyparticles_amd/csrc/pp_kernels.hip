@@ -848,9 +848,9 @@ __device__ __forceinline__ void pass_cell(const DevState &d, uint64_t sweep, int
 			double cu0 = 0., cu1 = 0., cu2 = 0.;
 			bool deep = false;
 			if constexpr(KEEP) {
-				cu0 = ux + tv[1] * gnx;
-				cu1 = uy + tv[2] * gny;
-				cu2 = uz + tv[3] * gnz;
+				cu0 = __builtin_fma(tv[1], gnx, ux);  // u is an estimate: fused is fine
+				cu1 = __builtin_fma(tv[2], gny, uy);
+				cu2 = __builtin_fma(tv[3], gnz, uz);
 				constexpr double CELL_EPS = 1e-9;
 				deep = __builtin_fmax(__builtin_fabs(cu0 - 0.5), __builtin_fmax(__builtin_fabs(cu1 - 0.5),
 				                                                               __builtin_fabs(cu2 - 0.5))) <
