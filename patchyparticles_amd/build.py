@@ -80,15 +80,20 @@ def build_host():
 
 def build_checks():
     """GPU checkers run by the tests: kf_filter_check (the colour pass's
-    filtered pair test against the reference arithmetic at the thresholds)."""
-    src = os.path.join(ROOT, "tests", "gpu_checks", "kf_filter_check.hip")
-    if not os.path.exists(src):
-        return None
-    os.makedirs(BIN, exist_ok=True)
-    out = os.path.join(BIN, "kf_filter_check")
+    filtered pair test against the reference arithmetic at the thresholds)
+    and devmath_check (the device's sin, cos, exp, log and sqrt against the
+    host's C library), compiled with the library's flags."""
+    built = []
     flags = [f for f in HIPCC_FLAGS if f != "-fPIC"]
-    _run([hipcc(), *flags, "-I", CSRC, "-I", os.path.join(ROOT, "include"), src, "-o", out])
-    return out
+    for name in ("kf_filter_check", "devmath_check"):
+        src = os.path.join(ROOT, "tests", "gpu_checks", name + ".hip")
+        if not os.path.exists(src):
+            continue
+        os.makedirs(BIN, exist_ok=True)
+        out = os.path.join(BIN, name)
+        _run([hipcc(), *flags, "-I", CSRC, "-I", os.path.join(ROOT, "include"), src, "-o", out])
+        built.append(out)
+    return built
 
 
 def build_all():
