@@ -12,7 +12,12 @@ step is exactly N attempts).  The metric counts sweeps of N attempted moves
 Prints one JSON line: sweeps/s (value), the dominant kernel's achieved
 algorithmic bandwidth against the HBM roofline (HIP events on the kernel's
 own stream, inside the timed region), decorrelated samples/s of E/N, and the
-reference's own CPU code timed on this host (cpu_baseline).
+reference's own CPU code timed on this host (cpu_baseline).  Decorrelated
+samples/s = sweeps/s / (2 tau_int[E/N]); every tau_int run (plain, plain with
+one trial per cell, learned) starts from the state the timed region ends in
+and runs the same number of sweeps, untimed for the plain one, and is given
+with its statistical error, its length in units of tau and the drift of E/N
+across it.
 """
 from __future__ import annotations
 
@@ -105,7 +110,40 @@ def measured_traffic(path=os.path.join(ROOT, "profiles", "k_pass_n32768_rho0.5_p
     return None
 
 
-def learned_leg(args, pp, tau_int, r_eq, O_eq, box_eq, opts):
+def series_stats(e, steps_per_sweep):
+    """tau_int[E/N] of a per-step E/N series in sweeps, with its statistical
+    error, the run's length in sweeps and in units of tau, and the drift of
+    E/N (second half's mean minus the first's, with the block standard error
+    of that difference): a drift of several standard errors means the run is
+    not stationary and tau is overestimated."""
+    import numpy as np
+    from patchyparticles_amd.stats import block_mean, tau_int_err
+    tau_s, tau_e, _ = tau_int_err(e)
+    tau = tau_s / steps_per_sweep
+    h = len(e) // 2
+    m1, se1 = block_mean(e[:h])
+    m2, se2 = block_mean(e[h:])
+    run = len(e) / steps_per_sweep
+    return {"tau_int_sweeps": round(tau, 2), "tau_int_err_sweeps": round(tau_e / steps_per_sweep, 2),
+            "run_sweeps": round(run), "run_over_tau": round(run / tau, 1), "E_per_N": round(float(e.mean()), 5),
+            "E_drift": round(m2 - m1, 5), "E_drift_se": round(float(np.hypot(se1, se2)), 5)}
+
+
+def tau_leg(pp, r_eq, O_eq, box_eq, opts, trials_per_cell, sweeps, attempts_per_step):
+    """tau_int[E/N] of the plain sampler from the equilibrated state over
+    `sweeps` sweeps (untimed; its rate is the timed region's)."""
+    steps = max(1, int(math.ceil(sweeps * N / attempts_per_step)))
+    with pp.System(r_eq, O_eq, box_eq, seed=SEED + 4, trials_per_cell=trials_per_cell, **opts) as s:
+        s.do_ensemble(50)
+        s.reset_counters()
+        s.set_trace(steps)
+        s.do_ensemble(steps)
+        c = s.counters()
+        e = s.trace() / N
+    return series_stats(e, N * steps / c["tries"][0])
+
+
+def learned_leg(args, pp, r_eq, O_eq, box_eq, opts):
     """The learned proposal against the plain move on the same schedule (one
     trial per non-empty cell and pass), from the equilibrated state: a
     ProposalNet trained on --train-steps snapshots of the plain sampler (one
@@ -131,9 +169,9 @@ def learned_leg(args, pp, tau_int, r_eq, O_eq, box_eq, opts):
             b, ov, tot, anyo = s.energy()
             assert not anyo and tot == c["bonds"], "tracked energy disagrees with a full recompute"
         sw = c["tries"][0] / N / el
-        tau = tau_int(e) / (N * args.learned_steps / c["tries"][0])
-        out = {"sweeps_per_s": round(sw, 2), "tau_int_sweeps": round(tau, 2),
-               "decorrelated_samples_per_s": round(sw / (2 * tau), 4), "E_per_N": round(float(e.mean()), 5),
+        st = series_stats(e, N * args.learned_steps / c["tries"][0])
+        out = {"sweeps_per_s": round(sw, 2), **st,
+               "decorrelated_samples_per_s": round(sw / (2 * st["tau_int_sweeps"]), 4),
                "acceptance": round(c["accepted"][0] / max(c["tries"][0], 1), 5)}
         if net is not None:
             out["learned_acceptance"] = round(al / max(tl, 1), 5)
@@ -177,15 +215,16 @@ def _jumps(s, sweeps):
     return tr / att, rot / att
 
 
-def bench_learned(args, pp, tau_int):
+def bench_learned(args, pp):
     """The learned proposal at the gel state point: N=65536, rho=0.3, T=0.09.
     Plain equilibration (--warmup steps of 8 trials per cell), a network
     trained on --train-steps snapshots of the plain sampler, then --steps
     timed steps each of the learned form (--learned-fraction of the passes
     learned, the rest plain; one trial per non-empty cell and pass) and of
-    the plain move on the same schedule.  value = sweeps/s of the learned
-    form; jumps per attempt and tau_int[E/N] for both (tau_int is only a
-    lower bound when the run is shorter than a few tau)."""
+    the plain move on the same schedule, both from the same state.  value =
+    sweeps/s of the learned form; jumps per attempt and tau_int[E/N] for
+    both, with tau's error, the run's length in units of tau and the drift of
+    E/N across it."""
     n, rho, temp = 65536, 0.3, 0.09
     r, O, box = pp.generate(n, rho, SEED)
     opts = dict(T=temp, disp_max=DISP, theta_max=THETA, kf_delta=DELTA, kf_cosmax=COSMAX)
@@ -218,11 +257,10 @@ def bench_learned(args, pp, tau_int):
             b, ov, tot, anyo = s.energy()
             assert not anyo and tot == c["bonds"], "tracked energy disagrees with a full recompute"
         sw = c["tries"][0] / n / el
-        tau = tau_int(e) / (n * args.steps / c["tries"][0])
+        st_e = series_stats(e, n * args.steps / c["tries"][0])
         res[name] = {"sweeps_per_s": round(sw, 2), "acceptance": round(c["accepted"][0] / c["tries"][0], 5),
-                     "jump_sq_per_attempt": round(jt, 8), "one_minus_cos_per_attempt": round(jr, 8),
-                     "tau_int_sweeps": round(tau, 2), "decorrelated_samples_per_s": round(sw / (2 * tau), 4),
-                     "E_per_N": round(float(e.mean()), 5), "run_sweeps": round(c["tries"][0] / n, 1)}
+                     "jump_sq_per_attempt": round(jt, 8), "one_minus_cos_per_attempt": round(jr, 8), **st_e,
+                     "decorrelated_samples_per_s": round(sw / (2 * st_e["tau_int_sweeps"]), 4)}
         if name == "learned":
             res[name]["learned_pass_us"] = round(ml / max(nl, 1) * 1e3, 3)
         print(f"# {name}: {res[name]}", file=sys.stderr, flush=True)
@@ -343,7 +381,10 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=4000, help="timed steps (a step: rebuild + 8 colour passes)")
-    ap.add_argument("--warmup", type=int, default=6000, help="untimed steps first (equilibration)")
+    ap.add_argument("--warmup", type=int, default=None,
+                    help="untimed steps first (equilibration; default 40000 for the single workload, else 6000)")
+    ap.add_argument("--tau-sweeps", type=int, default=24000,
+                    help="sweeps of the plain tau_int run when the learned leg is off")
     ap.add_argument("--trials-per-cell", type=int, default=16,
                     help="trials per non-empty cell and pass (0: one per member, a step of exactly N attempts)")
     ap.add_argument("--per-member-steps", type=int, default=2000,
@@ -369,6 +410,8 @@ def main():
                     help="only copy a MIB-sized device buffer 5 times (the reference point that scales "
                          "the FETCH_SIZE / WRITE_SIZE counters of a profile)")
     args = ap.parse_args()
+    if args.warmup is None:
+        args.warmup = 40000 if args.workload == "single" else 6000
     if args.calibrate_copy:
         import torch
         n = args.calibrate_copy * (1 << 20) // 4
@@ -386,12 +429,11 @@ def main():
 
     import numpy as np
     import patchyparticles_amd as pp
-    from patchyparticles_amd.stats import tau_int
 
     if args.workload == "batch":
         return bench_batch(args, pp)
     if args.workload == "learned":
-        return bench_learned(args, pp, tau_int)
+        return bench_learned(args, pp)
     if args.workload == "npt":
         return bench_npt(args, pp)
 
@@ -407,7 +449,6 @@ def main():
     # --time-every-th step (the per-kernel averages of the roofline line)
     sysm.reset_counters()
     sysm.set_timing(0 if args.no_timing else args.time_every)
-    sysm.set_trace(args.steps)
     t0 = time.perf_counter()
     sysm.do_ensemble(args.steps)  # synchronous: returns when the device is done
     t1 = time.perf_counter()
@@ -415,7 +456,6 @@ def main():
     n_pass, ms_pass = sysm.timing("pass")
     n_reb, ms_reb = sysm.timing("rebuild")
     c = sysm.counters()
-    e_series = sysm.trace() / N
     sysm.set_timing(0)
     print(f"# timed {args.steps} steps in {elapsed:.3f} s", file=sys.stderr, flush=True)
 
@@ -427,9 +467,6 @@ def main():
     bytes_per_launch = bpm * moves_per_launch
     avg_pass_s = ms_pass / max(n_pass, 1) / 1e3
     achieved = bytes_per_launch / avg_pass_s / 1e9 if avg_pass_s > 0 else 0.0
-    steps_per_sweep = N * args.steps / attempts
-    tau = tau_int(e_series) / steps_per_sweep  # in sweeps of N attempts
-    decorrelated = sweeps_per_s / (2 * tau)
 
     # parity spot check of the final snapshot (energy recomputed vs tracked)
     b, ov, tot, anyo = sysm.energy()
@@ -450,9 +487,16 @@ def main():
         per_member = {"trials_per_cell": 0, "sweeps_per_s": round(c0["tries"][0] / N / (tb - ta), 2),
                       "steps": args.per_member_steps}
 
+    # decorrelated samples: every tau run starts from this state and runs as
+    # many sweeps as the one-trial-per-cell plain run (or --tau-sweeps)
     learned = plain_same = None
+    tau_sweeps = args.tau_sweeps
     if args.learned_steps > 0:
-        learned, plain_same = learned_leg(args, pp, tau_int, r_eq, O_eq, box_eq, opts)
+        learned, plain_same = learned_leg(args, pp, r_eq, O_eq, box_eq, opts)
+        tau_sweeps = plain_same["run_sweeps"]
+    print(f"# tau_int of the plain run over {tau_sweeps} sweeps ...", file=sys.stderr, flush=True)
+    plain_tau = tau_leg(pp, r_eq, O_eq, box_eq, opts, args.trials_per_cell, tau_sweeps, attempts / args.steps)
+    decorrelated = sweeps_per_s / (2 * plain_tau["tau_int_sweeps"])
 
     cpu = None
     if not args.no_cpu:
@@ -483,7 +527,7 @@ def main():
             "E_per_N": round(float(-c["bonds"] / N), 6),
         },
         "decorrelated_samples_per_s": {
-            "plain": round(decorrelated, 3), "tau_int_sweeps": round(tau, 2),
+            "plain": round(decorrelated, 3), "plain_run": plain_tau,
             "learned": learned["decorrelated_samples_per_s"] if learned else None,
             "learned_over_plain": round(learned["decorrelated_samples_per_s"] / decorrelated, 4) if learned else None,
             "learned_run": learned,

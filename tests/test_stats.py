@@ -6,7 +6,7 @@ import math
 import numpy as np
 import pytest
 
-from patchyparticles_amd.stats import autocorr, block_mean, tau_int
+from patchyparticles_amd.stats import autocorr, block_mean, tau_int, tau_int_err
 
 
 def _ar1(phi, n, seed):
@@ -40,6 +40,26 @@ def test_block_standard_error_of_ar1():
     want = math.sqrt(2 * tau * var / n)
     assert abs(se - want) < 0.4 * want
     assert abs(m) < 5 * want
+
+
+def test_tau_int_error_matches_the_spread_of_estimates():
+    """Over 300 independent AR(1) series (phi = 0.9, tau = 9.5, n = 20000,
+    about 2100 tau) the stated error is conservative, at most 1.5 times the
+    spread of the estimates (the formula holds for windows much longer than
+    tau; at W = 5 tau it is about 1.3 times the spread), and the estimates
+    centre on the true tau."""
+    phi, n = 0.9, 20000
+    want = 0.5 + phi / (1 - phi)
+    est, errs = [], []
+    for k in range(300):
+        t, e, W = tau_int_err(_ar1(phi, n, seed=1000 + k))
+        assert W >= 5 * t
+        est.append(t)
+        errs.append(e)
+    est = np.array(est)
+    spread = est.std(ddof=1)
+    assert spread <= np.mean(errs) <= 1.5 * spread, (np.mean(errs), spread)
+    assert abs(est.mean() - want) < 4 * spread / math.sqrt(len(est)) + 0.02 * want
 
 
 def test_constant_series():
