@@ -359,7 +359,9 @@ pp_status create_impl(int R, const pp_params *prm, const pp_options *opts, const
 	d.stride = (int) std::max<long long>(Ntot, 1);
 	d.R = R;
 	d.C_total = (int) coff;
-	d.ntiles = (d.C_total + SCAN_TILE - 1) / SCAN_TILE;
+	d.scan_tile = (d.C_total + SCAN_TILE_SMALL - 1) / SCAN_TILE_SMALL <= SCAN_SMALL_MAX_TILES ? SCAN_TILE_SMALL
+	                                                                                   : SCAN_TILE_LARGE;
+	d.ntiles = (d.C_total + d.scan_tile - 1) / d.scan_tile;
 	d.total_act = (int) aoff;
 	{
 		// the kernels carry the base patches as compile-time constants

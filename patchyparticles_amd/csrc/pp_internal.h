@@ -14,7 +14,12 @@ constexpr int NSLOT = 64;      // spread of per-replica atomic counters
 constexpr int NCOMP = 12;      // x, y, z, O00..O22 per particle
 constexpr int TRIAL_W = 14;    // one trial: member rank, dx, dy, dz, R[9], acceptance draw
 constexpr int MAX_TRIALS_PER_CELL = 16;
-constexpr int SCAN_TILE = 2048;
+// cells per scan workgroup of 256 lanes: small tiles shorten each lane's
+// chain where the whole grid is a few tiles; large ones keep the per-tile
+// overheads down on grids of many tiles
+constexpr int SCAN_TILE_SMALL = 512;
+constexpr int SCAN_TILE_LARGE = 2048;
+constexpr int SCAN_SMALL_MAX_TILES = 128;  // small tiles up to this many (look-back scan)
 constexpr int REC_W = 64;      // ints of one cell record of a pass
 constexpr int ACC_TAB = 16;    // Metropolis factors exp(-k/T) tabulated per replica, k < ACC_TAB
 
@@ -76,9 +81,9 @@ enum Err : int { ERR_CELL_CAP = 1, ERR_BOX = 2, ERR_SCAN = 8, ERR_BARRIER = 16 }
 // rebuild modes
 enum Reb : int { REB_KEEP = 0, REB_SHIFT = 1, REB_TRIAL = 2 };
 
-// grids of more scan tiles than this are scanned in three steps (tile sums,
-// one scan of the sums, the tiles) instead of one pass with a look-back,
-// whose chains grow with the number of tiles in flight
+// grids of more large scan tiles than this are scanned in three steps (tile
+// sums, one scan of the sums, the tiles) instead of one pass with a
+// look-back, whose chains grow with the number of tiles in flight
 constexpr int SCAN_LOOKBACK_MAX_TILES = 64;
 
 // spin bound of the scan's look-back (a stuck predecessor is reported, not waited on forever)
@@ -168,6 +173,7 @@ struct DevState {
 	int *scan_ticket;  // [0]: tiles taken, [1]: tiles done
 	int *tile_sum;     // [ntiles] per-tile sums, then their exclusive prefixes (grids of many tiles)
 	int ntiles;
+	int scan_tile;     // SCAN_TILE_SMALL or SCAN_TILE_LARGE
 	unsigned long long *ctr;  // [R][NCTR][NSLOT]
 	int *err;
 	int32_t *out_bonds;

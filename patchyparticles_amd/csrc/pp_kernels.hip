@@ -130,12 +130,13 @@ constexpr int SCAN_KEYS = 512;
 // c; count is cleared for the next rebuild; a flipping scan also switches
 // every replica to the buffer the scatter fills.
 // Sum of the cell counts of each scan tile.
+template <int PER>
 __global__ __launch_bounds__(256) void k_tile_sums(DevState d) {
 	__shared__ int part[4];
-	const int t = threadIdx.x, base = blockIdx.x * SCAN_TILE + t * 8;
+	const int t = threadIdx.x, base = blockIdx.x * (256 * PER) + t * PER;
 	int tot = 0;
 #pragma unroll
-	for(int e = 0; e < 8; e++)
+	for(int e = 0; e < PER; e++)
 		if(base + e < d.C_total) tot += d.count[base + e];
 	tot = team_sum<64>(tot);
 	if((t & 63) == 0) part[t >> 6] = tot;
@@ -168,6 +169,8 @@ __global__ __launch_bounds__(1024) void k_tile_scan(DevState d) {
 
 // pre = 1: the tiles' exclusive prefixes are in d.tile_sum (k_tile_sums,
 // k_tile_scan); pre = 0: a decoupled look-back finds them.
+// PER cells per lane: tiles of 256 * PER cells.
+template <int PER>
 __global__ __launch_bounds__(256) void k_scan(DevState d, int flip, int lists, int pre) {
 	__shared__ int wsum[4];
 	__shared__ int sh_tile, sh_excl, sh_r0;
@@ -177,21 +180,21 @@ __global__ __launch_bounds__(256) void k_scan(DevState d, int flip, int lists, i
 	for(int k = t; k < SCAN_KEYS; k += 256) lcnt[k] = 0;
 	__syncthreads();
 	const int tile = sh_tile;
-	const int base = tile * SCAN_TILE + t * 8;
+	const int base = tile * (256 * PER) + t * PER;
 	if(t == 0 && lists) {
 		// replica of the tile's first cell
 		int lo = 0, hi = d.R;
 		while(hi - lo > 1) {
 			int mid = (lo + hi) >> 1;
-			if(d.reps[mid].cell_off <= tile * SCAN_TILE) lo = mid;
+			if(d.reps[mid].cell_off <= tile * (256 * PER)) lo = mid;
 			else hi = mid;
 		}
 		sh_r0 = lo;
 	}
-	int v[8];
+	int v[PER];
 	int tot = 0;
 #pragma unroll
-	for(int e = 0; e < 8; e++) {
+	for(int e = 0; e < PER; e++) {
 		int c = base + e;
 		v[e] = c < d.C_total ? d.count[c] : 0;
 		tot += v[e];
@@ -248,9 +251,9 @@ __global__ __launch_bounds__(256) void k_scan(DevState d, int flip, int lists, i
 	int woff = 0;
 	for(int w = 0; w < wid; w++) woff += wsum[w];
 	int run = sh_excl + woff + inc - tot;
-	int lkey[8], lpos[8];
+	int lkey[PER], lpos[PER];
 #pragma unroll
-	for(int e = 0; e < 8; e++) {
+	for(int e = 0; e < PER; e++) {
 		int c = base + e;
 		lkey[e] = -1;
 		lpos[e] = 0;
@@ -293,7 +296,7 @@ __global__ __launch_bounds__(256) void k_scan(DevState d, int flip, int lists, i
 		}
 		__syncthreads();
 #pragma unroll
-		for(int e = 0; e < 8; e++)
+		for(int e = 0; e < PER; e++)
 			if(lkey[e] >= 0) {
 				const int r = sh_r0 + lkey[e] / 8;
 				const Rep &rp = d.reps[r];
@@ -406,12 +409,19 @@ void launch_rebuild(const DevState &d, int mode, uint64_t sweep, long long trace
 	if(gp == 0) gp = 1;
 	const int flip = mode != REB_TRIAL;
 	hipLaunchKernelGGL(k_count, dim3(gp), dim3(B), 0, s, d, mode, sweep, trace_pos);
-	const int pre = d.ntiles > SCAN_LOOKBACK_MAX_TILES;
-	if(pre) {
-		hipLaunchKernelGGL(k_tile_sums, dim3(d.ntiles), dim3(256), 0, s, d);
-		hipLaunchKernelGGL(k_tile_scan, dim3(1), dim3(1024), 0, s, d);
+	const int lists = mode == REB_SHIFT ? 1 : 0;
+	if(d.scan_tile == SCAN_TILE_SMALL) {
+		// at most SCAN_SMALL_MAX_TILES tiles: one pass with a look-back
+		hipLaunchKernelGGL(k_scan<SCAN_TILE_SMALL / 256>, dim3(d.ntiles), dim3(256), 0, s, d, flip, lists, 0);
 	}
-	hipLaunchKernelGGL(k_scan, dim3(d.ntiles), dim3(256), 0, s, d, flip, mode == REB_SHIFT ? 1 : 0, pre);
+	else {
+		const int pre = d.ntiles > SCAN_LOOKBACK_MAX_TILES;
+		if(pre) {
+			hipLaunchKernelGGL(k_tile_sums<SCAN_TILE_LARGE / 256>, dim3(d.ntiles), dim3(256), 0, s, d);
+			hipLaunchKernelGGL(k_tile_scan, dim3(1), dim3(1024), 0, s, d);
+		}
+		hipLaunchKernelGGL(k_scan<SCAN_TILE_LARGE / 256>, dim3(d.ntiles), dim3(256), 0, s, d, flip, lists, pre);
+	}
 	if(mode == REB_SHIFT) hipLaunchKernelGGL(k_pass_prefix, dim3(1), dim3(64), 0, s, d);
 	unsigned gs = gp;
 	if(mode == REB_SHIFT) gs = std::max(gp, (unsigned) ((d.C_total + B - 1) / B));
