@@ -384,7 +384,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=None,
                     help="untimed steps first (equilibration; default 40000 for the single workload, else 6000)")
     ap.add_argument("--tau-sweeps", type=int, default=24000,
-                    help="sweeps of the plain tau_int run when the learned leg is off")
+                    help="sweeps of the plain tau_int run when the learned leg is off (0: no tau_int run)")
     ap.add_argument("--trials-per-cell", type=int, default=16,
                     help="trials per non-empty cell and pass (0: one per member, a step of exactly N attempts)")
     ap.add_argument("--per-member-steps", type=int, default=2000,
@@ -494,9 +494,11 @@ def main():
     if args.learned_steps > 0:
         learned, plain_same = learned_leg(args, pp, r_eq, O_eq, box_eq, opts)
         tau_sweeps = plain_same["run_sweeps"]
-    print(f"# tau_int of the plain run over {tau_sweeps} sweeps ...", file=sys.stderr, flush=True)
-    plain_tau = tau_leg(pp, r_eq, O_eq, box_eq, opts, args.trials_per_cell, tau_sweeps, attempts / args.steps)
-    decorrelated = sweeps_per_s / (2 * plain_tau["tau_int_sweeps"])
+    plain_tau = decorrelated = None
+    if tau_sweeps > 0:
+        print(f"# tau_int of the plain run over {tau_sweeps} sweeps ...", file=sys.stderr, flush=True)
+        plain_tau = tau_leg(pp, r_eq, O_eq, box_eq, opts, args.trials_per_cell, tau_sweeps, attempts / args.steps)
+        decorrelated = sweeps_per_s / (2 * plain_tau["tau_int_sweeps"])
 
     cpu = None
     if not args.no_cpu:
@@ -527,9 +529,10 @@ def main():
             "E_per_N": round(float(-c["bonds"] / N), 6),
         },
         "decorrelated_samples_per_s": {
-            "plain": round(decorrelated, 3), "plain_run": plain_tau,
+            "plain": round(decorrelated, 3) if decorrelated else None, "plain_run": plain_tau,
             "learned": learned["decorrelated_samples_per_s"] if learned else None,
-            "learned_over_plain": round(learned["decorrelated_samples_per_s"] / decorrelated, 4) if learned else None,
+            "learned_over_plain": round(learned["decorrelated_samples_per_s"] / decorrelated, 4)
+            if learned and decorrelated else None,
             "learned_run": learned,
             "plain_one_trial_per_cell": plain_same,
         },
