@@ -284,20 +284,23 @@ def test_uneven_batch(pp, oracle, trials_per_cell):
 
 
 @pytest.mark.parametrize("ensemble", ["nvt", "npt"])
-def test_large_grid_scan(pp, oracle, ensemble):
-    """A dilute system whose cell grid spans more scan tiles than the
-    look-back handles (N=20000 at rho=0.05: 64^3 cells, 128 tiles), so every
-    rebuild takes the three-step scan: the tracked energy equals a full
-    recompute, the cell populations check, and the energies of the evolved
-    snapshot equal the oracle's bit for bit (NPT: through trial rebuilds and
-    accepted volume moves)."""
-    r, O, box = _start(oracle, 20000, 0.05, 8)
+@pytest.mark.parametrize("n,cells_lo,cells_hi", [(4000, 0, 128 * 512), (8000, 128 * 512, 64 * 2048),
+                                                 (20000, 64 * 2048, 1 << 30)])
+def test_large_grid_scan(pp, oracle, ensemble, n, cells_lo, cells_hi):
+    """Dilute systems (rho=0.05) whose cell grids take each form of the
+    rebuild's scan: up to 128 tiles of 512 cells (N=4000: 38^3 cells), 2048-cell
+    tiles with the look-back (N=8000: 48^3) and 2048-cell tiles in three
+    steps, more than 64 of them (N=20000: 64^3).  The tracked energy equals a
+    full recompute, the cell populations check, and the energies of the
+    evolved snapshot equal the oracle's bit for bit (NPT: through trial
+    rebuilds and accepted volume moves)."""
+    r, O, box = _start(oracle, n, 0.05, 8)
     prm = dict(T=0.11)
     if ensemble == "npt":
         prm = dict(T=0.11, ensemble=pp.NPT, P=0.02, rescale_factor_max=0.002)
     with pp.System(r, O, box, seed=5, trials_per_cell=4, **prm) as s:
         nside, _ = s.grid()
-        assert np.prod(nside) > 64 * 2048
+        assert cells_lo < np.prod(nside) <= cells_hi
         s.do_ensemble(60)
         c = s.counters()
         b, ov, tot, anyo = s.energy()
