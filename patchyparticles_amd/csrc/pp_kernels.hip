@@ -867,9 +867,10 @@ __device__ __forceinline__ void pass_cell(const DevState &d, uint64_t sweep, int
 				       0.5 - CELL_EPS;
 			}
 			if(!deep) {
-				const bool out = (uni<T>(cell_coord(tr.nx, sh0, tr.iLx, n0)) != ix) |
-				                 (uni<T>(cell_coord(tr.ny, sh1, tr.iLy, n1)) != iy) |
-				                 (uni<T>(cell_coord(tr.nz, sh2, tr.iLz, n2)) != iz);
+				// all three axes evaluated (no short circuit)
+				const int out = (int) (uni<T>(cell_coord(tr.nx, sh0, tr.iLx, n0)) != ix) |
+				                (int) (uni<T>(cell_coord(tr.ny, sh1, tr.iLy, n1)) != iy) |
+				                (int) (uni<T>(cell_coord(tr.nz, sh2, tr.iLz, n2)) != iz);
 				if(out) continue;
 			}
 
