@@ -19,7 +19,7 @@ constexpr int MAX_TRIALS_PER_CELL = 16;
 // overheads down on grids of many tiles
 constexpr int SCAN_TILE_SMALL = 512;
 constexpr int SCAN_TILE_LARGE = 2048;
-constexpr int SCAN_SMALL_MAX_TILES = 128;  // small tiles up to this many (look-back scan)
+constexpr int SCAN_SMALL_MAX_TILES = 320;  // small tiles up to this many (look-back scan)
 constexpr int REC_W = 64;      // ints of one cell record of a pass
 constexpr int ACC_TAB = 16;    // Metropolis factors exp(-k/T) tabulated per replica, k < ACC_TAB
 

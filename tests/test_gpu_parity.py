@@ -284,13 +284,14 @@ def test_uneven_batch(pp, oracle, trials_per_cell):
 
 
 @pytest.mark.parametrize("ensemble", ["nvt", "npt"])
-@pytest.mark.parametrize("n,cells_lo,cells_hi", [(4000, 0, 128 * 512), (8000, 128 * 512, 64 * 2048),
-                                                 (20000, 64 * 2048, 1 << 30)])
+@pytest.mark.parametrize("n,cells_lo,cells_hi", [(4000, 0, 128 * 512), (8000, 128 * 512, 320 * 512),
+                                                 (20000, 320 * 512, 1 << 30)])
 def test_large_grid_scan(pp, oracle, ensemble, n, cells_lo, cells_hi):
     """Dilute systems (rho=0.05) whose cell grids take each form of the
-    rebuild's scan: up to 128 tiles of 512 cells (N=4000: 38^3 cells), 2048-cell
-    tiles with the look-back (N=8000: 48^3) and 2048-cell tiles in three
-    steps, more than 64 of them (N=20000: 64^3).  The tracked energy equals a
+    rebuild's scan: 512-cell tiles whose look-back reaches back at most 64
+    tiles (N=4000: 38^3 cells, 108 tiles) or further, up to the limit of 320
+    tiles (N=8000: 48^3, 216 tiles), and 2048-cell tiles in three steps
+    (N=20000: 64^3).  The tracked energy equals a
     full recompute, the cell populations check, and the energies of the
     evolved snapshot equal the oracle's bit for bit (NPT: through trial
     rebuilds and accepted volume moves)."""
