@@ -325,6 +325,12 @@ pp_status create_impl(int R, const pp_params *prm, const pp_options *opts, const
 			return bail(fail(PP_ERR_BOX,
 			                 "replica %d: box (%g, %g, %g) holds fewer than %d cells of side %g per axis", k, b[0],
 			                 b[1], b[2], rp.min_n, side));
+		// a team stages at most CELL_CAP members: a grid whose mean occupancy
+		// is above that cannot run (a lower one may still fill single cells
+		// beyond it, which the passes report as PP_ERR_CELL_CAP)
+		if(!energy_only && (double) n[k] > (double) CELL_CAP * gn[0] * gn[1] * gn[2])
+			return bail(fail(PP_ERR_CELL_CAP, "replica %d: %d particles in %d sweep cells exceed %d per cell", k, n[k],
+			                 gn[0] * gn[1] * gn[2], CELL_CAP));
 		for(int a = 0; a < 3; a++) {
 			int cap = gn[a];
 			if(p.ensemble == PP_NPT) cap = (gn[a] + std::max(2, gn[a] / 8) + 1) & ~1;

@@ -658,7 +658,7 @@ __global__ __launch_bounds__(64 * LP_WAVES) void k_learn_pass(DevState d, NetDev
 	const int g0 = blockIdx.x * LP_ROWS;
 	if(g0 >= count) return;  // the whole workgroup: before any barrier
 	const int gi = g0 + cell;
-	const bool live = gi < count;
+	bool live = gi < count;
 	CellView v{};
 	const double *F = nullptr;
 	const int SS = d.stride;
@@ -669,7 +669,14 @@ __global__ __launch_bounds__(64 * LP_WAVES) void k_learn_pass(DevState d, NetDev
 	double L[3], iL[3];
 	if(live) {
 		v = cell_view(d, pass, gi);
-		for(int e = tl; e < REC_W; e += T) CL.rec[e] = v.rec[e];
+		// a cell of more members than a team holds (one per lane) is not
+		// moved and is reported; its threads stay for the barriers below
+		if(v.cnt > CELL_CAP || v.cnt > T) {
+			if(tl == 0) atomicOr(d.err, (int) ERR_CELL_CAP);
+			live = false;
+		}
+		else
+			for(int e = tl; e < REC_W; e += T) CL.rec[e] = v.rec[e];
 	}
 	__syncthreads();
 

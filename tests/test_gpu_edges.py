@@ -133,3 +133,23 @@ def test_sweep_grids_keep_same_colour_cells_apart(pp, oracle):
             nside, side = s.grid()
             assert all(k % 2 == 0 and k >= 4 for k in nside) and min(side) >= 1.12
         assert s.counters()["volume"] < np.prod(box)  # the box did shrink
+
+
+@pytest.mark.parametrize("learned_fraction", [0.0, 1.0])
+def test_over_full_cell_is_reported_by_every_pass(pp, learned_fraction):
+    """A simple-cubic slab of spacing 1 filling half of a 32 x 16 x 16 box,
+    swept on 8 x 4 x 4 cells of side 4: the mean occupancy (16) passes the
+    host's check, but every cell inside the slab holds 64 particles, more
+    than a team of 32 lanes holds one per lane.  Plain passes and learned
+    passes (learned_fraction = 1: no plain pass runs) both refuse such a
+    cell with PP_ERR_CELL_CAP instead of moving a member no lane holds."""
+    g = np.stack(np.meshgrid(np.arange(8.0), np.arange(16.0), np.arange(16.0), indexing="ij"), -1).reshape(-1, 3) + 0.5
+    O = np.broadcast_to(np.eye(3), (len(g), 3, 3)).copy()
+    box = np.array([32.0, 16.0, 16.0])
+    with pp.System(g, O, box, T=0.1, cell_occupancy=15.0, seed=3) as s, pp.ProposalNet(seed=1) as net:
+        assert s.grid()[0] == [8, 4, 4]
+        if learned_fraction > 0:
+            s.attach(net, learned_fraction)
+        with pytest.raises(pp.PPError) as e:
+            s.do_ensemble(1)
+        assert e.value.status == 5  # PP_ERR_CELL_CAP
