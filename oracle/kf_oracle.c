@@ -783,3 +783,7 @@ void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t ou
 	out[2] = c2;
 	out[3] = c3;
 }
+
+void orc_philox4x32_10_n(int n, const uint32_t *ctr, const uint32_t *key, uint32_t *out) {
+	for(int i = 0; i < n; i++) orc_philox4x32_10(ctr + 4 * (size_t) i, key + 2 * (size_t) i, out + 4 * (size_t) i);
+}

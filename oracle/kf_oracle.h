@@ -129,6 +129,8 @@ int orc_rrr_save_generated(const char *path, int n, const double box[3], const d
 /* Philox4x32-10 (Salmon et al., SC'11), the counter RNG of the GPU path:
  * out = bijection(ctr; key). Restated here as the checker for the device. */
 void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+/* n blocks at once: ctr[4n], key[2n] -> out[4n]. */
+void orc_philox4x32_10_n(int n, const uint32_t *ctr, const uint32_t *key, uint32_t *out);
 
 #ifdef __cplusplus
 }

@@ -84,6 +84,7 @@ def lib():
         L.orc_rrr_save_generated.argtypes = [C.c_char_p, C.c_int, _dp, _dp, _dp]
         L.orc_rrr_save_generated.restype = C.c_int
         L.orc_philox4x32_10.argtypes = [_u32p, _u32p, _u32p]
+        L.orc_philox4x32_10_n.argtypes = [C.c_int, _u32p, _u32p, _u32p]
         _lib = L
     return _lib
 
@@ -164,6 +165,15 @@ def rrr_save(path, step, box, r, O, generator_header=False):
 def philox(ctr, key):
     out = np.zeros(4, np.uint32)
     lib().orc_philox4x32_10(np.asarray(ctr, np.uint32), np.asarray(key, np.uint32), out)
+    return out
+
+
+def philox_n(ctr, key):
+    """Philox4x32-10 of n blocks: ctr (n, 4), key (n, 2) uint32 -> (n, 4)."""
+    ctr = np.ascontiguousarray(ctr, np.uint32).reshape(-1, 4)
+    key = np.ascontiguousarray(key, np.uint32).reshape(-1, 2)
+    out = np.zeros_like(ctr)
+    lib().orc_philox4x32_10_n(len(ctr), ctr, key, out)
     return out
 
 

@@ -80,12 +80,14 @@ def build_host():
 
 def build_checks():
     """GPU checkers run by the tests: kf_filter_check (the colour pass's
-    filtered pair test against the reference arithmetic at the thresholds)
-    and devmath_check (the device's sin, cos, exp, log and sqrt against the
-    host's C library), compiled with the library's flags."""
+    filtered pair test against the reference arithmetic at the thresholds),
+    devmath_check (the device's sin, cos, exp, log and sqrt against the
+    host's C library) and philox_check (the device's Philox blocks, draw
+    counters and colour orders, compared with the oracle by the test),
+    compiled with the library's flags."""
     built = []
     flags = [f for f in HIPCC_FLAGS if f != "-fPIC"]
-    for name in ("kf_filter_check", "devmath_check"):
+    for name in ("kf_filter_check", "devmath_check", "philox_check"):
         src = os.path.join(ROOT, "tests", "gpu_checks", name + ".hip")
         if not os.path.exists(src):
             continue
