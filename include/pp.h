@@ -165,6 +165,20 @@ pp_status pp_energy_snapshot(int n, const double *r, const double *O, const doub
                              double kf_cosmax, int32_t *bonds, uint8_t *overlap, long long *total_bonds,
                              int *any_overlap);
 
+/* pp_energy_snapshot without allocating: no counterpart in the reference;
+ * for callers that cannot allocate (code under stream capture, serving
+ * loops, harnesses that forbid hipMalloc).  It calls no hipMalloc or
+ * hipFree, and allocates no host memory except the error message of a
+ * failed call.  Same arguments, results and device (0) as
+ * pp_energy_snapshot, for 0 <= n <= PP_NOALLOC_MAX_N (PP_ERR_ARG beyond).
+ * Its device memory is one fixed array of about 16 MB per GPU, held until
+ * the process ends and shared by every call on that GPU: calls that overlap
+ * on one GPU race. */
+#define PP_NOALLOC_MAX_N 65536
+pp_status pp_energy_snapshot_noalloc(int n, const double *r, const double *O, const double box[3], double kf_delta,
+                                     double kf_cosmax, int32_t *bonds, uint8_t *overlap, long long *total_bonds,
+                                     int *any_overlap);
+
 /* Random sequential insertion of n non-overlapping unit spheres in a cubic
  * box of side (n/rho)^(1/3), each with a uniformly random orientation
  * (generator.c:43-84, the reference's generator program; host code, no GPU).

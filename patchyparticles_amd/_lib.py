@@ -22,6 +22,7 @@ EXPORTS = (
     "pp_net_create", "pp_net_set", "pp_net_get", "pp_net_forward", "pp_net_loss_grad", "pp_net_destroy",
     "pp_net_attach", "pp_net_train", "pp_get_learn_counters", "pp_net_descriptor",
     "pp_checkpoint_save", "pp_checkpoint_load", "pp_bond_list", "pp_set_lx_move",
+    "pp_energy_snapshot_noalloc",
 )
 
 # sizes of the proposal network (PP_NET_* in include/pp.h)
@@ -29,6 +30,9 @@ NET_INPUTS = 84
 NET_OUTPUTS = 26
 NET_WEIGHTS = 49152
 NET_BIASES = 416
+
+# PP_NOALLOC_MAX_N in include/pp.h
+NOALLOC_MAX_N = 65536
 
 PP_OK = 0
 STATUS = {
@@ -125,6 +129,8 @@ def lib():
     L.pp_energy_snapshot.argtypes = [C.c_int, _dp, _dp, _dp, C.c_double, C.c_double, _vp, _vp,
                                      C.POINTER(C.c_longlong), C.POINTER(C.c_int)]
     L.pp_energy_snapshot.restype = st
+    L.pp_energy_snapshot_noalloc.argtypes = L.pp_energy_snapshot.argtypes
+    L.pp_energy_snapshot_noalloc.restype = st
     L.pp_generate.argtypes = [C.c_int, C.c_double, C.c_long, _dp, _dp, _dp]
     L.pp_generate.restype = st
     L.pp_set_trace.argtypes = [_vp, C.c_int]
@@ -205,6 +211,21 @@ def energy_snapshot(r, O, box, kf_delta=0.12, kf_cosmax=0.92):
     anyo = C.c_int(0)
     check(lib().pp_energy_snapshot(n, r, O, _arr(box), kf_delta, kf_cosmax, _ptr(bonds), _ptr(ov),
                                    C.byref(tot), C.byref(anyo)))
+    return bonds[:n], ov[:n].astype(bool), tot.value, bool(anyo.value)
+
+
+def energy_snapshot_noalloc(r, O, box, kf_delta=0.12, kf_cosmax=0.92):
+    """energy_snapshot through the allocation-free entry point (at most
+    NOALLOC_MAX_N particles; the library allocates nothing for the call)."""
+    r = _arr(r)
+    O = _arr(O)
+    n = r.size // 3
+    bonds = np.zeros(max(n, 1), np.int32)
+    ov = np.zeros(max(n, 1), np.uint8)
+    tot = C.c_longlong(0)
+    anyo = C.c_int(0)
+    check(lib().pp_energy_snapshot_noalloc(n, r, O, _arr(box), kf_delta, kf_cosmax, _ptr(bonds), _ptr(ov),
+                                           C.byref(tot), C.byref(anyo)))
     return bonds[:n], ov[:n].astype(bool), tot.value, bool(anyo.value)
 
 

@@ -23,7 +23,7 @@ HOST = os.path.join(CSRC, "host")
 LIB = os.path.join(PKG, "libpp_hip.so")
 BIN = os.path.join(PKG, "bin")
 
-HIP_SOURCES = ["pp_kernels.hip", "pp_learn.hip", "pp_api.cpp", "pp_net.cpp", "pp_host.cpp"]
+HIP_SOURCES = ["pp_kernels.hip", "pp_learn.hip", "pp_api.cpp", "pp_net.cpp", "pp_host.cpp", "pp_noalloc.cpp"]
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",
     "-O3",
