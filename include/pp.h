@@ -36,12 +36,14 @@
  *   pp_generate                   the generator program's random insertion
  *                                 (generator.c:10-124)
  *   pp_autotune, pp_set_trace/pp_get_trace, pp_set_timing/pp_get_timing,
- *   pp_get_grid                   no reference counterpart (launch-shape
+ *   pp_get_grid, pp_get_layout    no reference counterpart (launch-shape
  *                                 tuning, per-sweep energy series, kernel
  *                                 timing, grid introspection)
  *   pp_bond_list                  output_print_bonds (output.c:213-268)
  *   pp_set_lx_move                the Lx_move keys and MC_change_Lx
  *                                 (system.c:170-196, MC.c:515-585)
+ *   pp_set_lx_acceptance          MC_change_Lx's acceptance test (MC.c:562),
+ *                                 or Metropolis in its place
  *   pp_checkpoint_save/_load     .rrr restart (output.c:37-46, system.c:213-245),
  *                                 exact and with the random-number counter
  *   pp_net_*, pp_get_learn_counters
@@ -217,6 +219,16 @@ pp_status pp_autotune(pp_ctx *ctx, int sweeps, int *threads_per_cell, int *block
  * as the reference's: Ly != Lz, lyz_min >= Ly, lyz_max <= Ly. */
 pp_status pp_set_lx_move(pp_ctx *ctx, int rep, double lx_change_max, double lyz_min, double lyz_max);
 
+/* The acceptance rule of the box-shape moves of replica rep (-1: all).
+ * PP_LX_METROPOLIS (the default): min(1, exp(-dE/T)), which keeps the
+ * Boltzmann distribution.  PP_LX_REFERENCE: the reference's own test,
+ * u < exp(+dE/T) with dE = E_new - E_old (MC_change_Lx, MC.c:562), for runs
+ * that must reproduce the reference program's box-shape statistics; it
+ * favours moves that raise the energy. */
+#define PP_LX_METROPOLIS 0
+#define PP_LX_REFERENCE 1
+pp_status pp_set_lx_acceptance(pp_ctx *ctx, int rep, int rule);
+
 /* Bonded partners of every particle of replica rep, the pairs of
  * output_print_bonds (output.c:213-268, MC_interact's test):
  * count[i] = number of particles bonded to particle i, and their indices at
@@ -235,6 +247,10 @@ pp_status pp_checkpoint_load(pp_ctx *ctx, const char *path);
 
 /* Grid of replica rep: cells per side and cell side lengths. */
 pp_status pp_get_grid(pp_ctx *ctx, int rep, int n_side[3], double side[3]);
+/* Cell layout of the context: the cells reserved over all replicas (NPT
+ * replicas reserve room for their grids to grow), and the tiles the
+ * rebuild's scan takes them in (scan_tile cells each, scan_tiles of them). */
+pp_status pp_get_layout(pp_ctx *ctx, long long *cell_capacity, int *scan_tile, int *scan_tiles);
 
 /* ---- learned proposal (no reference counterpart) --------------------------
  * A move of particle p: the descriptor of its neighbourhood (the 12 nearest

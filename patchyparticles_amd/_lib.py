@@ -22,7 +22,7 @@ EXPORTS = (
     "pp_net_create", "pp_net_set", "pp_net_get", "pp_net_forward", "pp_net_loss_grad", "pp_net_destroy",
     "pp_net_attach", "pp_net_train", "pp_get_learn_counters", "pp_net_descriptor",
     "pp_checkpoint_save", "pp_checkpoint_load", "pp_bond_list", "pp_set_lx_move",
-    "pp_energy_snapshot_noalloc",
+    "pp_energy_snapshot_noalloc", "pp_set_lx_acceptance", "pp_get_layout",
 )
 
 # sizes of the proposal network (PP_NET_* in include/pp.h)
@@ -145,6 +145,8 @@ def lib():
     L.pp_get_timing.restype = st
     L.pp_get_grid.argtypes = [_vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double)]
     L.pp_get_grid.restype = st
+    L.pp_get_layout.argtypes = [_vp, C.POINTER(C.c_longlong), C.POINTER(C.c_int), C.POINTER(C.c_int)]
+    L.pp_get_layout.restype = st
     fp = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
     L.pp_net_create.argtypes = [C.c_int, C.c_uint64, C.POINTER(_vp)]
     L.pp_net_create.restype = st
@@ -175,6 +177,8 @@ def lib():
     L.pp_bond_list.restype = st
     L.pp_set_lx_move.argtypes = [_vp, C.c_int, C.c_double, C.c_double, C.c_double]
     L.pp_set_lx_move.restype = st
+    L.pp_set_lx_acceptance.argtypes = [_vp, C.c_int, C.c_int]
+    L.pp_set_lx_acceptance.restype = st
     L.pp_destroy.argtypes = [_vp]
     L.pp_destroy.restype = None
     L.pp_strerror.argtypes = [st]

@@ -21,7 +21,9 @@
  * Initial_conditions_file then only gives N, normally the last.rrr written
  * with the checkpoint).  Reference keys beyond the ones above: Print_bonds /
  * Bonds_folder (bond lists) and Lx_move / Lx_change_max / Lyz_min / Lyz_max
- * (box-shape moves, boxshape.dat).  The learned proposal: Learned_fraction,
+ * (box-shape moves, boxshape.dat; accepted by Metropolis, with a warning at
+ * start-up, or with the reference's own test when this program's key
+ * Lx_reference_acceptance = 1).  The learned proposal: Learned_fraction,
  * Learned_weights, Learned_train_steps, Learned_train_batch,
  * Learned_pos_noise (see setup_learned).
  */
@@ -220,6 +222,17 @@ int main(int argc, char *argv[]) {
 			ppo_log(&u.out, "Ly and Lz are allowed to vary between %lf and %lf\n", lyz_min, lyz_max);
 		}
 	}
+	int lx_ref = 0;
+	if(lx) {
+		ppi_int(in, "Lx_reference_acceptance", &lx_ref, 0);
+		if(lx_ref)
+			ppo_log(&u.out, "Lx moves: accepted with exp(+dE/T) as the reference program does (MC.c:562); this rule "
+			                "does not sample the Boltzmann distribution\n");
+		else
+			ppo_log(&u.out, "WARNING: Lx moves are accepted with min(1, exp(-dE/T)) (Metropolis); the reference program "
+			                "accepts them with exp(+dE/T) (MC.c:562). Set Lx_reference_acceptance = 1 for the "
+			                "reference's rule\n");
+	}
 	u.out.lx = lx != 0;
 
 	pp_options opt;
@@ -232,7 +245,10 @@ int main(int argc, char *argv[]) {
 	const pp_status st = pp_create(&prm, &opt, c.n, c.r, c.O, c.box, &u.ctx);
 	if(st == PP_ERR_OVERLAP) ppo_exit(&u.out, "Initial configuration contains an overlap, aborting\n");
 	if(st != PP_OK) ppo_exit(&u.out, "pp_create failed: %s (%s)\n", pp_strerror(st), pp_last_error());
-	if(lx) check(&u, pp_set_lx_move(u.ctx, 0, lx_change_max, lyz_min, lyz_max), "pp_set_lx_move");
+	if(lx) {
+		check(&u, pp_set_lx_move(u.ctx, 0, lx_change_max, lyz_min, lyz_max), "pp_set_lx_move");
+		check(&u, pp_set_lx_acceptance(u.ctx, 0, lx_ref ? PP_LX_REFERENCE : PP_LX_METROPOLIS), "pp_set_lx_acceptance");
+	}
 	setup_learned(&u, in, &prm, &opt, &c);
 	pprrr_free(&c);
 	u.r = malloc(sizeof(double) * 3 * (size_t) (u.n > 0 ? u.n : 1));

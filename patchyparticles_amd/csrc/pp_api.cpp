@@ -10,6 +10,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -87,6 +88,7 @@ struct pp_ctx {
 	pp_net *net = nullptr;
 	double learned_fraction = 0.;
 	std::vector<void *> allocs;
+	int *h_vslow = nullptr;  // pinned: whether the box trial in flight needs a trial rebuild
 
 	~pp_ctx() {
 		for(auto &p : pending) {
@@ -94,6 +96,7 @@ struct pp_ctx {
 			(void) hipEventDestroy(p.second.second);
 		}
 		for(void *p : allocs) (void) hipFree(p);
+		if(h_vslow) (void) hipHostFree(h_vslow);
 		if(stream) (void) hipStreamDestroy(stream);
 	}
 };
@@ -406,13 +409,23 @@ pp_status create_impl(int R, const pp_params *prm, const pp_options *opts, const
 	   (st = dev_alloc(c, &d.rec, (size_t) 8 * std::max(d.total_act, 1) * REC_W)) ||
 	   (st = dev_alloc(c, &d.ctr, (size_t) R * NCTR * NSLOT)) || (st = dev_alloc(c, &d.err, 1)) ||
 	   (st = dev_alloc(c, &d.out_bonds, d.stride)) || (st = dev_alloc(c, &d.out_ovl, d.stride)) ||
-	   (st = dev_alloc(c, &d.bar, 2)))
+	   (st = dev_alloc(c, &d.bar, 2)) || (st = dev_alloc(c, &d.vflag, R)) || (st = dev_alloc(c, &d.vslow, 1)))
 		return bail(st);
+	if(hipHostMalloc((void **) &c->h_vslow, sizeof(int), hipHostMallocDefault) != hipSuccess || !c->h_vslow) {
+		c->h_vslow = nullptr;
+		return bail(fail(PP_ERR_NOMEM, "pinned host flag allocation failed"));
+	}
+	// a diagnostic switch: every box trial regroups the particles into the
+	// spare buffer (the form without the fast path, for comparisons)
+	{
+		const char *e = getenv("PP_NPT_FULL_TRIAL");
+		d.npt_full_trial = (e && *e && strcmp(e, "0") != 0) ? 1 : 0;
+	}
 	{
 		// every buffer a kernel may touch must exist before the first launch
 		const void *need[] = {d.f[0], d.f[1], d.pid[0], d.pid[1], d.reps, d.prep, d.act_off, d.count, d.start,
 		                      d.slot, d.cellid, d.scan_status, d.tile_sum, d.clist, d.ccount, d.pfx, d.rec, d.scan_ticket,
-		                      d.ctr, d.err, d.out_bonds, d.out_ovl, d.bar};
+		                      d.ctr, d.err, d.out_bonds, d.out_ovl, d.bar, d.vflag, d.vslow};
 		for(const void *q : need)
 			if(!q) return bail(fail(PP_ERR_NOMEM, "a device buffer was not allocated"));
 	}
@@ -440,6 +453,8 @@ pp_status create_impl(int R, const pp_params *prm, const pp_options *opts, const
 	launch_fill_i32(d.count, 0, d.C_total + 1, c->stream);
 	launch_fill_i32(d.err, 0, 1, c->stream);
 	launch_fill_i32((int *) d.bar, 0, 2, c->stream);
+	launch_fill_i32(d.vflag, 0, R, c->stream);
+	launch_fill_i32(d.vslow, 0, 1, c->stream);
 	launch_fill_u64(d.ctr, 0ull, (long long) R * NCTR * NSLOT, c->stream);
 
 	// initial energy (main.c:54-61): any overlap is an error
@@ -532,9 +547,15 @@ pp_status enqueue_sweep(pp_ctx *c) {
 			hipEvent_t v{};
 			tic(c, TK_VOLUME, &v);
 			launch_vol_propose(c->d, sw, k, c->stream);
-			launch_rebuild(c->d, REB_TRIAL, sw, -1, c->stream);
+			// a trial rebuild only when some replica's trial box cannot be
+			// served by its current lists (MC.c:481-485 rebuilds only when
+			// the cells become too small)
+			HIPCHECK(hipMemcpyAsync(c->h_vslow, c->d.vslow, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+			HIPCHECK(hipStreamSynchronize(c->stream));
+			if(*c->h_vslow) launch_rebuild(c->d, REB_TRIAL, sw, -1, c->stream);
 			launch_energy(c->d, 1, 0, c->stream);
 			launch_vol_decide(c->d, sw, k, c->stream);
+			launch_vol_apply(c->d, c->stream);
 			toc(c, TK_VOLUME, v);
 		}
 	}
@@ -1046,6 +1067,17 @@ pp_status pp_set_lx_move(pp_ctx *c, int rep, double lx_change_max, double lyz_mi
 	return push_reps(c);
 }
 
+pp_status pp_set_lx_acceptance(pp_ctx *c, int rep, int rule) {
+	if(!c) return fail(PP_ERR_ARG, "ctx is NULL");
+	if(rep < -1 || rep >= c->R) return fail(PP_ERR_ARG, "replica %d out of range", rep);
+	if(rule != PP_LX_METROPOLIS && rule != PP_LX_REFERENCE) return fail(PP_ERR_ARG, "unknown box-shape acceptance rule %d", rule);
+	HIPCHECK(hipSetDevice(c->device));
+	TRY(pull_reps(c));
+	const int lo = rep < 0 ? 0 : rep, hi = rep < 0 ? c->R : rep + 1;
+	for(int k = lo; k < hi; k++) c->reps[k].lx_ref_rule = rule == PP_LX_REFERENCE ? 1 : 0;
+	return push_reps(c);
+}
+
 pp_status pp_bond_list(pp_ctx *c, int rep, int max_per, int32_t *count, int32_t *partners) {
 	if(!c || !count || (max_per > 0 && !partners) || max_per < 0) return fail(PP_ERR_ARG, "bad arguments");
 	if(rep < 0 || rep >= c->R) return fail(PP_ERR_ARG, "replica %d out of range", rep);
@@ -1163,6 +1195,14 @@ pp_status pp_get_grid(pp_ctx *c, int rep, int n_side[3], double side[3]) {
 		if(n_side) n_side[a] = rp.g.n[a];
 		if(side) side[a] = rp.g.box[a] / rp.g.n[a];
 	}
+	return PP_OK;
+}
+
+pp_status pp_get_layout(pp_ctx *c, long long *cell_capacity, int *scan_tile, int *scan_tiles) {
+	if(!c) return fail(PP_ERR_ARG, "ctx is NULL");
+	if(cell_capacity) *cell_capacity = c->d.C_total;
+	if(scan_tile) *scan_tile = c->d.scan_tile;
+	if(scan_tiles) *scan_tiles = c->d.ntiles;
 	return PP_OK;
 }
 

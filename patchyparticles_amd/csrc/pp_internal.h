@@ -81,9 +81,9 @@ enum Err : int { ERR_CELL_CAP = 1, ERR_BOX = 2, ERR_SCAN = 8, ERR_BARRIER = 16 }
 // rebuild modes
 enum Reb : int { REB_KEEP = 0, REB_SHIFT = 1, REB_TRIAL = 2 };
 
-// grids of more large scan tiles than this are scanned in three steps (tile
-// sums, one scan of the sums, the tiles) instead of one pass with a
-// look-back, whose chains grow with the number of tiles in flight
+// a decoupled look-back serves at most this many tiles well (its chains grow
+// with the tiles in flight); grids of large tiles, which always have more,
+// are scanned in three steps (tile sums, one scan of the sums, the tiles)
 constexpr int SCAN_LOOKBACK_MAX_TILES = 64;
 
 // spin bound of the scan's look-back (a stuck predecessor is reported, not waited on forever)
@@ -124,6 +124,17 @@ struct Rep {
 	double trial_s[3];        // scale of each position component in the trial (1: unchanged)
 	int trial_kind;           // 0: volume move (NPT), 1: Lx move (NVT with box-shape moves)
 	int lx_move;              // box-shape moves on (MC_change_Lx, MC.c:515-585)
+	// The grid the cell lists of the current buffer were built on, in the
+	// current coordinates: membership by lg.  It differs from g after a box
+	// move accepted without a rebuild (the lists keep their members, lg is
+	// rescaled with the positions).  lists_ok = 0: the lists describe no
+	// grouping of the current buffer (a rejected trial rebuilt them into the
+	// spare one) until the next rebuild.
+	Grid lg;
+	int lists_ok;
+	int trial_fast;  // the trial in flight is evaluated on the current lists (no trial rebuild)
+	int vol_apply;   // an accepted fast trial: positions still to be rescaled in place
+	int lx_ref_rule;  // box-shape moves accepted with exp(+dE/T) as MC.c:562 (not Metropolis)
 	double lx_change_max, lyz_min, lyz_max;
 	double acc_tab[ACC_TAB];  // exp(-k / T): the acceptance factor of an energy rise of k bonds
 };
@@ -183,6 +194,9 @@ struct DevState {
 	long long *trace;  // [R][trace_cap] bonded pairs at the start of each sweep (optional)
 	int trace_cap;
 	uint64_t seed;
+	int *vflag;        // [R] a trial energy found an overlap: the replica's other teams stop
+	int *vslow;        // [1] some replica's trial needs a trial rebuild
+	int npt_full_trial;  // every box trial takes the trial rebuild (no fast path)
 };
 
 // launchers (pp_kernels.hip)
@@ -217,6 +231,8 @@ void launch_bond_list(const DevState &d, int max_per, int *count, int *partners,
 // volume-move attempt k of sweep `sweep` (NPT replicas only)
 void launch_vol_propose(const DevState &d, uint64_t sweep, int k, hipStream_t s);
 void launch_vol_decide(const DevState &d, uint64_t sweep, int k, hipStream_t s);
+// rescale the positions of replicas whose fast trial was accepted
+void launch_vol_apply(const DevState &d, hipStream_t s);
 void launch_gram_schmidt(const DevState &d, hipStream_t s);
 
 }  // namespace ppd

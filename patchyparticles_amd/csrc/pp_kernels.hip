@@ -74,6 +74,21 @@ __device__ __forceinline__ void sweep_shift(const DevState &d, int r, const Grid
 	sh[2] = u01(v.z) * (G.box[2] / (double) G.n[2]);
 }
 
+// Replica whose cell region holds global cell c.
+__device__ __forceinline__ int replica_of_cell(const DevState &d, int c) {
+	int lo = 0, hi = d.R;
+	while(hi - lo > 1) {
+		const int mid = (lo + hi) >> 1;
+		if(d.reps[mid].cell_off <= c) lo = mid;
+		else hi = mid;
+	}
+	return lo;
+}
+
+// A trial that regroups the replica's particles into the spare buffer (a
+// box trial whose grid the current lists cannot serve).
+__device__ __forceinline__ bool trial_rebuilds(const Rep &rp) { return rp.trial_active && !rp.trial_fast; }
+
 __global__ void k_count(DevState d, int mode, uint64_t sweep, long long trace_pos) {
 	int j = blockIdx.x * blockDim.x + threadIdx.x;
 	// reset the scan's look-back state for the scan that follows
@@ -82,10 +97,17 @@ __global__ void k_count(DevState d, int mode, uint64_t sweep, long long trace_po
 	if(j == 0) {
 		d.scan_ticket[0] = 0;
 	}
+	if(mode == REB_TRIAL)
+		// replicas without a trial rebuild keep their lists as they are:
+		// their cells' counts are the current populations, so the scan
+		// reproduces their starts
+		for(int c = j; c < d.C_total; c += gridDim.x * blockDim.x)
+			if(!trial_rebuilds(d.reps[replica_of_cell(d, c)])) d.count[c] = d.start[c + 1] - d.start[c];
 	if(j >= d.Ntot) return;
 	const int r = d.prep[j];
 	Rep &rp = d.reps[r];
-	const bool tr = (mode == REB_TRIAL) && rp.trial_active;
+	const bool tr = (mode == REB_TRIAL) && trial_rebuilds(rp);
+	if(mode == REB_TRIAL && !tr) return;
 	const Grid &G = tr ? rp.tg : rp.g;
 	double sh[3] = {G.shift[0], G.shift[1], G.shift[2]};
 	if(mode == REB_SHIFT) {
@@ -105,6 +127,12 @@ __global__ void k_count(DevState d, int mode, uint64_t sweep, long long trace_po
 			const uint32_t w4[4] = {v.x, v.y, v.z, v.w};
 			rp.col_perm = colour_perm_from(w4);
 		}
+	}
+	if(!tr && j == rp.p_off) {
+		// the lists the scan and scatter build are by membership in this grid
+		rp.lg = rp.g;
+		for(int k = 0; k < 3; k++) rp.lg.shift[k] = sh[k];
+		rp.lists_ok = 1;
 	}
 	const double *F = d.f[rp.sel];
 	const int S = d.stride;
@@ -387,7 +415,7 @@ __global__ void k_scatter(DevState d, int mode, uint64_t sweep) {
 	if(j >= d.Ntot) return;
 	const int r = d.prep[j];
 	const Rep &rp = d.reps[r];
-	const bool tr = (mode == REB_TRIAL) && rp.trial_active;
+	const bool tr = (mode == REB_TRIAL) && trial_rebuilds(rp);
 	if(mode == REB_TRIAL && !tr) return;
 	const int sb = tr ? rp.sel : (rp.sel ^ 1);
 	const double *F = d.f[sb];
@@ -415,12 +443,14 @@ void launch_rebuild(const DevState &d, int mode, uint64_t sweep, long long trace
 		hipLaunchKernelGGL(k_scan<SCAN_TILE_SMALL / 256>, dim3(d.ntiles), dim3(256), 0, s, d, flip, lists, 0);
 	}
 	else {
-		const int pre = d.ntiles > SCAN_LOOKBACK_MAX_TILES;
-		if(pre) {
-			hipLaunchKernelGGL(k_tile_sums<SCAN_TILE_LARGE / 256>, dim3(d.ntiles), dim3(256), 0, s, d);
-			hipLaunchKernelGGL(k_tile_scan, dim3(1), dim3(1024), 0, s, d);
-		}
-		hipLaunchKernelGGL(k_scan<SCAN_TILE_LARGE / 256>, dim3(d.ntiles), dim3(256), 0, s, d, flip, lists, pre);
+		// large tiles are taken only beyond SCAN_SMALL_MAX_TILES small ones,
+		// so they always come in more tiles than a look-back serves well:
+		// tile sums, one scan of the sums, then the tiles
+		static_assert(SCAN_SMALL_MAX_TILES * SCAN_TILE_SMALL / SCAN_TILE_LARGE > SCAN_LOOKBACK_MAX_TILES,
+		              "grids of large tiles always exceed the look-back's tile limit");
+		hipLaunchKernelGGL(k_tile_sums<SCAN_TILE_LARGE / 256>, dim3(d.ntiles), dim3(256), 0, s, d);
+		hipLaunchKernelGGL(k_tile_scan, dim3(1), dim3(1024), 0, s, d);
+		hipLaunchKernelGGL(k_scan<SCAN_TILE_LARGE / 256>, dim3(d.ntiles), dim3(256), 0, s, d, flip, lists, 1);
 	}
 	if(mode == REB_SHIFT) hipLaunchKernelGGL(k_pass_prefix, dim3(1), dim3(64), 0, s, d);
 	unsigned gs = gp;
@@ -1192,33 +1222,48 @@ struct NbLayout {
 	static constexpr int PER = (27 + STEP - 1) / STEP;
 };
 
+// trial = 1: the energy of the replicas' box trials.  A trial that rebuilt
+// its lists reads the spare buffer on the trial grid; a fast trial reads the
+// current buffer through the current lists (membership by the list grid,
+// computed from the unscaled positions exactly as the lists were built) and
+// scales every position as the trial does before it measures separations.
+// Once one team of a trial has met an overlap the trial is rejected
+// whatever the bonds are: the replica's later teams stop at once.
+// Workgroups reach the XCDs round-robin: each XCD takes a contiguous range of
+// slots (neighbouring cells share its L2).
 template <int T>
 __global__ __launch_bounds__(256) void k_energy(DevState d, int trial, int per_particle) {
 	using NB = NbLayout<T>;
 	const int lane = threadIdx.x & (T - 1);
 	const int team = threadIdx.x / T;
-	const int j = blockIdx.x * (blockDim.x / T) + team;
+	const int j = xcd_remap(blockIdx.x, gridDim.x) * (blockDim.x / T) + team;
 	if(j >= d.Ntot) return;
 	const int r = d.prep[j];
 	const Rep &rp = d.reps[r];
 	const bool tr = trial && rp.trial_active;
 	if(trial && !tr) return;
-	const Grid &G = tr ? rp.tg : rp.g;
-	const int b = tr ? (rp.sel ^ 1) : rp.sel;
+	if(tr && __hip_atomic_load(&d.vflag[r], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
+	const bool fast = tr && rp.trial_fast;
+	const Grid &G = tr ? (fast ? rp.lg : rp.tg) : rp.g;  // the grid of the lists' membership
+	const Grid &B = tr ? rp.tg : rp.g;                    // the box of the separations
+	const double s0 = fast ? rp.trial_s[0] : 1., s1 = fast ? rp.trial_s[1] : 1., s2 = fast ? rp.trial_s[2] : 1.;
+	const int b = (tr && !fast) ? (rp.sel ^ 1) : rp.sel;
 	const double *F = d.f[b];
 	const int S = d.stride;
-	const double px = F[j], py = F[S + j], pz = F[2 * S + j];
+	const double ux = F[j], uy = F[S + j], uz = F[2 * S + j];
 	double O[9];
 #pragma unroll
 	for(int c = 0; c < 9; c++) O[c] = F[(3 + c) * S + j];
 	double P[4][3];
 	kf_patches(O, P);
 	const int n0 = G.n[0], n1 = G.n[1], n2 = G.n[2];
-	const int ix = cell_coord(px, G.shift[0], G.inv_box[0], n0);
-	const int iy = cell_coord(py, G.shift[1], G.inv_box[1], n1);
-	const int iz = cell_coord(pz, G.shift[2], G.inv_box[2], n2);
-	const double Lx = G.box[0], Ly = G.box[1], Lz = G.box[2];
-	const double iLx = G.inv_box[0], iLy = G.inv_box[1], iLz = G.inv_box[2];
+	const int ix = cell_coord(ux, G.shift[0], G.inv_box[0], n0);
+	const int iy = cell_coord(uy, G.shift[1], G.inv_box[1], n1);
+	const int iz = cell_coord(uz, G.shift[2], G.inv_box[2], n2);
+	// (x * 1 is x: unscaled replicas are unchanged)
+	const double px = ux * s0, py = uy * s1, pz = uz * s2;
+	const double Lx = B.box[0], Ly = B.box[1], Lz = B.box[2];
+	const double iLx = B.inv_box[0], iLy = B.inv_box[1], iLz = B.inv_box[2];
 	const double c2 = rp.cosmax >= KF_FILTER_MIN_COS ? rp.cosmax * rp.cosmax : -1.;
 	int nbond = 0;
 	bool ov = false;
@@ -1231,9 +1276,9 @@ __global__ __launch_bounds__(256) void k_energy(DevState d, int trial, int per_p
 		int s = d.start[cc], e = d.start[cc + 1];
 		for(int q = s + lane % NB::S; q < e; q += NB::S) {
 			if(q == j) continue;
-			double dx = kf_min_image(F[q] - px, Lx, iLx);
-			double dy = kf_min_image(F[S + q] - py, Ly, iLy);
-			double dz = kf_min_image(F[2 * S + q] - pz, Lz, iLz);
+			double dx = kf_min_image(F[q] * s0 - px, Lx, iLx);
+			double dy = kf_min_image(F[S + q] * s1 - py, Ly, iLy);
+			double dz = kf_min_image(F[2 * S + q] * s2 - pz, Lz, iLz);
 			double d2 = dx * dx + dy * dy + dz * dz;
 			if(d2 < 1.) ov = true;
 			else if(d2 < rp.rc2) {
@@ -1269,6 +1314,7 @@ __global__ __launch_bounds__(256) void k_energy(DevState d, int trial, int per_p
 	nbond = team_sum<T>(nbond);
 	const bool any = team_any<T>(ov);
 	if(lane == 0) {
+		if(tr && any) __hip_atomic_store(&d.vflag[r], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 		int res = any ? 0 : nbond;
 		if(per_particle) {
 			int i = rp.p_off + d.pid[b][j];
@@ -1338,24 +1384,40 @@ void launch_bond_list(const DevState &d, int max_per, int *count, int *partners,
 // most of a 32-lane team would idle.  The result does not depend on it.
 void launch_energy(const DevState &d, int trial, int per_particle, hipStream_t s) {
 	hipLaunchKernelGGL(k_zero_energy_rows, dim3((d.R * 2 * NSLOT + 255) / 256), dim3(256), 0, s, d.ctr, d.R);
+	if(d.Ntot <= 0) return;
 	const int block = 256;
 	const double cand = 27. * (double) d.Ntot / (double) (d.C_total > 0 ? d.C_total : 1);
-	if(cand < 8.)
-		hipLaunchKernelGGL(k_energy<8>, dim3((d.Ntot + block / 8 - 1) / (block / 8)), dim3(block), 0, s, d, trial,
-		                   per_particle);
-	else
-		hipLaunchKernelGGL(k_energy<32>, dim3((d.Ntot + block / 32 - 1) / (block / 32)), dim3(block), 0, s, d, trial,
-		                   per_particle);
+	// a multiple of 8 workgroups: one contiguous range of slots per XCD
+	auto grid = [&](int per) { return dim3((unsigned) ((((long long) d.Ntot + per - 1) / per + 7) & ~7LL)); };
+	if(cand < 8.) hipLaunchKernelGGL(k_energy<8>, grid(block / 8), dim3(block), 0, s, d, trial, per_particle);
+	else hipLaunchKernelGGL(k_energy<32>, grid(block / 32), dim3(block), 0, s, d, trial, per_particle);
 }
 
 // ------------------------------------------------------------------------
 // NPT volume move (MC.c:450-513): log-volume step along one random axis
+
+// Whether the trial of a replica can be evaluated on its current lists: the
+// lists' cells, rescaled with the trial, stay at least side_min wide on
+// every axis, so every pair within the bond range still lies in adjacent
+// cells (the margin in side_min covers the roundings of the membership).
+// Otherwise (or when asked for every trial) the trial regroups the particles
+// on its own grid into the spare buffer; the host launches that rebuild only
+// when some replica needs it.
+__device__ __forceinline__ void choose_trial_path(const DevState &d, Rep &rp, const double tb[3]) {
+	bool fast = !d.npt_full_trial && rp.lists_ok;
+	for(int a = 0; a < 3; a++) fast = fast && tb[a] / (double) rp.lg.n[a] >= rp.side_min;
+	rp.trial_fast = fast ? 1 : 0;
+	if(!fast) atomicOr(d.vslow, 1);
+}
 
 __global__ void k_vol_propose(DevState d, uint64_t sweep, int k) {
 	int r = blockIdx.x * blockDim.x + threadIdx.x;
 	if(r >= d.R) return;
 	Rep &rp = d.reps[r];
 	rp.trial_active = 0;
+	rp.trial_fast = 0;
+	rp.vol_apply = 0;
+	d.vflag[r] = 0;
 	if(rp.ensemble != 5 && !rp.lx_move) return;
 	u32x4 v = rng(d.seed, r, sweep, (uint32_t) k, TAG_VOLUME, 0, 0);  // attempt k of the sweep
 	if(rp.ensemble != 5) {
@@ -1379,6 +1441,7 @@ __global__ void k_vol_propose(DevState d, uint64_t sweep, int k) {
 		}
 		rp.trial_kind = 1;
 		rp.trial_active = 1;
+		choose_trial_path(d, rp, tb);
 		return;
 	}
 	const int dir = (int) (u01(v.x) * 3.);
@@ -1410,6 +1473,7 @@ __global__ void k_vol_propose(DevState d, uint64_t sweep, int k) {
 	rp.trial_V0 = V0;
 	rp.trial_kind = 0;
 	rp.trial_active = 1;
+	choose_trial_path(d, rp, tb);
 }
 
 void launch_vol_propose(const DevState &d, uint64_t sweep, int k, hipStream_t s) {
@@ -1431,24 +1495,57 @@ __global__ void k_vol_decide(DevState d, uint64_t sweep, int k) {
 	const long long B0 = rp.bonds_base + (long long) db;
 	const double dE = (double) (B0 - B1);  // E' - E with E = -bonds
 	// volume: MC.c:497-499; box shape: exp(-dE/T) at constant volume (the
-	// Jacobian of the rescaling is 1)
+	// Jacobian of the rescaling is 1), or, on request, the reference's
+	// exp(+dE/T) (MC.c:562), which does not keep the Boltzmann distribution
 	const double arg = rp.trial_kind == 1
-	                       ? -dE / rp.T
+	                       ? (rp.lx_ref_rule ? dE / rp.T : -dE / rp.T)
 	                       : -(rp.P * (rp.trial_V - rp.trial_V0) + dE) / rp.T + (double) (rp.np + 1) * log(rp.trial_f);
 	u32x4 v = rng(d.seed, r, sweep, (uint32_t) k, TAG_VOLUME, 0, 0);  // attempt k of the sweep
 	atomicAdd(ctr_at(d.ctr, r, C_TRY_VOL, 0), 1ull);
 	if(ov == 0 && u01(v.z) < exp(arg)) {
-		rp.sel ^= 1;
+		if(rp.trial_fast) {
+			// the positions are rescaled in place (k_vol_apply); the lists keep
+			// their members, their grid is rescaled with the positions
+			rp.vol_apply = 1;
+			set_grid_box(rp.lg, rp.tg.box);
+			for(int a = 0; a < 3; a++) rp.lg.shift[a] *= rp.trial_s[a];
+		}
+		else {
+			// the spare buffer holds the trial, grouped on the trial grid
+			rp.sel ^= 1;
+			rp.lg = rp.tg;
+			rp.lists_ok = 1;
+		}
 		rp.g = rp.tg;
 		if(rp.trial_kind == 0) rp.V = rp.trial_V;
 		rp.bonds_base = B1 - (long long) db;
 		atomicAdd(ctr_at(d.ctr, r, C_ACC_VOL, 0), 1ull);
 	}
+	else if(!rp.trial_fast)
+		rp.lists_ok = 0;  // the lists now group the spare buffer, not the current one
 	rp.trial_active = 0;
 }
 
 void launch_vol_decide(const DevState &d, uint64_t sweep, int k, hipStream_t s) {
 	hipLaunchKernelGGL(k_vol_decide, dim3((d.R + 63) / 64), dim3(64), 0, s, d, sweep, k);
+}
+
+// The accepted fast trials: every position component times its trial scale,
+// as the trial rebuild writes it (x * 1 is x).  Clears the slow-trial flag
+// for the next attempt.
+__global__ void k_vol_apply(DevState d) {
+	const int j = blockIdx.x * blockDim.x + threadIdx.x;
+	if(j == 0) *d.vslow = 0;
+	if(j >= d.Ntot) return;
+	const Rep &rp = d.reps[d.prep[j]];
+	if(!rp.vol_apply) return;
+	double *F = d.f[rp.sel];
+	const int S = d.stride;
+	for(int c = 0; c < 3; c++) F[(size_t) c * S + j] *= rp.trial_s[c];
+}
+
+void launch_vol_apply(const DevState &d, hipStream_t s) {
+	hipLaunchKernelGGL(k_vol_apply, dim3((unsigned) std::max(1, (d.Ntot + 255) / 256)), dim3(256), 0, s, d);
 }
 
 // ------------------------------------------------------------------------

@@ -150,6 +150,12 @@ class System:
         check(lib().pp_get_grid(self._h, rep, n, s))
         return list(n), list(s)
 
+    def layout(self):
+        """(cells reserved over all replicas, cells per scan tile, scan tiles)."""
+        cap, tile, tiles = C.c_longlong(0), C.c_int(0), C.c_int(0)
+        check(lib().pp_get_layout(self._h, C.byref(cap), C.byref(tile), C.byref(tiles)))
+        return cap.value, tile.value, tiles.value
+
     def set_trace(self, capacity):
         """Record the bonded-pair count after each of the next `capacity` sweeps."""
         self._trace_cap = int(capacity)
@@ -182,10 +188,13 @@ class System:
         check(lib().pp_get_timing(self._h, None if name is None else name.encode(), C.byref(nl), C.byref(ms)))
         return nl.value, ms.value
 
-    def set_lx_move(self, lx_change_max, lyz_min, lyz_max, rep=-1):
+    def set_lx_move(self, lx_change_max, lyz_min, lyz_max, rep=-1, reference_rule=False):
         """Box-shape moves at constant volume (NVT): Lx += U(-1/2, 1/2)
-        lx_change_max, Ly = Lz = sqrt(V / Lx) within [lyz_min, lyz_max]."""
+        lx_change_max, Ly = Lz = sqrt(V / Lx) within [lyz_min, lyz_max],
+        accepted with min(1, exp(-dE/T)), or with the reference's exp(+dE/T)
+        (MC.c:562) when ``reference_rule``."""
         check(lib().pp_set_lx_move(self._h, int(rep), float(lx_change_max), float(lyz_min), float(lyz_max)))
+        check(lib().pp_set_lx_acceptance(self._h, int(rep), 1 if reference_rule else 0))
 
     def bond_list(self, rep=0, max_per=16):
         """Bonded partners of every particle: a list of sorted index arrays."""

@@ -67,11 +67,13 @@ def test_sweeps_track_energy_and_match_oracle(pp, oracle):
         assert tot > 0
 
 
-def test_config2_snapshot_parity(pp, oracle):
+@pytest.mark.parametrize("trials_per_cell", [0, 16])
+def test_config2_snapshot_parity(pp, oracle, trials_per_cell):
     """N=32768, rho=0.5, T=0.12: snapshot bonds/overlaps bit-exact against the
-    oracle after 1000 sweeps."""
+    oracle after 1000 sweeps, with one trial per member and with the
+    benchmarked form (16 trials per non-empty cell, kept bonds)."""
     r, O, box = _start(oracle, 32768, 0.5, 1)
-    with pp.System(r, O, box, T=0.12, seed=7) as s:
+    with pp.System(r, O, box, T=0.12, seed=7, trials_per_cell=trials_per_cell) as s:
         s.do_ensemble(1000)
         rr, OO, bb = s.state()
         b, ov, tot, anyo = s.energy()
@@ -283,25 +285,30 @@ def test_uneven_batch(pp, oracle, trials_per_cell):
         assert (c["tries"][:, 0] > 0).all()
 
 
-@pytest.mark.parametrize("ensemble", ["nvt", "npt"])
-@pytest.mark.parametrize("n,cells_lo,cells_hi", [(4000, 0, 128 * 512), (8000, 128 * 512, 320 * 512),
-                                                 (20000, 320 * 512, 1 << 30)])
-def test_large_grid_scan(pp, oracle, ensemble, n, cells_lo, cells_hi):
-    """Dilute systems (rho=0.05) whose cell grids take each form of the
-    rebuild's scan: 512-cell tiles whose look-back reaches back at most 64
-    tiles (N=4000: 38^3 cells, 108 tiles) or further, up to the limit of 320
-    tiles (N=8000: 48^3, 216 tiles), and 2048-cell tiles in three steps
-    (N=20000: 64^3).  The tracked energy equals a
-    full recompute, the cell populations check, and the energies of the
-    evolved snapshot equal the oracle's bit for bit (NPT: through trial
-    rebuilds and accepted volume moves)."""
+@pytest.mark.parametrize("ensemble,n,tile,tiles_lo,tiles_hi", [
+    ("nvt", 1500, 512, 1, 64),      # 512-cell tiles, a look-back over all of them (26^3 cells, 35 tiles)
+    ("nvt", 4000, 512, 65, 320),    # 512-cell tiles beyond 64 (38^3 cells, 108 tiles)
+    ("nvt", 20000, 2048, 65, 1 << 30),  # 2048-cell tiles in three steps (64^3 cells, 128 tiles)
+    ("npt", 1000, 512, 1, 64),      # NPT grids reserve room to grow: 24^3 -> 28^3 cells, 43 tiles
+    ("npt", 3000, 512, 65, 320),    # 34^3 -> 38^3 cells, 108 tiles
+    ("npt", 10000, 2048, 65, 1 << 30)])  # 52^3 -> 58^3 cells, 96 tiles of 2048
+def test_large_grid_scan(pp, oracle, ensemble, n, tile, tiles_lo, tiles_hi):
+    """Dilute systems (rho=0.05) whose cell layouts take each form of the
+    rebuild's scan: 512-cell tiles with a look-back over at most 64 tiles or
+    over more, up to the limit of 320 tiles, and 2048-cell tiles in three
+    steps.  The form follows from the cells the context reserves (NPT
+    replicas reserve room for their grids to grow), which the test reads
+    back.  The tracked energy equals a full recompute, the cell populations
+    check, and the energies of the evolved snapshot equal the oracle's bit
+    for bit (NPT: through trial rebuilds and accepted volume moves)."""
     r, O, box = _start(oracle, n, 0.05, 8)
     prm = dict(T=0.11)
     if ensemble == "npt":
         prm = dict(T=0.11, ensemble=pp.NPT, P=0.02, rescale_factor_max=0.002)
     with pp.System(r, O, box, seed=5, trials_per_cell=4, **prm) as s:
-        nside, _ = s.grid()
-        assert cells_lo < np.prod(nside) <= cells_hi
+        cap, got_tile, tiles = s.layout()
+        assert got_tile == tile and tiles_lo <= tiles <= tiles_hi, (cap, got_tile, tiles)
+        assert tiles == -(-cap // tile)
         s.do_ensemble(60)
         c = s.counters()
         b, ov, tot, anyo = s.energy()

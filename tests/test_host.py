@@ -314,6 +314,13 @@ def test_driver_lx_move_outputs(host_built, tmp_path):
     assert shape[-1][1] != shape[0][1]
     acc = (tmp_path / "acceptance.dat").read_text().splitlines()
     assert all(len(ln.split()) == 3 for ln in acc[1:])
+    # the acceptance rule differs from the reference's (MC.c:562): said at start-up
+    assert "WARNING: Lx moves are accepted with min(1, exp(-dE/T))" in res.stderr
+    # and the reference's own rule on request
+    _write_input(tmp_path, dict(keys, Lx_reference_acceptance=1))
+    res = subprocess.run([DRIVER, "input"], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert res.returncode == 0, res.stderr
+    assert "WARNING" not in res.stderr and "as the reference program does (MC.c:562)" in res.stderr
 
 
 @pytest.mark.gpu
