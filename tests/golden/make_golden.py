@@ -28,6 +28,16 @@ executables.  Outputs are data only (inputs and expected outputs):
     python tests/golden/make_golden.py stats      (stats_ref.json only)
     python tests/golden/make_golden.py bonds      (the bond-list files only)
     python tests/golden/make_golden.py batch      (batch_ref.json only)
+    python tests/golden/make_golden.py c1         (c1_ref.json only)
+    python tests/golden/make_golden.py cold       (cold_ref.json only)
+    python tests/golden/make_golden.py lx         (lx_ref.json only)
+
+  c1_ref.json       E/N series of 8 reference runs at the reference-runnable
+                    configuration (N=1000, rho=0.1, T=0.1, 10^5 sweeps)
+  cold_ref.json     the same for 8 state points of the batched workload's
+                    distribution, down to T=0.08
+  lx_ref.json       <E/N> and <Lx> of 8 reference runs with box-shape moves
+                    (Lx_move = 1) under the reference's acceptance test
 """
 from __future__ import annotations
 
@@ -450,7 +460,133 @@ def stats_ref():
         json.dump(out, f, indent=1)
 
 
+# The state points of the configurations themselves, where the system keeps
+# relaxing over a whole run: the reference-runnable configuration (N=1000,
+# rho=0.1, T=0.1, 10^5 sweeps from the generator's start) and 8 points drawn
+# from the batched workload's distribution (N log-uniform in [500, 8000], T
+# uniform in [0.08, 0.20], rho uniform in [0.05, 0.60]; this draw holds
+# three points below T=0.1).  The whole E/N series of every run is kept, so
+# the GPU can be compared with it at matched attempt counts.
+SERIES_RUNS, SERIES_SWEEPS, SERIES_PRINT = 8, 100000, 1000
+C1_POINT = dict(n=1000, rho=0.1, T=0.1)
+
+
+def cold_points():
+    rng = np.random.default_rng(20268)
+    n = np.exp(rng.uniform(np.log(500), np.log(8000), 8)).astype(int)
+    T = rng.uniform(0.08, 0.20, 8)
+    rho = rng.uniform(0.05, 0.60, 8)
+    return [dict(n=int(n[k]), T=round(float(T[k]), 4), rho=round(float(rho[k]), 4)) for k in range(8)]
+
+
+def series_ref(points, out_name, seed0):
+    """SERIES_RUNS runs of the reference program (production flags) per
+    point from its generator's start; energy.dat of every run."""
+    build_ref()
+    exe = os.path.join(ROOT, "oracle", "_ref", "PatchyParticles_fast")
+    jobs = []
+    for k, sp in enumerate(points):
+        d0 = tempfile.mkdtemp(prefix=f"series{k}_")
+        subprocess.run([os.path.join(ROOT, "oracle", "_ref", "generator"), str(sp["n"]), str(sp["rho"])], cwd=d0,
+                       check=True, capture_output=True)
+        for j in range(SERIES_RUNS):
+            d = os.path.join(d0, f"run{j}")
+            os.makedirs(d)
+            keys = {"Dynamics": 0, "Ensemble": 0, "Seed": seed0 + 100 * k + j, "Temperature": sp["T"],
+                    "Disp_max": 0.05, "Theta_max": 0.1, "KF_delta": 0.12, "KF_cosmax": 0.92,
+                    "Initial_conditions_file": os.path.join(d0, "generated.rrr"), "Steps": SERIES_SWEEPS,
+                    "Print_every": SERIES_PRINT, "Save_every": SERIES_SWEEPS, "Configuration_folder": "."}
+            with open(os.path.join(d, "input"), "w") as f:
+                f.write("".join(f"{a} = {b}\n" for a, b in keys.items()))
+            jobs.append((sp["n"], k, d))
+    running = []
+    for _, k, d in sorted(jobs, key=lambda j: -j[0]):  # longest first, one run per core
+        while len(running) >= (os.cpu_count() or 4):
+            for p in list(running):
+                if p.poll() is not None:
+                    assert p.returncode == 0
+                    running.remove(p)
+            if len(running) >= (os.cpu_count() or 4):
+                running[0].wait()
+        running.append(subprocess.Popen([exe, "input"], cwd=d, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL))
+    for p in running:
+        assert p.wait() == 0
+    out = []
+    for k, sp in enumerate(points):
+        series = []
+        for _, kk, d in jobs:
+            if kk == k:
+                e = np.loadtxt(os.path.join(d, "energy.dat"))
+                assert len(e) == SERIES_SWEEPS // SERIES_PRINT  # steps 0, P, ..., Steps - P
+                series.append(e[:, 1].tolist())
+        half = np.array(series)[:, len(series[0]) // 2:].mean(axis=1)
+        out.append(dict(sp, E_series=series, E_late=float(half.mean()),
+                        E_late_se=float(half.std(ddof=1) / np.sqrt(len(half)))))
+        print(out_name, k, sp, "late E/N", out[-1]["E_late"], "+-", out[-1]["E_late_se"])
+    with open(os.path.join(HERE, out_name), "w") as f:
+        json.dump(dict(sweeps=SERIES_SWEEPS, print_every=SERIES_PRINT, runs=SERIES_RUNS, disp_max=0.05,
+                       theta_max=0.1, points=out), f)
+
+
+# Box-shape moves (Lx_move = 1) with the reference's own acceptance test
+# (MC.c:562): N=300 from the generator's start at rho=0.3, T=0.2, Ly = Lz
+# allowed within 10 % of the start, Lx_change_max 0.5; energy.dat and
+# boxshape.dat of 8 runs.
+LX_POINT = dict(n=300, rho=0.3, T=0.2, lx_change_max=0.5, lyz_rel=0.1)
+LX_RUNS, LX_SWEEPS, LX_PRINT = 8, 60000, 100
+
+
+def lx_ref():
+    build_ref()
+    exe = os.path.join(ROOT, "oracle", "_ref", "PatchyParticles_fast")
+    sp = LX_POINT
+    d0 = tempfile.mkdtemp(prefix="lx_")
+    subprocess.run([os.path.join(ROOT, "oracle", "_ref", "generator"), str(sp["n"]), str(sp["rho"])], cwd=d0,
+                   check=True, capture_output=True)
+    with open(os.path.join(d0, "generated.rrr")) as f:
+        L = float(f.readline().split()[2])
+    lyz = (L * (1 - sp["lyz_rel"]), L * (1 + sp["lyz_rel"]))
+    procs = []
+    for j in range(LX_RUNS):
+        d = os.path.join(d0, f"run{j}")
+        os.makedirs(d)
+        keys = {"Dynamics": 0, "Ensemble": 0, "Seed": 7000 + j, "Temperature": sp["T"], "Disp_max": 0.05,
+                "Theta_max": 0.1, "KF_delta": 0.12, "KF_cosmax": 0.92, "Lx_move": 1,
+                "Lx_change_max": sp["lx_change_max"], "Lyz_min": lyz[0], "Lyz_max": lyz[1],
+                "Initial_conditions_file": os.path.join(d0, "generated.rrr"), "Steps": LX_SWEEPS,
+                "Print_every": LX_PRINT, "Save_every": LX_SWEEPS, "Configuration_folder": "."}
+        with open(os.path.join(d, "input"), "w") as f:
+            f.write("".join(f"{a} = {b}\n" for a, b in keys.items()))
+        procs.append(subprocess.Popen([exe, "input"], cwd=d, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL))
+    for p in procs:
+        assert p.wait() == 0
+    e_runs, lx_runs, acc = [], [], []
+    for j in range(LX_RUNS):
+        d = os.path.join(d0, f"run{j}")
+        e = np.loadtxt(os.path.join(d, "energy.dat"))[:, 1]
+        b = np.loadtxt(os.path.join(d, "boxshape.dat"))[:, 1]
+        e_runs.append(e[int(STAT_DISCARD * len(e)):].mean())
+        lx_runs.append(b[int(STAT_DISCARD * len(b)):].mean())
+    e_runs, lx_runs = np.array(e_runs), np.array(lx_runs)
+    out = dict(sp, box0=L, lyz_min=lyz[0], lyz_max=lyz[1], runs=LX_RUNS, sweeps=LX_SWEEPS, print_every=LX_PRINT,
+               discard=STAT_DISCARD, E_per_N=float(e_runs.mean()),
+               E_per_N_se=float(e_runs.std(ddof=1) / np.sqrt(LX_RUNS)), Lx=float(lx_runs.mean()),
+               Lx_se=float(lx_runs.std(ddof=1) / np.sqrt(LX_RUNS)), E_runs=e_runs.tolist(), Lx_runs=lx_runs.tolist())
+    print("lx", {k: v for k, v in out.items() if not k.endswith("_runs")})
+    with open(os.path.join(HERE, "lx_ref.json"), "w") as f:
+        json.dump(out, f, indent=1)
+
+
 if __name__ == "__main__":
+    if sys.argv[1:] == ["lx"]:
+        lx_ref()
+        sys.exit(0)
+    if sys.argv[1:] == ["c1"]:
+        series_ref([C1_POINT], "c1_ref.json", 3000)
+        sys.exit(0)
+    if sys.argv[1:] == ["cold"]:
+        series_ref(cold_points(), "cold_ref.json", 5000)
+        sys.exit(0)
     if sys.argv[1:] == ["host"]:
         host_runs()
     elif sys.argv[1:] == ["stats"]:

@@ -6,9 +6,12 @@ errors, and NPT hard spheres against the Carnahan-Starling equation of state.
 The GPU side runs independent replicas of the same start in one batched
 context; each replica's mean over its samples after the reference's
 discarded fraction is one independent estimate, as each reference run's is.
-The state points equilibrate well within the discarded part (T = 0.2);
-colder ones keep relaxing for 10^5+ sweeps at N = 300, where a comparison of
-running means would compare relaxation speeds instead."""
+The T = 0.2 state points equilibrate well within the discarded part.  The
+configurations' own state points (the reference-runnable N=1000, rho=0.1,
+T=0.1 and cold points of the batched workload) keep relaxing over 10^5
+sweeps: there the GPU copies and the reference runs are compared at matched
+attempt counts, window by window, which checks that both samplers relax the
+same way per attempt as well as where they end."""
 import math
 import os
 import subprocess
@@ -198,3 +201,86 @@ def test_box_shape_moves_keep_the_reference_energy(pp, tmp_path):
     m, se = _mean_se(e[int(ref["discard"] * len(e)):].mean(axis=0))
     tol = 3 * math.hypot(se, ref["E_per_N_se"])
     assert abs(m - ref["E_per_N"]) < tol, (m, se, ref["E_per_N"], ref["E_per_N_se"])
+
+
+def _series(pp, tmp_path, points, copies, sweeps, every, seed):
+    """E/N of `copies` independent copies of each point's reference start
+    (generator_hip writes the reference generator's file), all in one batched
+    context, sampled as the reference prints: after 0, every, 2 every, ...
+    sweeps (main.c:78-93).  One trial per member, so a sweep is N attempts
+    as in do_NVT (MC.c:19-29).  Returns (samples, points, copies)."""
+    rs, Os, bs, ps, ns = [], [], [], [], []
+    for k, sp in enumerate(points):
+        d = tmp_path / f"start{k}"
+        d.mkdir()
+        r, O, box = _reference_start(pp, d, sp["n"], sp["rho"])
+        for _ in range(copies):
+            rs.append(r)
+            Os.append(O)
+            bs.append(box)
+            ps.append(dict(T=sp["T"]))
+            ns.append(sp["n"])
+    ns = np.array(ns)
+    out = []
+    with pp.System.batch(rs, Os, bs, ps, seed=seed) as s:
+        for _ in range(sweeps // every):
+            out.append(-s.counters_all()["bonds"] / ns)
+            s.do_ensemble(every)
+        for k in range(s.R):
+            assert not s.energy(k)[3]
+    return np.array(out).reshape(len(out), len(points), copies)
+
+
+def _windows(ref_series, gpu):
+    """Per window (first and second half of the run, at matched attempt
+    counts): the reference's mean and standard error over its runs, the
+    GPU's over its copies."""
+    ref = np.array(ref_series)  # (runs, samples)
+    h = ref.shape[1] // 2
+    out = []
+    for lo, hi in ((0, h), (h, 2 * h)):
+        rm = ref[:, lo:hi].mean(axis=1)
+        gm = gpu[lo:hi].mean(axis=0)
+        out.append((_mean_se(rm), _mean_se(gm)))
+    return out
+
+
+def test_reference_runnable_configuration_matches_reference_runs(pp, tmp_path):
+    """The reference-runnable configuration itself: N=1000, rho=0.1, T=0.1,
+    Disp_max 0.05, Theta_max 0.1, 10^5 sweeps from the reference generator's
+    start, E/N printed every 1000 sweeps.  The system is still aggregating at
+    the end (E/N falls from 0 to about -1.2), so the comparison is at matched
+    attempt counts: over the first and over the second half of the run, the
+    mean E/N of 32 GPU copies and of 8 reference runs (tests/golden/
+    c1_ref.json) agree within 3 combined standard errors."""
+    ref = golden("c1_ref.json")
+    sp = ref["points"][0]
+    gpu = _series(pp, tmp_path, [sp], 32, ref["sweeps"], ref["print_every"], seed=77)[:, 0, :]
+    for (rm, rse), (gm, gse) in _windows(sp["E_series"], gpu):
+        assert abs(gm - rm) < 3 * math.hypot(rse, gse), (gm, gse, rm, rse)
+
+
+def test_box_shape_moves_with_the_reference_rule_match_reference_runs(pp, tmp_path):
+    """Box-shape moves under the reference's own acceptance test, u <
+    exp(+dE/T) (MC_change_Lx, MC.c:562), against the reference program's
+    runs with Lx_move = 1 (tests/golden/lx_ref.json: N=300, rho=0.3, T=0.2,
+    Ly = Lz within 10 % of the start): <E/N> and <Lx> of 32 GPU copies over
+    the second half of the run within 3 combined standard errors of the
+    reference's."""
+    ref = golden("lx_ref.json")
+    r, O, box = _reference_start(pp, tmp_path, ref["n"], ref["rho"])
+    R = 32
+    with pp.System.batch([r] * R, [O] * R, [box] * R, [dict(T=ref["T"])] * R, seed=2026) as s:
+        s.set_lx_move(ref["lx_change_max"], ref["lyz_min"], ref["lyz_max"], reference_rule=True)
+        e, lx = [], []
+        for _ in range(ref["sweeps"] // ref["print_every"]):
+            c = s.counters_all()
+            e.append(-c["bonds"] / ref["n"])
+            lx.append(c["box"][:, 0])
+            s.do_ensemble(ref["print_every"])
+        c = s.counters_all()
+        assert c["tries"][:, 1].sum() > 0
+    cut = int(ref["discard"] * len(e))
+    for got, want, want_se in ((np.array(e), ref["E_per_N"], ref["E_per_N_se"]), (np.array(lx), ref["Lx"], ref["Lx_se"])):
+        m, se = _mean_se(got[cut:].mean(axis=0))
+        assert abs(m - want) < 3 * math.hypot(se, want_se), (m, se, want, want_se)

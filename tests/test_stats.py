@@ -6,7 +6,8 @@ import math
 import numpy as np
 import pytest
 
-from patchyparticles_amd.stats import autocorr, block_mean, tau_int, tau_int_err
+from patchyparticles_amd.stats import (autocorr, block_mean, replica_drift, stationary, tau_int, tau_int_err,
+                                      tau_int_replicas)
 
 
 def _ar1(phi, n, seed):
@@ -64,3 +65,45 @@ def test_tau_int_error_matches_the_spread_of_estimates():
 
 def test_constant_series():
     assert tau_int(np.ones(100)) == 0.5
+
+
+def test_tau_int_of_replicas_pools_independent_series():
+    """16 independent AR(1) series (phi = 0.95, tau = 19.5) of 5000 samples
+    each: the pooled estimate centres on the closed form within its stated
+    error (over 100 such sets the error is at least the estimates' spread),
+    and one replica alone gives the single-series estimator's answer."""
+    phi, n, R = 0.95, 5000, 16
+    want = 0.5 + phi / (1 - phi)
+    est, errs = [], []
+    for k in range(100):
+        X = np.stack([_ar1(phi, n, seed=10000 + 100 * k + j) for j in range(R)], 1)
+        t, e, W = tau_int_replicas(X)
+        assert W >= 5 * t
+        est.append(t)
+        errs.append(e)
+    est = np.array(est)
+    spread = est.std(ddof=1)
+    assert spread <= np.mean(errs) <= 1.6 * spread, (np.mean(errs), spread)
+    assert abs(est.mean() - want) < 4 * spread / math.sqrt(len(est)) + 0.02 * want
+    x = _ar1(0.9, 20000, seed=5)
+    a, b = tau_int_replicas(x[:, None])[:2], tau_int_err(x)[:2]
+    assert a[0] == pytest.approx(b[0]) and a[1] == pytest.approx(b[1])
+
+
+def test_stationarity_rule_flags_a_drift():
+    """Stationary AR(1) replicas pass the rule; the same series plus a slow
+    linear trend (a relaxation) fail it on the drift, and too short a run
+    fails it on tau's error."""
+    phi, n, R = 0.9, 20000, 8
+    X = np.stack([_ar1(phi, n, seed=700 + j) for j in range(R)], 1)
+    tau, err, _ = tau_int_replicas(X)
+    d, dse = replica_drift(X)
+    assert stationary(tau, err, d, dse)
+    trend = np.linspace(0.0, -1.0, n)[:, None]
+    tau2, err2, _ = tau_int_replicas(X + trend)
+    d2, dse2 = replica_drift(X + trend)
+    assert d2 < -0.4 and not stationary(tau2, err2, d2, dse2)
+    short = X[:200]
+    tau3, err3, _ = tau_int_replicas(short)
+    d3, dse3 = replica_drift(short)
+    assert not stationary(tau3, err3, d3, dse3, max_rel_err=0.1)
