@@ -25,7 +25,7 @@
  *                                 (MC.c:272-309; main.c:54-61)
  *   pp_check_energy               MC_check_energy + cells_check
  *                                 (MC.c:766-784, cells.c:65-76; main.c:86-87)
- *   pp_download / pp_upload       the System particle arrays read by
+ *   pp_download / pp_upload(_n)   the System particle arrays read by
  *                                 output_save (output.c:302-317) / written by
  *                                 the .rrr loader (system.c:213-245)
  *   pp_get_counters(_all)         syst->tries/accepted/energy/V/box read by
@@ -77,11 +77,12 @@ typedef enum pp_status {
 
 /* Ensembles (defs.h:21-25). */
 #define PP_NVT 0
+#define PP_GC 1
 #define PP_NPT 5
 
 /* Physical and move parameters of one system, the reference's input keys. */
 typedef struct pp_params {
-	int ensemble;              /* Ensemble: PP_NVT or PP_NPT */
+	int ensemble;              /* Ensemble: PP_NVT, PP_GC or PP_NPT */
 	double temperature;        /* Temperature */
 	double disp_max;           /* Disp_max: displacement uniform in +-Disp_max/2 per axis */
 	double theta_max;          /* Theta_max: rotation angle uniform in [0, Theta_max) */
@@ -89,6 +90,8 @@ typedef struct pp_params {
 	double kf_cosmax;          /* KF_cosmax: patch half-angle cosine */
 	double pressure;           /* P (NPT) */
 	double rescale_factor_max; /* rescale_factor_max (NPT): ln V step */
+	double activity;           /* Activity z (GC, system.c:91) */
+	int gc_n_max;              /* GC_N_max (GC, system.c:94): most particles the replica may hold */
 } pp_params;
 
 /* Options of the GPU engine (no reference counterpart). */
@@ -119,6 +122,10 @@ typedef struct pp_counters {
 	long long sweeps;      /* sweeps done since creation */
 	double box[3];
 	double volume;         /* V as the reference tracks it (MC.c:500) */
+	long long gc_tries[2];    /* GC: [0] insertions, [1] removals (ADD, REMOVE of defs.h:13-14) */
+	long long gc_accepted[2];
+	int n;                 /* particles in the replica now (GC: changes; otherwise fixed) */
+	int pad;
 } pp_counters;
 
 typedef struct pp_ctx pp_ctx;
@@ -137,7 +144,13 @@ pp_status pp_create_batch(int n_rep, const pp_params *params, const pp_options *
 /* Advance every replica by n sweeps.  A sweep is N single-particle
  * rototranslation attempts (MC.c:21) done as 8 checkerboard colour passes,
  * plus, under NPT, one volume-move attempt (expected once per sweep in
- * do_NPT, MC.c:62-63). */
+ * do_NPT, MC.c:62-63), and, for a grand-canonical replica (PP_GC), one
+ * phase of insertions and removals (MC_add_remove, MC.c:342-448) in which
+ * every cell of one random colour tries one of the two, accepted with the
+ * reference's factors for the cell's volume and population:
+ * z v_c / (n_c + 1) exp(-dE/T) and n_c / (z v_c) exp(-dE/T).  That phase
+ * needs n_c free slots beyond the particles (GC_N_max >= N + cells of one
+ * colour), else the sweep fails with PP_ERR_ARG. */
 pp_status pp_sweeps(pp_ctx *ctx, long long n);
 
 /* Recompute the energy of replica rep from scratch: bonds[i] = number of
@@ -152,14 +165,23 @@ pp_status pp_energy(pp_ctx *ctx, int rep, int32_t *bonds, uint8_t *overlap, long
  * Gram-Schmidt, count the cell population.  energy_per_rep may be NULL. */
 pp_status pp_check_energy(pp_ctx *ctx, double *energy_per_rep);
 
+/* Particles of replica rep in index order (a grand-canonical replica: its
+ * pp_counters.n particles, renumbered 0 .. n - 1 in the order of their
+ * indices; removals and insertions change the numbering as the reference's
+ * removals do, MC.c:405-433). */
 pp_status pp_download(pp_ctx *ctx, int rep, double *r, double *O, double box[3]);
 pp_status pp_upload(pp_ctx *ctx, int rep, const double *r, const double *O, const double box[3]);
+/* pp_upload with the particle count given: a grand-canonical replica takes
+ * any count up to its GC_N_max, any other replica only its own. */
+pp_status pp_upload_n(pp_ctx *ctx, int rep, int n, const double *r, const double *O, const double box[3]);
 
 pp_status pp_get_counters(pp_ctx *ctx, int rep, pp_counters *out);
 /* pp_get_counters for every replica at once: out[pp_num_replicas(ctx)]. */
 pp_status pp_get_counters_all(pp_ctx *ctx, pp_counters *out);
 pp_status pp_reset_counters(pp_ctx *ctx);
 int pp_num_replicas(const pp_ctx *ctx);
+/* Particles of replica rep; for a grand-canonical replica the most it may
+ * hold (GC_N_max): its current count is pp_counters.n. */
 int pp_num_particles(const pp_ctx *ctx, int rep);
 
 /* Per-particle energy of a snapshot without a context (GPU, same kernels). */

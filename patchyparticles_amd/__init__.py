@@ -5,8 +5,8 @@ ABI of include/pp.h (libpp_hip.so).
 """
 from ._lib import EXPORTS, LIB_PATH, NOALLOC_MAX_N, PPError, energy_snapshot, energy_snapshot_noalloc, generate, lib, net_descriptor
 from .rrr import read_rrr, write_rrr
-from .system import NPT, NVT, ProposalNet, System
+from .system import GC, NPT, NVT, ProposalNet, System
 from .batch import uneven_replicas
 
-__all__ = ["System", "NVT", "NPT", "energy_snapshot", "generate", "read_rrr", "write_rrr", "PPError", "lib", "LIB_PATH",
+__all__ = ["System", "NVT", "NPT", "GC", "energy_snapshot", "generate", "read_rrr", "write_rrr", "PPError", "lib", "LIB_PATH",
            "EXPORTS", "uneven_replicas", "ProposalNet", "net_descriptor", "energy_snapshot_noalloc", "NOALLOC_MAX_N"]

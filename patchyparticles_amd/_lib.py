@@ -22,7 +22,7 @@ EXPORTS = (
     "pp_net_create", "pp_net_set", "pp_net_get", "pp_net_forward", "pp_net_loss_grad", "pp_net_destroy",
     "pp_net_attach", "pp_net_train", "pp_get_learn_counters", "pp_net_descriptor",
     "pp_checkpoint_save", "pp_checkpoint_load", "pp_bond_list", "pp_set_lx_move",
-    "pp_energy_snapshot_noalloc", "pp_set_lx_acceptance", "pp_get_layout",
+    "pp_energy_snapshot_noalloc", "pp_set_lx_acceptance", "pp_get_layout", "pp_upload_n",
 )
 
 # sizes of the proposal network (PP_NET_* in include/pp.h)
@@ -57,6 +57,8 @@ class Params(C.Structure):
         ("kf_cosmax", C.c_double),
         ("pressure", C.c_double),
         ("rescale_factor_max", C.c_double),
+        ("activity", C.c_double),
+        ("gc_n_max", C.c_int),
     ]
 
 
@@ -80,6 +82,10 @@ class Counters(C.Structure):
         ("sweeps", C.c_longlong),
         ("box", C.c_double * 3),
         ("volume", C.c_double),
+        ("gc_tries", C.c_longlong * 2),
+        ("gc_accepted", C.c_longlong * 2),
+        ("n", C.c_int),
+        ("pad", C.c_int),
     ]
 
 
@@ -116,6 +122,8 @@ def lib():
     L.pp_download.restype = st
     L.pp_upload.argtypes = [_vp, C.c_int, _vp, _vp, _vp]
     L.pp_upload.restype = st
+    L.pp_upload_n.argtypes = [_vp, C.c_int, C.c_int, _vp, _vp, _vp]
+    L.pp_upload_n.restype = st
     L.pp_get_counters.argtypes = [_vp, C.c_int, C.POINTER(Counters)]
     L.pp_get_counters.restype = st
     L.pp_get_counters_all.argtypes = [_vp, C.POINTER(Counters)]

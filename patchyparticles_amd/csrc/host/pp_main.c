@@ -1,6 +1,7 @@
 /*
  * pp_main.c -- PatchyParticles_hip: the reference's driver program (main.c)
- * for Dynamics = 0 (rototranslation MC) in Ensemble 0 (NVT) or 5 (NPT), with
+ * for Dynamics = 0 (rototranslation MC) in Ensemble 0 (NVT), 1 (GC, keys
+ * Activity and GC_N_max, system.c:90-95) or 5 (NPT), with
  * every sweep run on the GPU through libpp_hip.so.
  *
  * Same input keys (system.c:58-89, output.c:19-110), same .rrr files and the
@@ -68,10 +69,18 @@ static void check(run *u, pp_status st, const char *what) {
 	ppo_exit(&u->out, "%s failed: %s (%s)\n", what, pp_strerror(st), pp_last_error());
 }
 
+/* particles in the system now (grand canonical: changes with the run) */
+static int current_n(run *u) {
+	pp_counters c;
+	check(u, pp_get_counters(u->ctx, 0, &c), "pp_get_counters");
+	return c.n;
+}
+
 static void save_conf(run *u, long long step, const char *path) {
 	double box[3];
+	const int n = current_n(u);
 	check(u, pp_download(u->ctx, 0, u->r, u->O, box), "pp_download");
-	if(pprrr_write(path, step, u->n, box, u->r, u->O) != 0) ppo_exit(&u->out, "File '%s' is not writable\n", path);
+	if(pprrr_write(path, step, n, box, u->r, u->O) != 0) ppo_exit(&u->out, "File '%s' is not writable\n", path);
 }
 
 /* the checkpoint: the state at the top of an output step, before its energy
@@ -84,8 +93,8 @@ static void save_checkpoint(run *u) {
 static void print_step(run *u, long long step) {
 	pp_counters c;
 	check(u, pp_get_counters(u->ctx, 0, &c), "pp_get_counters");
-	ppo_print(&u->out, step, u->n, -(double) c.bonds, c.volume, c.box, c.accepted[0], c.tries[0], c.accepted[1],
-	          c.tries[1]);
+	ppo_print(&u->out, step, c.n, -(double) c.bonds, c.volume, c.box, c.accepted[0], c.tries[0], c.accepted[1],
+	          c.tries[1], c.gc_accepted, c.gc_tries);
 	check(u, pp_reset_counters(u->ctx), "pp_reset_counters");
 	save_conf(u, step, u->out.last);
 	if(u->out.print_bonds) {
@@ -183,7 +192,8 @@ int main(int argc, char *argv[]) {
 
 	ppi_int(in, "Dynamics", &dynamics, 1);
 	if(dynamics != 0) ppo_exit(&u.out, "Dynamics %d not supported\n", dynamics);
-	if(prm.ensemble != PP_NVT && prm.ensemble != PP_NPT) ppo_exit(&u.out, "Ensemble %d not supported\n", prm.ensemble);
+	if(prm.ensemble != PP_NVT && prm.ensemble != PP_NPT && prm.ensemble != PP_GC)
+		ppo_exit(&u.out, "Ensemble %d not supported\n", prm.ensemble);
 	ppi_double(in, "Disp_max", &prm.disp_max, 1);
 	ppi_double(in, "Theta_max", &prm.theta_max, 1);
 	ppi_double(in, "Temperature", &prm.temperature, 1);
@@ -198,7 +208,7 @@ int main(int argc, char *argv[]) {
 	char err[1024];
 	if(pprrr_read(conf, &c, err, sizeof err) != 0) ppo_exit(&u.out, "%s", err);
 	u.n = c.n;
-	u.npt = prm.ensemble == PP_NPT;
+	u.npt = prm.ensemble != PP_NVT;
 
 	ppi_double(in, "KF_delta", &prm.kf_delta, 1);
 	ppi_double(in, "KF_cosmax", &prm.kf_cosmax, 1);
@@ -206,6 +216,14 @@ int main(int argc, char *argv[]) {
 	if(prm.ensemble == PP_NPT) {
 		ppi_double(in, "rescale_factor_max", &prm.rescale_factor_max, 1);
 		ppi_double(in, "P", &prm.pressure, 1);
+	}
+	if(prm.ensemble == PP_GC) {
+		/* system.c:90-95 */
+		ppi_double(in, "Activity", &prm.activity, 1);
+		ppi_int(in, "GC_N_max", &prm.gc_n_max, 1);
+		if(prm.gc_n_max < c.n)
+			ppo_exit(&u.out, "GC_N_max (%d) is smaller than the number of particles (%d)\n", prm.gc_n_max, c.n);
+		u.n = prm.gc_n_max; /* the arrays hold the most particles the run may have */
 	}
 	/* box-shape moves (system.c:170-196): NVT only */
 	int lx = 0;
@@ -269,7 +287,7 @@ int main(int argc, char *argv[]) {
 
 	pp_counters cnt;
 	check(&u, pp_get_counters(u.ctx, 0, &cnt), "pp_get_counters");
-	ppo_log(&u.out, "Initial energy: %lf\n", u.n > 0 ? -(double) cnt.bonds / u.n : 0.);
+	ppo_log(&u.out, "Initial energy: %lf\n", cnt.n > 0 ? -(double) cnt.bonds / cnt.n : 0.);
 
 	long long steps = 0;
 	ppi_llint(in, "Steps", &steps, 1);
@@ -303,8 +321,8 @@ int main(int argc, char *argv[]) {
 	save_checkpoint(&u);
 	check(&u, pp_get_counters(u.ctx, 0, &cnt), "pp_get_counters");
 	const double E = -(double) cnt.bonds;
-	printf("%lld %lf %lf", step, u.n > 0 ? E / u.n : 0., E);
-	if(u.npt) printf(" %lf", u.n / cnt.volume);
+	printf("%lld %lf %lf", step, cnt.n > 0 ? E / cnt.n : 0., E);
+	if(u.npt) printf(" %lf", cnt.n / cnt.volume);
 	printf("\n");
 
 	pp_destroy(u.ctx);

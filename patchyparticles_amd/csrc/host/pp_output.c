@@ -91,6 +91,7 @@ void ppo_init(ppo *o, ppi_file *in, int ensemble, const char *initial_conf) {
 	ppi_string(in, "Configuration_last", o->last, sizeof o->last, 0);
 
 	o->npt = ensemble != 0;
+	o->gc = ensemble == 1;
 	if(o->npt) {
 		strcpy(name, "density.dat");
 		ppi_string(in, "Density_file", name, sizeof name, 0);
@@ -184,7 +185,8 @@ void ppo_free(ppo *o) {
 }
 
 void ppo_print(ppo *o, long long step, int n, double energy, double volume, const double box[3], long long acc_roto,
-               long long try_roto, long long acc_vol, long long try_vol) {
+               long long try_roto, long long acc_vol, long long try_vol, const long long gc_acc[2],
+               const long long gc_try[2]) {
 	const double e = n > 0 ? energy / n : 0.;
 	fprintf(o->energy, "%lld %lf\n", step, e);
 	fflush(o->energy);
@@ -198,7 +200,13 @@ void ppo_print(ppo *o, long long step, int n, double energy, double volume, cons
 	}
 	fprintf(o->acc, "%lld", step);
 	fprintf(o->acc, " %e", acc_roto / (double) try_roto);
-	if(o->npt || o->lx) fprintf(o->acc, " %e", acc_vol / (double) try_vol);
+	if(o->gc) {
+		/* ADD, REMOVE (output.c:166-169), then the box-shape moves */
+		fprintf(o->acc, " %e", gc_acc[0] / (double) gc_try[0]);
+		fprintf(o->acc, " %e", gc_acc[1] / (double) gc_try[1]);
+		if(o->lx) fprintf(o->acc, " %e", acc_vol / (double) try_vol);
+	}
+	else if(o->npt || o->lx) fprintf(o->acc, " %e", acc_vol / (double) try_vol);
 	fprintf(o->acc, "\n");
 	fflush(o->acc);
 	printf("%lld %lf %lf", step, e, energy);

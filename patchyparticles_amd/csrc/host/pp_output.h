@@ -21,7 +21,8 @@ typedef struct ppo {
 	long long save_every;   /* Save_every */
 	char folder[512];       /* Configuration_folder */
 	char last[512];         /* Configuration_last (last.rrr) */
-	int npt;
+	int npt;                /* a non-canonical ensemble (NPT or GC): density.dat is written */
+	int gc;                 /* grand canonical: insertion and removal acceptances */
 	int print_bonds;        /* Print_bonds (NVT only) */
 	int lx;                 /* box-shape moves (set by the driver, NVT only) */
 	FILE *boxshape;         /* boxshape.dat, opened when Lx_move is set (output.c:101-108) */
@@ -38,13 +39,15 @@ void ppo_log(ppo *o, const char *fmt, ...);
 void ppo_exit(ppo *o, const char *fmt, ...);
 
 /* The print-step lines: energy.dat "step E/N", density.dat "step N/V N"
- * (NPT), boxshape.dat "step Lx Ly Lz" (box-shape moves), acceptance.dat
- * "step acc_roto [acc_volume | acc_Lx]", stdout "step E/N E [N/V]"
- * (output.c:119-201).  Acceptances are accepted / tries since the previous
- * print; acc_vol / try_vol count the volume moves (NPT) or the box-shape
- * moves (NVT). */
+ * (NPT, GC), boxshape.dat "step Lx Ly Lz" (box-shape moves), acceptance.dat
+ * "step acc_roto [acc_add acc_remove | acc_volume] [acc_Lx]", stdout
+ * "step E/N E [N/V]" (output.c:119-201).  Acceptances are accepted / tries
+ * since the previous print; acc_vol / try_vol count the volume moves (NPT) or
+ * the box-shape moves (NVT, GC); gc_acc / gc_try the insertions and the
+ * removals (GC). */
 void ppo_print(ppo *o, long long step, int n, double energy, double volume, const double box[3], long long acc_roto,
-               long long try_roto, long long acc_vol, long long try_vol);
+               long long try_roto, long long acc_vol, long long try_vol, const long long gc_acc[2],
+               const long long gc_try[2]);
 
 /* <Bonds_folder>/bonds_<step>.dat (output.c:205-268): N, then per particle
  * "index count" and the line of its bonded partners, each followed by a

@@ -289,7 +289,8 @@ enum : uint32_t {
 	TAG_AXIS_FALLBACK = 5,
 	TAG_SHIFT = 8,
 	TAG_COLOUR = 9,
-	TAG_VOLUME = 10
+	TAG_VOLUME = 10,
+	TAG_GC = 11
 };
 
 __device__ __forceinline__ u32x4 rng(uint64_t seed, int rep, uint64_t sweep, uint32_t cell, uint32_t tag,

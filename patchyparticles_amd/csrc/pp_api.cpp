@@ -74,6 +74,7 @@ struct pp_ctx {
 	int coop_blocks = 0;  // > 0: the passes of a sweep run as one cooperative launch of that many workgroups
 	long long sweep = 0;
 	bool any_npt = false;
+	bool any_gc = false;  // a replica makes grand-canonical insertions and removals
 	bool any_lx = false;  // a replica makes box-shape moves
 	long long trace_len = -1;  // entries written to d.trace; -1 = tracing off
 	long long trace_from = 0;  // sweep index at which tracing began
@@ -155,6 +156,9 @@ pp_status check_err(pp_ctx *c) {
 	if(e & ERR_CELL_CAP) return fail(PP_ERR_CELL_CAP, "a sweep cell holds more than %d particles", CELL_CAP);
 	if(e & ERR_BOX) return fail(PP_ERR_BOX, "box too small for the cell grid");
 	if(e & ERR_SCAN) return fail(PP_ERR_HIP, "cell scan look-back did not complete");
+	if(e & ERR_GC_FULL)
+		return fail(PP_ERR_ARG, "The system contains too many particles for GC_N_max: a grand-canonical phase needs "
+		                        "N plus the cells of one colour in free slots, increase GC_N_max");
 	if(e & ERR_BARRIER) {
 		launch_fill_i32((int *) c->d.bar, 0, 2, c->stream);
 		launch_fill_i32(c->d.err, 0, 1, c->stream);
@@ -226,9 +230,34 @@ pp_status energy_now(pp_ctx *c, int per_particle) {
 	return PP_OK;
 }
 
+// The particle indices of replica rep's particles in ascending order, and
+// the slot-order copy they were read in.  After a rebuild a replica's
+// particles fill its first np slots; a grand-canonical replica's indices are
+// unique but, after an insertion, not 0 .. np - 1: callers see its particles
+// in index order, renumbered 0 .. np - 1.
+pp_status slot_indices(pp_ctx *c, int rep, std::vector<int> &in_slots, std::vector<int> &sorted) {
+	const Rep &rp = c->reps[rep];
+	in_slots.assign(std::max(rp.np, 1), 0);
+	if(rp.np)
+		HIPCHECK(hipMemcpy(in_slots.data(), c->d.pid[rp.sel] + rp.p_off, sizeof(int) * rp.np, hipMemcpyDeviceToHost));
+	in_slots.resize(rp.np);
+	sorted = in_slots;
+	std::sort(sorted.begin(), sorted.end());
+	for(int i = 0; i < rp.np; i++)
+		if(sorted[i] < 0 || (i > 0 && sorted[i] == sorted[i - 1]) || (!rp.gc && sorted[i] != i))
+			return fail(PP_ERR_HIP, "corrupt particle index %d", sorted[i]);
+	return PP_OK;
+}
+
+int rank_of(const std::vector<int> &sorted, int id) {
+	return (int) (std::lower_bound(sorted.begin(), sorted.end(), id) - sorted.begin());
+}
+
 pp_status validate_params(const pp_params &p, int r) {
-	if(p.ensemble != PP_NVT && p.ensemble != PP_NPT)
-		return fail(PP_ERR_UNSUPPORTED, "replica %d: Ensemble %d not supported (0 = NVT, 5 = NPT)", r, p.ensemble);
+	if(p.ensemble != PP_NVT && p.ensemble != PP_NPT && p.ensemble != PP_GC)
+		return fail(PP_ERR_UNSUPPORTED, "replica %d: Ensemble %d not supported (0 = NVT, 1 = GC, 5 = NPT)", r,
+		            p.ensemble);
+	if(p.ensemble == PP_GC && !(p.activity > 0.)) return fail(PP_ERR_ARG, "replica %d: Activity must be > 0", r);
 	if(!(p.temperature > 0.)) return fail(PP_ERR_ARG, "replica %d: Temperature must be > 0", r);
 	if(!(p.kf_delta > 0.)) return fail(PP_ERR_ARG, "replica %d: KF_delta must be > 0", r);
 	if(!(p.disp_max >= 0.) || !(p.theta_max >= 0.)) return fail(PP_ERR_ARG, "replica %d: negative move size", r);
@@ -265,14 +294,26 @@ pp_status create_impl(int R, const pp_params *prm, const pp_options *opts, const
 		return fail(PP_ERR_ARG, "threads_per_cell %d with block_size %d needs %zu bytes of LDS (max %zu)", Tv,
 		            o.block_size, pass_lds_bytes(Tv, o.block_size), MAX_LDS_BYTES);
 
+	// slots per replica: its particles, or for a grand-canonical replica the
+	// most it may hold (GC_N_max)
+	std::vector<int> cap(R);
 	long long Ntot = 0;
 	for(int k = 0; k < R; k++) {
 		if(n[k] < 0) return fail(PP_ERR_ARG, "negative particle count");
 		TRY(validate_params(prm[k], k));
-		Ntot += n[k];
+		cap[k] = n[k];
+		if(prm[k].ensemble == PP_GC) {
+			if(prm[k].gc_n_max < n[k])
+				return fail(PP_ERR_ARG, "replica %d: GC_N_max (%d) is below the initial number of particles (%d)", k,
+				            prm[k].gc_n_max, n[k]);
+			cap[k] = prm[k].gc_n_max;
+		}
+		Ntot += cap[k];
 	}
+	long long nsum = 0;
+	for(int k = 0; k < R; k++) nsum += n[k];
 	if(Ntot > (1LL << 30)) return fail(PP_ERR_ARG, "too many particles");
-	if(Ntot > 0 && (!r || !O)) return fail(PP_ERR_ARG, "positions/orientations are NULL");
+	if(nsum > 0 && (!r || !O)) return fail(PP_ERR_ARG, "positions/orientations are NULL");
 
 	int ndev = 0;
 	if(hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return fail(PP_ERR_HIP, "no HIP device available");
@@ -284,7 +325,7 @@ pp_status create_impl(int R, const pp_params *prm, const pp_options *opts, const
 	c->T = Tv;
 	c->block = o.block_size;
 	c->reps.resize(R);
-	c->np.assign(n, n + R);
+	c->np.assign(cap.begin(), cap.end());  // a grand-canonical replica: the most it may hold
 	c->p_off.resize(R);
 	c->act_off.resize(R + 1);
 	auto bail = [&](pp_status s) {
@@ -346,6 +387,10 @@ pp_status create_impl(int R, const pp_params *prm, const pp_options *opts, const
 		rp.V = b[0] * b[1] * b[2];
 		rp.p_off = (int) poff;
 		rp.np = n[k];
+		rp.cap_np = cap[k];
+		rp.np_hw = n[k];
+		rp.gc = p.ensemble == PP_GC ? 1 : 0;
+		rp.activity = p.activity;
 		rp.cell_off = (int) coff;
 		rp.cell_cap = rp.cap_n[0] * rp.cap_n[1] * rp.cap_n[2];
 		rp.act_cap = (rp.cap_n[0] / 2) * (rp.cap_n[1] / 2) * (rp.cap_n[2] / 2);
@@ -355,10 +400,11 @@ pp_status create_impl(int R, const pp_params *prm, const pp_options *opts, const
 		rp.sel = 0;
 		c->p_off[k] = (int) poff;
 		c->act_off[k] = (int) aoff;
-		poff += n[k];
-		coff += rp.cell_cap;
+		poff += cap[k];
+		coff += rp.cell_cap + 1;  // and the parking cell of free slots
 		aoff += (long long) (rp.cap_n[0] / 2) * (rp.cap_n[1] / 2) * (rp.cap_n[2] / 2);
 		if(p.ensemble == PP_NPT) c->any_npt = true;
+		if(p.ensemble == PP_GC) c->any_gc = true;
 		if(coff > (1LL << 30)) return bail(fail(PP_ERR_NOMEM, "cell grid too large"));
 	}
 	c->act_off[R] = (int) aoff;
@@ -430,20 +476,24 @@ pp_status create_impl(int R, const pp_params *prm, const pp_options *opts, const
 			if(!q) return bail(fail(PP_ERR_NOMEM, "a device buffer was not allocated"));
 	}
 
-	// host staging: SoA in particle-index order, slot j = particle j
+	// host staging: SoA in particle-index order, slot j = particle j; the
+	// free slots of grand-canonical replicas hold no particle (index -1)
 	{
 		std::vector<double> h((size_t) NCOMP * d.stride, 0.);
-		std::vector<int> hp(d.stride, 0), hr(d.stride, 0);
-		for(int k = 0; k < R; k++)
-			for(int i = 0; i < n[k]; i++) {
+		std::vector<int> hp(d.stride, -1), hr(d.stride, 0);
+		long long src = 0;  // the caller's arrays hold the replicas' particles back to back
+		for(int k = 0; k < R; k++) {
+			for(int i = 0; i < cap[k]; i++) hr[c->p_off[k] + i] = k;
+			for(int i = 0; i < n[k]; i++, src++) {
 				long long j = c->p_off[k] + i;
-				for(int a = 0; a < 3; a++) h[(size_t) a * d.stride + j] = r[3 * j + a];
-				for(int a = 0; a < 9; a++) h[(size_t) (3 + a) * d.stride + j] = O[9 * j + a];
+				for(int a = 0; a < 3; a++) h[(size_t) a * d.stride + j] = r[3 * src + a];
+				for(int a = 0; a < 9; a++) h[(size_t) (3 + a) * d.stride + j] = O[9 * src + a];
 				hp[j] = i;
-				hr[j] = k;
 			}
+		}
 		if(hipMemcpy(d.f[0], h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
 		   hipMemcpy(d.pid[0], hp.data(), hp.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
+		   hipMemcpy(d.pid[1], hp.data(), hp.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
 		   hipMemcpy((void *) d.prep, hr.data(), hr.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
 		   hipMemcpy((void *) d.act_off, c->act_off.data(), (R + 1) * sizeof(int), hipMemcpyHostToDevice) !=
 		       hipSuccess ||
@@ -537,6 +587,9 @@ pp_status enqueue_sweep(pp_ctx *c) {
 				toc(c, TK_PASS, b);
 			}
 		}
+	// grand-canonical insertions and removals (MC_add_remove, MC.c:342-448)
+	// on the lists the passes leave; box-shape trials after them rebuild
+	if(c->any_gc) launch_gc_phase(c->d, sw, c->stream);
 	if(c->any_npt || c->any_lx) {
 		// one volume (or box-shape) attempt per N particle attempts on
 		// average, as do_NPT and do_NVT (MC.c:19-29, 60-68): with s trials per
@@ -640,9 +693,26 @@ pp_status pp_energy(pp_ctx *c, int rep, int32_t *bonds, uint8_t *overlap, long l
 	TRY(energy_now(c, 1));
 	TRY(collect_timing(c));
 	TRY(check_err(c));
-	const int n = c->np[rep], off = c->p_off[rep];
-	if(bonds && n) HIPCHECK(hipMemcpy(bonds, c->d.out_bonds + off, n * sizeof(int32_t), hipMemcpyDeviceToHost));
-	if(overlap && n) HIPCHECK(hipMemcpy(overlap, c->d.out_ovl + off, n * sizeof(uint8_t), hipMemcpyDeviceToHost));
+	TRY(pull_reps(c));
+	const Rep &rp = c->reps[rep];
+	const int n = rp.np, off = rp.p_off;
+	if(!rp.gc) {
+		if(bonds && n) HIPCHECK(hipMemcpy(bonds, c->d.out_bonds + off, n * sizeof(int32_t), hipMemcpyDeviceToHost));
+		if(overlap && n) HIPCHECK(hipMemcpy(overlap, c->d.out_ovl + off, n * sizeof(uint8_t), hipMemcpyDeviceToHost));
+	}
+	else if(n) {
+		// the results sit at their particles' indices: gather them in index order
+		std::vector<int> ids, sorted;
+		TRY(slot_indices(c, rep, ids, sorted));
+		std::vector<int32_t> hb(rp.cap_np);
+		std::vector<uint8_t> ho(rp.cap_np);
+		HIPCHECK(hipMemcpy(hb.data(), c->d.out_bonds + off, rp.cap_np * sizeof(int32_t), hipMemcpyDeviceToHost));
+		HIPCHECK(hipMemcpy(ho.data(), c->d.out_ovl + off, rp.cap_np * sizeof(uint8_t), hipMemcpyDeviceToHost));
+		for(int i = 0; i < n; i++) {
+			if(bonds) bonds[i] = hb[sorted[i]];
+			if(overlap) overlap[i] = ho[sorted[i]];
+		}
+	}
 	std::vector<unsigned long long> h;
 	TRY(pull_ctr(c, h));
 	if(total_bonds) *total_bonds = ctr_sum(h, rep, C_EBONDS) / 2;
@@ -697,18 +767,22 @@ pp_status pp_download(pp_ctx *c, int rep, double *r, double *O, double box[3]) {
 	if(!c) return fail(PP_ERR_ARG, "ctx is NULL");
 	if(rep < 0 || rep >= c->R) return fail(PP_ERR_ARG, "replica %d out of range", rep);
 	HIPCHECK(hipSetDevice(c->device));
+	if(c->any_gc) {
+		// grand-canonical replicas: their live particles first (a rebuild)
+		rebuild_now(c);
+		HIPCHECK(hipStreamSynchronize(c->stream));
+	}
 	TRY(pull_reps(c));
 	const Rep &rp = c->reps[rep];
 	const int n = rp.np, off = rp.p_off, S = c->d.stride;
 	std::vector<double> h((size_t) NCOMP * std::max(n, 1));
-	std::vector<int> hp(std::max(n, 1));
+	std::vector<int> hp, sorted;
 	for(int a = 0; a < NCOMP && n; a++)
 		HIPCHECK(hipMemcpy(h.data() + (size_t) a * n, c->d.f[rp.sel] + (size_t) a * S + off, n * sizeof(double),
 		                   hipMemcpyDeviceToHost));
-	if(n) HIPCHECK(hipMemcpy(hp.data(), c->d.pid[rp.sel] + off, n * sizeof(int), hipMemcpyDeviceToHost));
+	TRY(slot_indices(c, rep, hp, sorted));
 	for(int j = 0; j < n; j++) {
-		int i = hp[j];
-		if(i < 0 || i >= n) return fail(PP_ERR_HIP, "corrupt particle index %d", i);
+		const int i = rp.gc ? rank_of(sorted, hp[j]) : hp[j];
 		if(r)
 			for(int a = 0; a < 3; a++) r[3 * i + a] = h[(size_t) a * n + j];
 		if(O)
@@ -719,14 +793,28 @@ pp_status pp_download(pp_ctx *c, int rep, double *r, double *O, double box[3]) {
 	return PP_OK;
 }
 
-pp_status pp_upload(pp_ctx *c, int rep, const double *r, const double *O, const double box[3]) {
-	if(!c) return fail(PP_ERR_ARG, "ctx is NULL");
-	if(rep < 0 || rep >= c->R) return fail(PP_ERR_ARG, "replica %d out of range", rep);
+}  // extern "C"
+
+namespace {
+
+// Replace replica rep's particles by n given ones (n = its particle count,
+// or for a grand-canonical replica any count up to its GC_N_max).
+pp_status upload_impl(pp_ctx *c, int rep, int n, const double *r, const double *O, const double box[3]) {
 	HIPCHECK(hipSetDevice(c->device));
 	TRY(pull_reps(c));
 	Rep &rp = c->reps[rep];
-	const int n = rp.np, off = rp.p_off, S = c->d.stride;
+	if(n != rp.np && !(rp.gc && n >= 0 && n <= rp.cap_np))
+		return fail(PP_ERR_ARG, "replica %d holds %d particles, not %d", rep, rp.np, n);
+	const int off = rp.p_off, S = c->d.stride;
 	if(n && (!r || !O)) return fail(PP_ERR_ARG, "positions/orientations are NULL");
+	if(rp.gc) {
+		// the replica's other slots are free
+		std::vector<int> none(rp.cap_np, -1);
+		for(int b = 0; b < 2; b++)
+			HIPCHECK(hipMemcpy(c->d.pid[b] + off, none.data(), sizeof(int) * rp.cap_np, hipMemcpyHostToDevice));
+		rp.np = n;
+		rp.np_hw = n;
+	}
 	if(box) {
 		int gn[3];
 		for(int a = 0; a < 3; a++)
@@ -763,6 +851,24 @@ pp_status pp_upload(pp_ctx *c, int rep, const double *r, const double *O, const 
 	return push_reps(c);
 }
 
+}  // namespace
+
+extern "C" {
+
+pp_status pp_upload(pp_ctx *c, int rep, const double *r, const double *O, const double box[3]) {
+	if(!c) return fail(PP_ERR_ARG, "ctx is NULL");
+	if(rep < 0 || rep >= c->R) return fail(PP_ERR_ARG, "replica %d out of range", rep);
+	HIPCHECK(hipSetDevice(c->device));
+	TRY(pull_reps(c));
+	return upload_impl(c, rep, c->reps[rep].np, r, O, box);
+}
+
+pp_status pp_upload_n(pp_ctx *c, int rep, int n, const double *r, const double *O, const double box[3]) {
+	if(!c) return fail(PP_ERR_ARG, "ctx is NULL");
+	if(rep < 0 || rep >= c->R) return fail(PP_ERR_ARG, "replica %d out of range", rep);
+	return upload_impl(c, rep, n, r, O, box);
+}
+
 pp_status pp_get_counters(pp_ctx *c, int rep, pp_counters *out) {
 	if(!c || !out) return fail(PP_ERR_ARG, "NULL argument");
 	if(rep < 0 || rep >= c->R) return fail(PP_ERR_ARG, "replica %d out of range", rep);
@@ -779,6 +885,11 @@ pp_status pp_get_counters(pp_ctx *c, int rep, pp_counters *out) {
 	out->sweeps = c->sweep;
 	for(int a = 0; a < 3; a++) out->box[a] = rp.g.box[a];
 	out->volume = rp.V;
+	out->gc_tries[0] = ctr_sum(h, rep, C_TRY_ADD);
+	out->gc_tries[1] = ctr_sum(h, rep, C_TRY_REM);
+	out->gc_accepted[0] = ctr_sum(h, rep, C_ACC_ADD);
+	out->gc_accepted[1] = ctr_sum(h, rep, C_ACC_REM);
+	out->n = rp.np;
 	return PP_OK;
 }
 
@@ -799,6 +910,11 @@ pp_status pp_get_counters_all(pp_ctx *c, pp_counters *out) {
 		o.sweeps = c->sweep;
 		for(int a = 0; a < 3; a++) o.box[a] = rp.g.box[a];
 		o.volume = rp.V;
+		o.gc_tries[0] = ctr_sum(h, rep, C_TRY_ADD);
+		o.gc_tries[1] = ctr_sum(h, rep, C_TRY_REM);
+		o.gc_accepted[0] = ctr_sum(h, rep, C_ACC_ADD);
+		o.gc_accepted[1] = ctr_sum(h, rep, C_ACC_REM);
+		o.n = rp.np;
 	}
 	return PP_OK;
 }
@@ -810,7 +926,7 @@ pp_status pp_reset_counters(pp_ctx *c) {
 	TRY(pull_ctr(c, h));
 	for(int k = 0; k < c->R; k++)
 		for(int row : {(int) C_TRY_ROTO, (int) C_ACC_ROTO, (int) C_TRY_VOL, (int) C_ACC_VOL, (int) C_TRY_LEARN,
-		               (int) C_ACC_LEARN})
+		               (int) C_ACC_LEARN, (int) C_TRY_ADD, (int) C_ACC_ADD, (int) C_TRY_REM, (int) C_ACC_REM})
 			for(int s = 0; s < NSLOT; s++) h[((size_t) k * NCTR + row) * NSLOT + s] = 0;
 	HIPCHECK(hipMemcpy(c->d.ctr, h.data(), h.size() * sizeof(unsigned long long), hipMemcpyHostToDevice));
 	return PP_OK;
@@ -922,7 +1038,10 @@ namespace {
 // tracked volume, move counters, positions and orientations (FP64, bytes as
 // held on the device)
 constexpr char CHK_MAGIC[8] = {'P', 'P', 'C', 'H', 'K', '1', 0, 0};
-constexpr int CHK_ROWS[6] = {C_TRY_ROTO, C_ACC_ROTO, C_TRY_VOL, C_ACC_VOL, C_TRY_LEARN, C_ACC_LEARN};
+constexpr int CHK_NROWS = 10;
+constexpr int CHK_ROWS[CHK_NROWS] = {C_TRY_ROTO, C_ACC_ROTO, C_TRY_VOL, C_ACC_VOL, C_TRY_LEARN,
+                                     C_ACC_LEARN, C_TRY_ADD, C_ACC_ADD, C_TRY_REM, C_ACC_REM};
+constexpr int CHK_VERSION = 2;
 
 struct ChkHead {
 	char magic[8];
@@ -936,7 +1055,7 @@ struct ChkRep {
 	int32_t n, ensemble;
 	double T, disp_max, theta_max, rc2, cosmax, P, rfm;
 	double box[3], V;
-	int64_t ctr[6];
+	int64_t ctr[CHK_NROWS];
 };
 
 }  // namespace
@@ -953,7 +1072,7 @@ pp_status pp_checkpoint_save(pp_ctx *c, const char *path) {
 	if(!f) return fail(PP_ERR_ARG, "cannot open '%s' for writing", tmp.c_str());
 	ChkHead h{};
 	std::memcpy(h.magic, CHK_MAGIC, 8);
-	h.version = 1;
+	h.version = CHK_VERSION;
 	h.R = c->R;
 	h.sweep = c->sweep;
 	h.seed = c->d.seed;
@@ -974,7 +1093,7 @@ pp_status pp_checkpoint_save(pp_ctx *c, const char *path) {
 		q.rfm = rp.rfm;
 		for(int a = 0; a < 3; a++) q.box[a] = rp.g.box[a];
 		q.V = rp.V;
-		for(int t = 0; t < 6; t++) q.ctr[t] = ctr_sum(hc, k, CHK_ROWS[t]);
+		for(int t = 0; t < CHK_NROWS; t++) q.ctr[t] = ctr_sum(hc, k, CHK_ROWS[t]);
 		r.assign(3 * (size_t) std::max(rp.np, 1), 0.);
 		O.assign(9 * (size_t) std::max(rp.np, 1), 0.);
 		pp_status st = pp_download(c, k, r.data(), O.data(), nullptr);
@@ -999,7 +1118,7 @@ pp_status pp_checkpoint_load(pp_ctx *c, const char *path) {
 	std::vector<ChkRep> q(c->R);
 	std::vector<std::vector<double>> r(c->R), O(c->R);
 	pp_status st = PP_OK;
-	if(fread(&h, sizeof h, 1, f) != 1 || std::memcmp(h.magic, CHK_MAGIC, 8) != 0 || h.version != 1)
+	if(fread(&h, sizeof h, 1, f) != 1 || std::memcmp(h.magic, CHK_MAGIC, 8) != 0 || h.version != CHK_VERSION)
 		st = fail(PP_ERR_ARG, "'%s' is not a checkpoint of this library", path);
 	else if(h.R != c->R)
 		st = fail(PP_ERR_ARG, "checkpoint holds %d replicas, the context %d", h.R, c->R);
@@ -1013,7 +1132,8 @@ pp_status pp_checkpoint_load(pp_ctx *c, const char *path) {
 			st = fail(PP_ERR_ARG, "checkpoint '%s' is truncated", path);
 			break;
 		}
-		if(x.n != rp.np || x.ensemble != rp.ensemble || x.T != rp.T || x.disp_max != rp.disp_max ||
+		const bool n_ok = rp.gc ? (x.n >= 0 && x.n <= rp.cap_np) : x.n == rp.np;
+		if(!n_ok || x.ensemble != rp.ensemble || x.T != rp.T || x.disp_max != rp.disp_max ||
 		   x.theta_max != rp.theta_max || x.rc2 != rp.rc2 || x.cosmax != rp.cosmax || x.P != rp.P || x.rfm != rp.rfm) {
 			st = fail(PP_ERR_ARG, "replica %d of the checkpoint has other parameters or size than the context", k);
 			break;
@@ -1026,14 +1146,14 @@ pp_status pp_checkpoint_load(pp_ctx *c, const char *path) {
 	}
 	fclose(f);
 	if(st != PP_OK) return st;
-	for(int k = 0; k < c->R; k++) TRY(pp_upload(c, k, r[k].data(), O[k].data(), q[k].box));
+	for(int k = 0; k < c->R; k++) TRY(upload_impl(c, k, q[k].n, r[k].data(), O[k].data(), q[k].box));
 	TRY(pull_reps(c));
 	for(int k = 0; k < c->R; k++) c->reps[k].V = q[k].V;
 	TRY(push_reps(c));
 	std::vector<unsigned long long> hc;
 	TRY(pull_ctr(c, hc));
 	for(int k = 0; k < c->R; k++)
-		for(int t = 0; t < 6; t++)
+		for(int t = 0; t < CHK_NROWS; t++)
 			for(int sl = 0; sl < NSLOT; sl++)
 				hc[((size_t) k * NCTR + CHK_ROWS[t]) * NSLOT + sl] = sl == 0 ? (unsigned long long) q[k].ctr[t] : 0ull;
 	HIPCHECK(hipMemcpy(c->d.ctr, hc.data(), hc.size() * sizeof(unsigned long long), hipMemcpyHostToDevice));
@@ -1082,8 +1202,15 @@ pp_status pp_bond_list(pp_ctx *c, int rep, int max_per, int32_t *count, int32_t 
 	if(!c || !count || (max_per > 0 && !partners) || max_per < 0) return fail(PP_ERR_ARG, "bad arguments");
 	if(rep < 0 || rep >= c->R) return fail(PP_ERR_ARG, "replica %d out of range", rep);
 	HIPCHECK(hipSetDevice(c->device));
+	rebuild_now(c);  // every particle in the cell of its position
+	HIPCHECK(hipStreamSynchronize(c->stream));
 	TRY(pull_reps(c));
-	const int n = c->np[rep], off = c->p_off[rep];
+	const Rep &rp = c->reps[rep];
+	const int n = rp.np, off = rp.p_off;
+	// a grand-canonical replica's results sit at its particles' indices
+	std::vector<int> ids, sorted;
+	if(rp.gc) TRY(slot_indices(c, rep, ids, sorted));
+	const int span = rp.gc ? rp.cap_np : n;
 	int *dc = nullptr, *dp = nullptr;
 	const size_t np_all = (size_t) std::max(c->d.Ntot, 1);
 	if(hipMalloc(&dc, sizeof(int) * np_all) != hipSuccess ||
@@ -1091,16 +1218,24 @@ pp_status pp_bond_list(pp_ctx *c, int rep, int max_per, int32_t *count, int32_t 
 		if(dc) (void) hipFree(dc);
 		return fail(PP_ERR_NOMEM, "hipMalloc of the bond list failed");
 	}
-	rebuild_now(c);  // every particle in the cell of its position
 	launch_bond_list(c->d, max_per, dc, dp, c->stream);
 	hipError_t e = hipStreamSynchronize(c->stream);
 	if(e == hipSuccess) e = hipGetLastError();
-	if(e == hipSuccess && n > 0) e = hipMemcpy(count, dc + off, sizeof(int) * n, hipMemcpyDeviceToHost);
-	if(e == hipSuccess && n > 0 && max_per > 0)
-		e = hipMemcpy(partners, dp + (size_t) off * max_per, sizeof(int) * (size_t) n * max_per, hipMemcpyDeviceToHost);
+	std::vector<int32_t> hc((size_t) std::max(span, 1)), hp((size_t) std::max(span, 1) * std::max(max_per, 1));
+	if(e == hipSuccess && span > 0) e = hipMemcpy(hc.data(), dc + off, sizeof(int) * span, hipMemcpyDeviceToHost);
+	if(e == hipSuccess && span > 0 && max_per > 0)
+		e = hipMemcpy(hp.data(), dp + (size_t) off * max_per, sizeof(int) * (size_t) span * max_per, hipMemcpyDeviceToHost);
 	(void) hipFree(dc);
 	(void) hipFree(dp);
 	if(e != hipSuccess) return fail(PP_ERR_HIP, "bond list: %s", hipGetErrorString(e));
+	for(int i = 0; i < n; i++) {
+		const int at = rp.gc ? sorted[i] : i;
+		count[i] = hc[at];
+		for(int k = 0; k < max_per && k < hc[at]; k++) {
+			const int q = hp[(size_t) at * max_per + k];
+			partners[(size_t) i * max_per + k] = rp.gc ? rank_of(sorted, q) : q;
+		}
+	}
 	return check_err(c);
 }
 
@@ -1117,6 +1252,7 @@ pp_status pp_get_learn_counters(pp_ctx *c, int rep, long long *tries, long long 
 
 pp_status pp_net_attach(pp_ctx *c, pp_net *net, double learned_fraction, double theta_max) {
 	if(!c) return fail(PP_ERR_ARG, "ctx is NULL");
+	if(net && c->any_gc) return fail(PP_ERR_UNSUPPORTED, "learned proposals are not available with grand-canonical replicas");
 	if(!net) {
 		c->net = nullptr;
 		c->learned_fraction = 0.;
@@ -1150,6 +1286,7 @@ pp_status pp_net_attach(pp_ctx *c, pp_net *net, double learned_fraction, double 
 pp_status pp_net_train(pp_ctx *c, pp_net *net, int steps, int batch, double lr, double pos_noise, double theta_max,
                        int sweeps_between, double *loss_first, double *loss_last) {
 	if(!c || !net) return fail(PP_ERR_ARG, "NULL argument");
+	if(c->any_gc) return fail(PP_ERR_UNSUPPORTED, "learned proposals are not available with grand-canonical replicas");
 	if(net->device != c->device) return fail(PP_ERR_ARG, "network on device %d, context on device %d", net->device, c->device);
 	if(steps < 0 || batch < 1 || batch > (1 << 20)) return fail(PP_ERR_ARG, "steps >= 0 and batch in [1, 2^20] required");
 	if(!(lr > 0.) || !(pos_noise > 0.) || !(theta_max > 0. && theta_max <= M_PI) || sweeps_between < 0)

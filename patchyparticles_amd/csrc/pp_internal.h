@@ -24,7 +24,10 @@ constexpr int REC_W = 64;      // ints of one cell record of a pass
 constexpr int ACC_TAB = 16;    // Metropolis factors exp(-k/T) tabulated per replica, k < ACC_TAB
 
 // per-replica counter rows, each NSLOT wide
-enum Ctr : int { C_TRY_ROTO = 0, C_ACC_ROTO, C_DBONDS, C_TRY_VOL, C_ACC_VOL, C_EBONDS, C_OVL, C_TRY_LEARN, C_ACC_LEARN, NCTR };
+enum Ctr : int {
+	C_TRY_ROTO = 0, C_ACC_ROTO, C_DBONDS, C_TRY_VOL, C_ACC_VOL, C_EBONDS, C_OVL, C_TRY_LEARN, C_ACC_LEARN,
+	C_TRY_ADD, C_ACC_ADD, C_TRY_REM, C_ACC_REM, NCTR
+};
 
 // ------------------------------------------------------------------------
 // learned proposal network: descriptor of up to NN_NB neighbours within
@@ -76,7 +79,7 @@ struct LearnDev {
 };
 
 // device error bits
-enum Err : int { ERR_CELL_CAP = 1, ERR_BOX = 2, ERR_SCAN = 8, ERR_BARRIER = 16 };
+enum Err : int { ERR_CELL_CAP = 1, ERR_BOX = 2, ERR_SCAN = 8, ERR_BARRIER = 16, ERR_GC_FULL = 32 };
 
 // rebuild modes
 enum Reb : int { REB_KEEP = 0, REB_SHIFT = 1, REB_TRIAL = 2 };
@@ -101,10 +104,23 @@ struct Grid {
 };
 
 // One replica: its particle range, its cell region and its parameters.
+//
+// Slots [p_off, p_off + cap_np) belong to the replica; after a rebuild its np
+// particles fill [p_off, p_off + np) grouped by cell, and the rest of its
+// slots sit in its parking cell (local index cell_cap, after the grid's
+// cells, never listed or searched).  Only grand-canonical replicas have
+// cap_np > np: their insertions take free slots (np_hw: slots in use since
+// the last rebuild) and their removals mark slots dead (particle index -1).
 struct Rep {
 	Grid g;   // grid of the current cell structure
 	Grid tg;  // grid of an NPT trial (valid while trial_active)
 	int p_off, np;
+	int cap_np, np_hw;
+	int gc;           // grand-canonical insertions and removals (Ensemble = 1)
+	int gc_base;      // np when this sweep's GC phase began (-1: the phase is skipped)
+	int gc_colour;    // colour of this sweep's GC phase
+	int pad3_;
+	double activity;  // z (system.c:91)
 	int cell_off, cell_cap;
 	int cap_n[3];
 	int act_cap;       // cap_n[0]/2 * cap_n[1]/2 * cap_n[2]/2: cells of one colour at most
@@ -233,6 +249,8 @@ void launch_vol_propose(const DevState &d, uint64_t sweep, int k, hipStream_t s)
 void launch_vol_decide(const DevState &d, uint64_t sweep, int k, hipStream_t s);
 // rescale the positions of replicas whose fast trial was accepted
 void launch_vol_apply(const DevState &d, hipStream_t s);
+// grand-canonical insertions and removals of the sweep (GC replicas only)
+void launch_gc_phase(const DevState &d, uint64_t sweep, hipStream_t s);
 void launch_gram_schmidt(const DevState &d, hipStream_t s);
 
 }  // namespace ppd

@@ -134,15 +134,20 @@ __global__ void k_count(DevState d, int mode, uint64_t sweep, long long trace_po
 		for(int k = 0; k < 3; k++) rp.lg.shift[k] = sh[k];
 		rp.lists_ok = 1;
 	}
-	const double *F = d.f[rp.sel];
-	const int S = d.stride;
-	double p[3] = {F[j], F[S + j], F[2 * S + j]};
-	if(tr)
-		for(int k = 0; k < 3; k++) p[k] *= rp.trial_s[k];
-	int ix = cell_coord(p[0], sh[0], G.inv_box[0], G.n[0]);
-	int iy = cell_coord(p[1], sh[1], G.inv_box[1], G.n[1]);
-	int iz = cell_coord(p[2], sh[2], G.inv_box[2], G.n[2]);
-	int c = rp.cell_off + (ix * G.n[1] + iy) * G.n[2] + iz;
+	// a free or removed slot of a grand-canonical replica goes to the
+	// replica's parking cell
+	int c = rp.cell_off + rp.cell_cap;
+	if(j - rp.p_off < rp.np_hw && d.pid[rp.sel][j] >= 0) {
+		const double *F = d.f[rp.sel];
+		const int S = d.stride;
+		double p[3] = {F[j], F[S + j], F[2 * S + j]};
+		if(tr)
+			for(int k = 0; k < 3; k++) p[k] *= rp.trial_s[k];
+		int ix = cell_coord(p[0], sh[0], G.inv_box[0], G.n[0]);
+		int iy = cell_coord(p[1], sh[1], G.inv_box[1], G.n[1]);
+		int iz = cell_coord(p[2], sh[2], G.inv_box[2], G.n[2]);
+		c = rp.cell_off + (ix * G.n[1] + iy) * G.n[2] + iz;
+	}
 	d.cellid[j] = c;
 	d.slot[j] = atomicAdd(&d.count[c], 1);
 }
@@ -303,7 +308,10 @@ __global__ __launch_bounds__(256) void k_scan(DevState d, int flip, int lists, i
 				const int ix = cloc / (n1 * n2), iy = (cloc / n2) % n1, iz = cloc % n2;
 				const int col = ((ix & 1) << 2) | ((iy & 1) << 1) | (iz & 1);
 				const int key = (lo - sh_r0) * 8 + col;
-				if(key < SCAN_KEYS) {
+				if(cloc >= rp.g.n[0] * n1 * n2) {
+					// the parking cell: free slots, no particles to move
+				}
+				else if(key < SCAN_KEYS) {
 					lkey[e] = key;
 					lpos[e] = atomicAdd(&lcnt[key], 1);
 				}
@@ -417,6 +425,8 @@ __global__ void k_scatter(DevState d, int mode, uint64_t sweep) {
 	const Rep &rp = d.reps[r];
 	const bool tr = (mode == REB_TRIAL) && trial_rebuilds(rp);
 	if(mode == REB_TRIAL && !tr) return;
+	// the buffer this rebuild switches to holds the live particles first
+	if(mode != REB_TRIAL && j == rp.p_off) d.reps[r].np_hw = rp.np;
 	const int sb = tr ? rp.sel : (rp.sel ^ 1);
 	const double *F = d.f[sb];
 	double *G = d.f[sb ^ 1];
@@ -1240,6 +1250,7 @@ __global__ __launch_bounds__(256) void k_energy(DevState d, int trial, int per_p
 	if(j >= d.Ntot) return;
 	const int r = d.prep[j];
 	const Rep &rp = d.reps[r];
+	if(j - rp.p_off >= rp.np) return;  // a free slot (the live particles come first after a rebuild)
 	const bool tr = trial && rp.trial_active;
 	if(trial && !tr) return;
 	if(tr && __hip_atomic_load(&d.vflag[r], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
@@ -1337,6 +1348,7 @@ __global__ void k_bond_list(DevState d, int max_per, int *count, int *partners) 
 	if(j >= d.Ntot) return;
 	const int r = d.prep[j];
 	const Rep &rp = d.reps[r];
+	if(j - rp.p_off >= rp.np) return;
 	const Grid &G = rp.g;
 	const double *F = d.f[rp.sel];
 	const int *PID = d.pid[rp.sel];
@@ -1515,6 +1527,7 @@ __global__ void k_vol_decide(DevState d, uint64_t sweep, int k) {
 			rp.sel ^= 1;
 			rp.lg = rp.tg;
 			rp.lists_ok = 1;
+			rp.np_hw = rp.np;
 		}
 		rp.g = rp.tg;
 		if(rp.trial_kind == 0) rp.V = rp.trial_V;
@@ -1549,12 +1562,208 @@ void launch_vol_apply(const DevState &d, hipStream_t s) {
 }
 
 // ------------------------------------------------------------------------
+// Grand-canonical insertions and removals (MC_add_remove, MC.c:342-448),
+// made cell-local for the checkerboard: once per sweep, after the colour
+// passes, every cell of one colour (drawn per replica and sweep; empty
+// cells too) tries one insertion or one removal, each with probability 1/2.
+//   insertion: a position uniform in the cell and an orientation uniform on
+//     SO(3) (random_orientation, utils.c:133-169), accepted with
+//     min(1, z v_c / (n_c + 1) exp(-dE/T)), dE = -(bonds of the new particle)
+//   removal: a member uniform among the cell's n_c, accepted with
+//     min(1, n_c / (z v_c) exp(-dE/T)), dE = +(the member's bonds)
+// (MC.c:367 and MC.c:391 with the cell's volume and population in place of
+// the box's).  Cells of one colour are a cell side apart, so their moves do
+// not interact: each pair of moves satisfies detailed balance for
+// z^N / N! exp(-E/T) on its own.  A cell's insertion takes a slot reserved
+// for that cell (np + its index among the colour's cells), so the outcome
+// does not depend on thread timing; a replica with fewer free slots than
+// cells of one colour skips the phase and reports ERR_GC_FULL.
+
+// Begin a GC phase: particle indices of each grand-canonical replica become
+// 0 .. np - 1 in slot order.  After the sweep's passes every cell holds its
+// members in order of particle index, so slot order is the same for every
+// launch shape; this keeps the indices dense for the phase's insertions,
+// which take indices from np on.
+__global__ void k_gc_begin(DevState d, uint64_t sweep) {
+	const int j = blockIdx.x * blockDim.x + threadIdx.x;
+	if(j >= d.Ntot) return;
+	const int r = d.prep[j];
+	Rep &rp = d.reps[r];
+	if(!rp.gc) return;
+	const int loc = j - rp.p_off;
+	if(loc < rp.np) d.pid[rp.sel][j] = loc;
+	if(loc == 0) {
+		const u32x4 v = rng(d.seed, r, sweep, 0, TAG_GC, 0, 0);
+		rp.gc_colour = (int) (v.x & 7u);
+		const int ncol = (rp.g.n[0] / 2) * (rp.g.n[1] / 2) * (rp.g.n[2] / 2);
+		if(rp.np + ncol > rp.cap_np) {
+			atomicOr(d.err, (int) ERR_GC_FULL);
+			rp.gc_base = -1;
+		}
+		else {
+			rp.gc_base = rp.np;
+			rp.np_hw = rp.np + ncol;
+			rp.lists_ok = 0;  // the phase adds and removes particles the lists do not show
+		}
+	}
+}
+
+// Orientation uniform on SO(3) as random_orientation builds it (utils.c:
+// 133-169): three uniform unit vectors x, y, z, Gram-Schmidt in that order,
+// rows (z, x, y), or (x, z, y) when that has determinant -1.
+__device__ inline void gc_orientation(uint64_t seed, int r, uint64_t sweep, uint32_t cell, uint32_t col, double O[9]) {
+	double v[9];
+	for(int k = 0; k < 3; k++) sphere_axis(seed, r, sweep, cell, col, 0x8000u + (uint32_t) k, v + 3 * k);
+	gram_schmidt_rows(v);
+	const double *x = v, *y = v + 3, *z = v + 6;
+	const double det = z[0] * (x[1] * y[2] - x[2] * y[1]) - z[1] * (x[0] * y[2] - x[2] * y[0]) +
+	                   z[2] * (x[0] * y[1] - x[1] * y[0]);
+	const double *r0 = det < 0. ? x : z, *r1 = det < 0. ? z : x;
+	for(int k = 0; k < 3; k++) {
+		O[k] = r0[k];
+		O[3 + k] = r1[k];
+		O[6 + k] = y[k];
+	}
+}
+
+// Bonds of a particle at p with patches P against every particle of the 27
+// cells around cell (ix, iy, iz) but slot `skip` (MC_energy, MC.c:272-309):
+// T lanes, the cells shared out; every lane returns the team's count and
+// whether any overlap was met.
+template <int T>
+__device__ inline int gc_bonds(const DevState &d, const Rep &rp, const double *F, int ix, int iy, int iz,
+                               const double p[3], const double P[4][3], int skip, int lane, bool &any) {
+	const Grid &G = rp.g;
+	const int S = d.stride, n0 = G.n[0], n1 = G.n[1], n2 = G.n[2];
+	int nb_ = 0;
+	bool ov = false;
+	for(int nb = lane; nb < 27; nb += T) {
+		const int cx = wrapi(ix + nb / 9 - 1, n0), cy = wrapi(iy + (nb / 3) % 3 - 1, n1), cz = wrapi(iz + nb % 3 - 1, n2);
+		const int cc = rp.cell_off + (cx * n1 + cy) * n2 + cz;
+		for(int q = d.start[cc]; q < d.start[cc + 1]; q++) {
+			if(q == skip) continue;
+			double dx = kf_min_image(F[q] - p[0], G.box[0], G.inv_box[0]);
+			double dy = kf_min_image(F[S + q] - p[1], G.box[1], G.inv_box[1]);
+			double dz = kf_min_image(F[2 * S + q] - p[2], G.box[2], G.inv_box[2]);
+			const double d2 = dx * dx + dy * dy + dz * dz;
+			if(d2 < 1.) ov = true;
+			else if(d2 < rp.rc2) {
+				kf_unit(dx, dy, dz, d2);
+				if(kf_any_p(dx, dy, dz, &P[0][0], rp.cosmax)) {
+					double Oq[9];
+					for(int c = 0; c < 9; c++) Oq[c] = F[(3 + c) * S + q];
+					nb_ += kf_any_q(dx, dy, dz, Oq, rp.cosmax);
+				}
+			}
+		}
+	}
+	any = team_any<T>(ov);
+	return team_sum<T>(nb_);
+}
+
+// One team of 32 lanes per cell of the phase's colour, over every replica's
+// cells (entry gi of the replicas' per-colour capacities).
+__global__ __launch_bounds__(256) void k_gc_pass(DevState d, uint64_t sweep) {
+	constexpr int T = 32;
+	const int lane = threadIdx.x & (T - 1);
+	const int gi = blockIdx.x * (blockDim.x / T) + threadIdx.x / T;
+	if(gi >= d.total_act) return;
+	int lo = 0, hi = d.R;
+	while(hi - lo > 1) {
+		const int mid = (lo + hi) >> 1;
+		if(d.act_off[mid] <= gi) lo = mid;
+		else hi = mid;
+	}
+	const int r = lo;
+	const Rep &rp = d.reps[r];
+	if(!rp.gc || rp.gc_base < 0) return;
+	const int n0 = rp.g.n[0], n1 = rp.g.n[1], n2 = rp.g.n[2];
+	const int h0 = n0 / 2, h1 = n1 / 2, h2 = n2 / 2;
+	const int k = gi - d.act_off[r];
+	if(k >= h0 * h1 * h2) return;
+	const int col = rp.gc_colour;
+	const int ix = 2 * (k / (h1 * h2)) + ((col >> 2) & 1), iy = 2 * ((k / h2) % h1) + ((col >> 1) & 1),
+	          iz = 2 * (k % h2) + (col & 1);
+	const int cloc = (ix * n1 + iy) * n2 + iz;
+	const int c = rp.cell_off + cloc;
+	const int s0 = d.start[c], cnt = d.start[c + 1] - s0;
+	double *F = d.f[rp.sel];
+	int *PID = d.pid[rp.sel];
+	const int S = d.stride;
+	const u32x4 u = rng(d.seed, r, sweep, (uint32_t) cloc, TAG_GC, (uint32_t) col, 1);
+	const u32x4 w = rng(d.seed, r, sweep, (uint32_t) cloc, TAG_GC, (uint32_t) col, 2);
+	const double vc = (rp.g.box[0] / n0) * (rp.g.box[1] / n1) * (rp.g.box[2] / n2);
+	const int cslot = gi & (NSLOT - 1);
+	if(u01(u.x) < 0.5) {
+		// insertion at a uniform point of the cell: the point where the
+		// cell's own coordinate arithmetic puts it (a draw that rounds into a
+		// neighbouring cell is rejected)
+		double p[3];
+		const double fr[3] = {u01(u.y), u01(u.z), u01(u.w)};
+		const int ii[3] = {ix, iy, iz}, nn[3] = {n0, n1, n2};
+		bool inside = true;
+		for(int a = 0; a < 3; a++) {
+			p[a] = rp.g.shift[a] + ((double) ii[a] + fr[a]) / (double) nn[a] * rp.g.box[a];
+			inside = inside && cell_coord(p[a], rp.g.shift[a], rp.g.inv_box[a], nn[a]) == ii[a];
+		}
+		double O[9], P[4][3];
+		gc_orientation(d.seed, r, sweep, (uint32_t) cloc, (uint32_t) col, O);
+		kf_patches(O, P);
+		bool ov = false;
+		const int b = inside ? gc_bonds<T>(d, rp, F, ix, iy, iz, p, P, -1, lane, ov) : 0;
+		if(lane != 0) return;
+		atomicAdd(ctr_at(d.ctr, r, C_TRY_ADD, cslot), 1ull);
+		// MC.c:367: exp(-dE/T) z V / (N + 1), dE = -b
+		const double acc = exp((double) b / rp.T) * rp.activity * vc / (double) (cnt + 1);
+		if(!inside || ov || !(u01(w.x) < acc)) return;
+		const int slot = rp.p_off + rp.gc_base + k;
+		for(int a = 0; a < 3; a++) F[(size_t) a * S + slot] = p[a];
+		for(int a = 0; a < 9; a++) F[(size_t) (3 + a) * S + slot] = O[a];
+		PID[slot] = rp.gc_base + k;
+		atomicAdd(&d.reps[r].np, 1);
+		atomicAdd(ctr_at(d.ctr, r, C_ACC_ADD, cslot), 1ull);
+		if(b) atomicAdd(ctr_at(d.ctr, r, C_DBONDS, cslot), (unsigned long long) (long long) b);
+	}
+	else {
+		if(cnt == 0) {
+			if(lane == 0) atomicAdd(ctr_at(d.ctr, r, C_TRY_REM, cslot), 1ull);
+			return;
+		}
+		// members sit in order of particle index: rank m is slot s0 + m
+		const int q = s0 + (int) __umulhi(u.y, (uint32_t) cnt);
+		const double p[3] = {F[q], F[S + q], F[2 * S + q]};
+		double O[9], P[4][3];
+		for(int a = 0; a < 9; a++) O[a] = F[(size_t) (3 + a) * S + q];
+		kf_patches(O, P);
+		bool ov = false;
+		const int b = gc_bonds<T>(d, rp, F, ix, iy, iz, p, P, q, lane, ov);
+		if(lane != 0) return;
+		atomicAdd(ctr_at(d.ctr, r, C_TRY_REM, cslot), 1ull);
+		// MC.c:391: exp(-dE/T) N / (V z), dE = +b
+		const double acc = exp(-(double) b / rp.T) * (double) cnt / (vc * rp.activity);
+		if(!(u01(w.x) < acc)) return;
+		PID[q] = -1;
+		atomicSub(&d.reps[r].np, 1);
+		atomicAdd(ctr_at(d.ctr, r, C_ACC_REM, cslot), 1ull);
+		if(b) atomicAdd(ctr_at(d.ctr, r, C_DBONDS, cslot), (unsigned long long) (long long) -b);
+	}
+}
+
+void launch_gc_phase(const DevState &d, uint64_t sweep, hipStream_t s) {
+	if(d.Ntot <= 0 || d.total_act <= 0) return;
+	hipLaunchKernelGGL(k_gc_begin, dim3((d.Ntot + 255) / 256), dim3(256), 0, s, d, sweep);
+	hipLaunchKernelGGL(k_gc_pass, dim3((d.total_act + 7) / 8), dim3(256), 0, s, d, sweep);
+}
+
+// ------------------------------------------------------------------------
 // Gram-Schmidt of every orientation (MC_check_energy, MC.c:775)
 
 __global__ void k_gram_schmidt(DevState d) {
 	int j = blockIdx.x * blockDim.x + threadIdx.x;
 	if(j >= d.Ntot) return;
-	double *F = d.f[d.reps[d.prep[j]].sel];
+	const Rep &rp = d.reps[d.prep[j]];
+	if(j - rp.p_off >= rp.np) return;
+	double *F = d.f[rp.sel];
 	const int S = d.stride;
 	double O[9];
 #pragma unroll

@@ -33,11 +33,11 @@ namespace {
 // cells, more candidates per cell: the result is the same).
 constexpr int NA_MAX_N = PP_NOALLOC_MAX_N;
 constexpr int NA_MAX_CELLS = 1 << 18;
-// scan tiles of a grid of at most NA_MAX_CELLS cells: small tiles up to
-// SCAN_SMALL_MAX_TILES of them, large ones beyond
-constexpr int NA_MAX_TILES = SCAN_SMALL_MAX_TILES > (NA_MAX_CELLS + SCAN_TILE_LARGE - 1) / SCAN_TILE_LARGE
+// scan tiles of a grid of at most NA_MAX_CELLS cells and its parking cell:
+// small tiles up to SCAN_SMALL_MAX_TILES of them, large ones beyond
+constexpr int NA_MAX_TILES = SCAN_SMALL_MAX_TILES > (NA_MAX_CELLS + SCAN_TILE_LARGE) / SCAN_TILE_LARGE
                                  ? SCAN_SMALL_MAX_TILES
-                                 : (NA_MAX_CELLS + SCAN_TILE_LARGE - 1) / SCAN_TILE_LARGE;
+                                 : (NA_MAX_CELLS + SCAN_TILE_LARGE) / SCAN_TILE_LARGE;
 
 constexpr size_t align256(size_t x) { return (x + 255) & ~(size_t) 255; }
 
@@ -59,7 +59,7 @@ constexpr size_t part_bytes(int p) {
 	case P_BONDS: return sizeof(int) * (size_t) NA_MAX_N;
 	case P_OVL: return (size_t) NA_MAX_N;
 	case P_COUNT:
-	case P_START: return sizeof(int) * ((size_t) NA_MAX_CELLS + 1);
+	case P_START: return sizeof(int) * ((size_t) NA_MAX_CELLS + 2);  // the grid, the parking cell, the end
 	case P_STATUS: return sizeof(unsigned long long) * (size_t) NA_MAX_TILES;
 	case P_TILESUM: return sizeof(int) * (size_t) NA_MAX_TILES;
 	case P_TICKET: return sizeof(int) * 2;
@@ -189,6 +189,8 @@ extern "C" pp_status pp_energy_snapshot_noalloc(int n, const double *r, const do
 	rp.V = V;
 	rp.p_off = 0;
 	rp.np = n;
+	rp.cap_np = n;
+	rp.np_hw = n;
 	rp.cell_off = 0;
 	rp.cell_cap = (int) cells;
 	rp.act_cap = (gn[0] / 2) * (gn[1] / 2) * (gn[2] / 2);
@@ -206,7 +208,7 @@ extern "C" pp_status pp_energy_snapshot_noalloc(int n, const double *r, const do
 	d.prep = (const int *) at(P_PREP);
 	d.act_off = (const int *) at(P_ACTOFF);
 	d.total_act = rp.act_cap;
-	d.C_total = (int) cells;
+	d.C_total = (int) cells + 1;  // the grid and the replica's (empty) parking cell
 	d.count = (int *) at(P_COUNT);
 	d.start = (int *) at(P_START);
 	d.slot = (int *) at(P_SLOT);

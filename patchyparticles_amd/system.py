@@ -10,7 +10,7 @@ import numpy as np
 
 from ._lib import NET_BIASES, NET_INPUTS, NET_OUTPUTS, NET_WEIGHTS, Counters, Options, Params, PPError, _arr, _ptr, check, lib
 
-NVT, NPT = 0, 5  # defs.h:21-25
+NVT, GC, NPT = 0, 1, 5  # defs.h:21-25
 
 
 class System:
@@ -22,12 +22,13 @@ class System:
     """
 
     def __init__(self, r, O, box, *, T, disp_max=0.05, theta_max=0.1, kf_delta=0.12, kf_cosmax=0.92,
-                 ensemble=NVT, P=0.0, rescale_factor_max=0.0, seed=1, threads_per_cell=0, block_size=0,
-                 cell_occupancy=0.0, device=0, launch=0, trials_per_cell=0):
+                 ensemble=NVT, P=0.0, rescale_factor_max=0.0, activity=0.0, gc_n_max=0, seed=1, threads_per_cell=0,
+                 block_size=0, cell_occupancy=0.0, device=0, launch=0, trials_per_cell=0):
         self._h = None
         self._make([r], [O], [box], [dict(T=T, disp_max=disp_max, theta_max=theta_max, kf_delta=kf_delta,
                                           kf_cosmax=kf_cosmax, ensemble=ensemble, P=P,
-                                          rescale_factor_max=rescale_factor_max)],
+                                          rescale_factor_max=rescale_factor_max, activity=activity,
+                                          gc_n_max=gc_n_max)],
                    seed, threads_per_cell, block_size, cell_occupancy, device, launch, trials_per_cell)
 
     @classmethod
@@ -54,6 +55,8 @@ class System:
             P[k].kf_cosmax = float(p.get("kf_cosmax", 0.92))
             P[k].pressure = float(p.get("P", 0.0))
             P[k].rescale_factor_max = float(p.get("rescale_factor_max", 0.0))
+            P[k].activity = float(p.get("activity", 0.0))
+            P[k].gc_n_max = int(p.get("gc_n_max", 0))
         o = Options(int(seed) & 0xFFFFFFFFFFFFFFFF, int(tpc), int(block), float(occ), int(device), int(launch),
                     int(tpc_trials))
         r_all = np.concatenate([_arr(r) for r in rs]) if R else np.zeros(0)
@@ -64,14 +67,18 @@ class System:
                                     np.ascontiguousarray(O_all), np.ascontiguousarray(b_all), C.byref(h)))
         self._h = h
         self.R = R
+        self._gc = [P[k].ensemble == GC for k in range(R)]
         self._theta_max0 = float(P[0].theta_max) if R else 0.1
         self._net = None
 
     def _n(self, rep):
-        """Particles of replica rep; an index out of range is refused as the
-        C ABI refuses it (PP_ERR_ARG), not wrapped around."""
+        """Particles of replica rep now (a grand-canonical replica's count
+        changes); an index out of range is refused as the C ABI refuses it
+        (PP_ERR_ARG), not wrapped around."""
         if not 0 <= int(rep) < self.R:
             raise PPError(1, f"replica {rep} out of range")
+        if self._gc[rep]:
+            return self.counters(rep)["n"]
         return self.n[rep]
 
     def close(self):
@@ -122,16 +129,19 @@ class System:
         return r[:3 * n].reshape(n, 3), O[:9 * n].reshape(n, 3, 3), box
 
     def upload(self, rep, r, O, box=None):
+        """Replace replica rep's particles (a grand-canonical replica takes
+        any number up to its gc_n_max)."""
         r = _arr(r)
         O = _arr(O)
         b = None if box is None else _arr(box)
-        check(lib().pp_upload(self._h, rep, _ptr(r), _ptr(O), _ptr(b)))
+        check(lib().pp_upload_n(self._h, rep, r.size // 3, _ptr(r), _ptr(O), _ptr(b)))
 
     def counters(self, rep=0):
         c = Counters()
         check(lib().pp_get_counters(self._h, rep, C.byref(c)))
         return dict(tries=list(c.tries), accepted=list(c.accepted), bonds=c.bonds, sweeps=c.sweeps,
-                    box=list(c.box), volume=c.volume)
+                    box=list(c.box), volume=c.volume, n=c.n, gc_tries=list(c.gc_tries),
+                    gc_accepted=list(c.gc_accepted))
 
     def counters_all(self):
         """Counters of every replica: dict of numpy arrays (one entry per replica)."""
@@ -139,7 +149,9 @@ class System:
         check(lib().pp_get_counters_all(self._h, cs))
         return dict(tries=np.array([list(c.tries) for c in cs]), accepted=np.array([list(c.accepted) for c in cs]),
                     bonds=np.array([c.bonds for c in cs]), box=np.array([list(c.box) for c in cs]),
-                    volume=np.array([c.volume for c in cs]))
+                    volume=np.array([c.volume for c in cs]), n=np.array([c.n for c in cs]),
+                    gc_tries=np.array([list(c.gc_tries) for c in cs]),
+                    gc_accepted=np.array([list(c.gc_accepted) for c in cs]))
 
     def reset_counters(self):
         check(lib().pp_reset_counters(self._h))

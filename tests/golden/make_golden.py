@@ -31,6 +31,7 @@ executables.  Outputs are data only (inputs and expected outputs):
     python tests/golden/make_golden.py c1         (c1_ref.json only)
     python tests/golden/make_golden.py cold       (cold_ref.json only)
     python tests/golden/make_golden.py lx         (lx_ref.json only)
+    python tests/golden/make_golden.py gc         (gc_ref.json only)
 
   c1_ref.json       E/N series of 8 reference runs at the reference-runnable
                     configuration (N=1000, rho=0.1, T=0.1, 10^5 sweeps)
@@ -38,6 +39,7 @@ executables.  Outputs are data only (inputs and expected outputs):
                     distribution, down to T=0.08
   lx_ref.json       <E/N> and <Lx> of 8 reference runs with box-shape moves
                     (Lx_move = 1) under the reference's acceptance test
+  gc_ref.json       <N/V> and <E/N> of 8 grand-canonical reference runs
 """
 from __future__ import annotations
 
@@ -577,7 +579,56 @@ def lx_ref():
         json.dump(out, f, indent=1)
 
 
+# Grand-canonical runs (Ensemble = 1, MC_add_remove, MC.c:342-448): T=0.2,
+# Activity 0.2 in a box of side 10 from the generator's N=100 start, GC_N_max
+# 1000; the mean density and E/N of 8 runs over their second halves.
+GC_POINT = dict(n0=100, rho0=0.1, T=0.2, activity=0.2, n_max=1000)
+GC_RUNS, GC_SWEEPS, GC_PRINT = 8, 60000, 100
+
+
+def gc_ref():
+    build_ref()
+    exe = os.path.join(ROOT, "oracle", "_ref", "PatchyParticles_fast")
+    sp = GC_POINT
+    d0 = tempfile.mkdtemp(prefix="gc_")
+    subprocess.run([os.path.join(ROOT, "oracle", "_ref", "generator"), str(sp["n0"]), str(sp["rho0"])], cwd=d0,
+                   check=True, capture_output=True)
+    procs = []
+    for j in range(GC_RUNS):
+        d = os.path.join(d0, f"run{j}")
+        os.makedirs(d)
+        keys = {"Dynamics": 0, "Ensemble": 1, "Seed": 9000 + j, "Temperature": sp["T"], "Disp_max": 0.05,
+                "Theta_max": 0.1, "KF_delta": 0.12, "KF_cosmax": 0.92, "Activity": sp["activity"],
+                "GC_N_max": sp["n_max"], "Initial_conditions_file": os.path.join(d0, "generated.rrr"),
+                "Steps": GC_SWEEPS, "Print_every": GC_PRINT, "Save_every": GC_SWEEPS, "Configuration_folder": "."}
+        with open(os.path.join(d, "input"), "w") as f:
+            f.write("".join(f"{a} = {b}\n" for a, b in keys.items()))
+        procs.append(subprocess.Popen([exe, "input"], cwd=d, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL))
+    for p in procs:
+        assert p.wait() == 0
+    e_runs, rho_runs = [], []
+    for j in range(GC_RUNS):
+        d = os.path.join(d0, f"run{j}")
+        e = np.loadtxt(os.path.join(d, "energy.dat"))[:, 1]
+        rho = np.loadtxt(os.path.join(d, "density.dat"))[:, 1]
+        e_runs.append(e[int(STAT_DISCARD * len(e)):].mean())
+        rho_runs.append(rho[int(STAT_DISCARD * len(rho)):].mean())
+    e_runs, rho_runs = np.array(e_runs), np.array(rho_runs)
+    with open(os.path.join(d0, "generated.rrr")) as f:
+        L = float(f.readline().split()[2])
+    out = dict(sp, box=L, runs=GC_RUNS, sweeps=GC_SWEEPS, print_every=GC_PRINT, discard=STAT_DISCARD,
+               E_per_N=float(e_runs.mean()), E_per_N_se=float(e_runs.std(ddof=1) / np.sqrt(GC_RUNS)),
+               density=float(rho_runs.mean()), density_se=float(rho_runs.std(ddof=1) / np.sqrt(GC_RUNS)),
+               E_runs=e_runs.tolist(), density_runs=rho_runs.tolist())
+    print("gc", {k: v for k, v in out.items() if not k.endswith("_runs")})
+    with open(os.path.join(HERE, "gc_ref.json"), "w") as f:
+        json.dump(out, f, indent=1)
+
+
 if __name__ == "__main__":
+    if sys.argv[1:] == ["gc"]:
+        gc_ref()
+        sys.exit(0)
     if sys.argv[1:] == ["lx"]:
         lx_ref()
         sys.exit(0)

@@ -77,7 +77,8 @@ int main(int argc, char **argv) {
 		part[4 * i] = i + 1;
 		part[4 * (i + 1)] = i;
 	}
-	ppo_print(&o, 0, c.n, -1.0, 100.0, c.box, 3, 4, 0, 1);
+	const long long gc_acc[2] = {0, 0}, gc_try[2] = {1, 1};
+	ppo_print(&o, 0, c.n, -1.0, 100.0, c.box, 3, 4, 0, 1, gc_acc, gc_try);
 	ppo_print_bonds(&o, 0, c.n, c.box, c.r, 0.12, cnt, part, 4);
 	free(cnt);
 	free(part);
