@@ -39,6 +39,10 @@
  *   pp_get_grid, pp_get_layout    no reference counterpart (launch-shape
  *                                 tuning, per-sweep energy series, kernel
  *                                 timing, grid introspection)
+ *   pp_colour_conflicts, pp_colour_check
+ *                                 no reference counterpart (the reference is
+ *                                 serial): the checkerboard invariant of the
+ *                                 concurrent colour passes, as a debug check
  *   pp_bond_list                  output_print_bonds (output.c:213-268)
  *   pp_set_lx_move                the Lx_move keys and MC_change_Lx
  *                                 (system.c:170-196, MC.c:515-585)
@@ -114,6 +118,9 @@ typedef struct pp_options {
 #define PP_LAUNCH_DEFAULT 0   /* PP_LAUNCH_PER_PASS */
 #define PP_LAUNCH_PER_PASS 1  /* one launch per colour pass */
 #define PP_LAUNCH_COOP 2      /* one cooperative launch per sweep (grid barriers between passes); fail if unsupported */
+#define PP_LAUNCH_DEPS 3      /* one launch per sweep in which each cell starts its pass as soon as its neighbours
+                                 of earlier passes are done; 32 or 64 threads per cell.  Results equal the
+                                 per-pass form's bit for bit (learned passes still launch per pass) */
 
 typedef struct pp_counters {
 	long long tries[2];    /* [0] rototranslations, [1] volume moves, since the last reset */
@@ -273,6 +280,23 @@ pp_status pp_get_grid(pp_ctx *ctx, int rep, int n_side[3], double side[3]);
  * replicas reserve room for their grids to grow), and the tiles the
  * rebuild's scan takes them in (scan_tile cells each, scan_tiles of them). */
 pp_status pp_get_layout(pp_ctx *ctx, long long *cell_capacity, int *scan_tile, int *scan_tiles);
+
+/* The checkerboard invariant (a debug check).  The particles of one colour's
+ * cells move concurrently, so none may lie within the bond range 1 + delta of
+ * a particle in another cell of the same colour (colour = the parities of the
+ * cell's three indices).  pp_colour_conflicts counts the particles of replica
+ * rep that do, on the grid of its current cell lists (nside NULL) or on
+ * nside[0] x nside[1] x nside[2] cells over its box without shift (any grid:
+ * a grid with an odd count or with cells narrower than 1 + delta has
+ * conflicts); brute force over the replica's pairs, minimum image. */
+pp_status pp_colour_conflicts(pp_ctx *ctx, int rep, const int nside[3], long long *count);
+/* Live checks: with on = 1 every later sweep counts the conflicts of every
+ * replica on its lists' grid after the rebuild and after each colour pass
+ * (after the whole launch in the PP_LAUNCH_COOP and PP_LAUNCH_DEPS forms), and
+ * adds them to a total (reset here); on = 0 stops the checks, on = -1 only
+ * reads.  *total and *checks (each may be NULL) receive the conflicts found
+ * and the replica checks made since the last reset. */
+pp_status pp_colour_check(pp_ctx *ctx, int on, long long *total, long long *checks);
 
 /* ---- learned proposal (no reference counterpart) --------------------------
  * A move of particle p: the descriptor of its neighbourhood (the 12 nearest

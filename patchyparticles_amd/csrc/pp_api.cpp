@@ -72,6 +72,10 @@ struct pp_ctx {
 	DevState d{};
 	int T = 32, block = 256;
 	int coop_blocks = 0;  // > 0: the passes of a sweep run as one cooperative launch of that many workgroups
+	bool dd = false;      // the passes of a sweep run as one dependency-driven launch
+	bool colour_check = false;  // live checkerboard checks in every sweep
+	long long colour_checks = 0;  // replica checks made since the last reset
+	unsigned dd_mark = 0;  // the mark the cells of the last dependency-driven sweep published
 	long long sweep = 0;
 	bool any_npt = false;
 	bool any_gc = false;  // a replica makes grand-canonical insertions and removals
@@ -89,7 +93,8 @@ struct pp_ctx {
 	pp_net *net = nullptr;
 	double learned_fraction = 0.;
 	std::vector<void *> allocs;
-	int *h_vslow = nullptr;  // pinned: whether the box trial in flight needs a trial rebuild
+	int *h_vslow = nullptr;  // pinned: the trial-energy paths the box attempt in flight needs (TrialNeed bits)
+	int cus = 256;           // compute units of the device
 
 	~pp_ctx() {
 		for(auto &p : pending) {
@@ -159,6 +164,11 @@ pp_status check_err(pp_ctx *c) {
 	if(e & ERR_GC_FULL)
 		return fail(PP_ERR_ARG, "The system contains too many particles for GC_N_max: a grand-canonical phase needs "
 		                        "N plus the cells of one colour in free slots, increase GC_N_max");
+	if(e & ERR_DEPWAIT) {
+		launch_fill_i32(c->d.err, 0, 1, c->stream);
+		HIPCHECK(hipStreamSynchronize(c->stream));
+		return fail(PP_ERR_HIP, "a neighbour wait of the dependency-driven sweep kernel timed out");
+	}
 	if(e & ERR_BARRIER) {
 		launch_fill_i32((int *) c->d.bar, 0, 2, c->stream);
 		launch_fill_i32(c->d.err, 0, 1, c->stream);
@@ -285,9 +295,11 @@ pp_status create_impl(int R, const pp_params *prm, const pp_options *opts, const
 	if(Tv != 16 && Tv != 32 && Tv != 64) return fail(PP_ERR_ARG, "threads_per_cell must be 16, 32 or 64");
 	if(o.block_size % 64 || o.block_size < 64 || o.block_size > 256)
 		return fail(PP_ERR_ARG, "block_size must be 64, 128, 192 or 256");
-	if(o.launch < PP_LAUNCH_DEFAULT || o.launch > PP_LAUNCH_COOP) return fail(PP_ERR_ARG, "launch must be 0, 1 or 2");
+	if(o.launch < PP_LAUNCH_DEFAULT || o.launch > PP_LAUNCH_DEPS) return fail(PP_ERR_ARG, "launch must be 0, 1, 2 or 3");
 	if(o.launch == PP_LAUNCH_COOP && Tv == 16)
 		return fail(PP_ERR_ARG, "the cooperative launch form takes 32 or 64 threads per cell");
+	if(o.launch == PP_LAUNCH_DEPS && Tv == 16)
+		return fail(PP_ERR_ARG, "the dependency-driven launch form takes 32 or 64 threads per cell");
 	if(o.trials_per_cell < 0 || o.trials_per_cell > MAX_TRIALS_PER_CELL)
 		return fail(PP_ERR_ARG, "trials_per_cell must be in [0, %d]", MAX_TRIALS_PER_CELL);
 	if(pass_lds_bytes(Tv, o.block_size) > MAX_LDS_BYTES)
@@ -441,6 +453,7 @@ pp_status create_impl(int R, const pp_params *prm, const pp_options *opts, const
 			return bail(fail(PP_ERR_UNSUPPORTED, "device %d cannot launch the sweep kernel cooperatively", o.device));
 		c->coop_blocks = std::min(need, fit);
 	}
+	c->dd = o.launch == PP_LAUNCH_DEPS;
 	pp_status st;
 	if((st = dev_alloc(c, &d.f[0], (size_t) NCOMP * d.stride)) || (st = dev_alloc(c, &d.f[1], (size_t) NCOMP * d.stride)) ||
 	   (st = dev_alloc(c, &d.pid[0], d.stride)) || (st = dev_alloc(c, &d.pid[1], d.stride)) ||
@@ -455,23 +468,32 @@ pp_status create_impl(int R, const pp_params *prm, const pp_options *opts, const
 	   (st = dev_alloc(c, &d.rec, (size_t) 8 * std::max(d.total_act, 1) * REC_W)) ||
 	   (st = dev_alloc(c, &d.ctr, (size_t) R * NCTR * NSLOT)) || (st = dev_alloc(c, &d.err, 1)) ||
 	   (st = dev_alloc(c, &d.out_bonds, d.stride)) || (st = dev_alloc(c, &d.out_ovl, d.stride)) ||
-	   (st = dev_alloc(c, &d.bar, 2)) || (st = dev_alloc(c, &d.vflag, R)) || (st = dev_alloc(c, &d.vslow, 1)))
+	   (st = dev_alloc(c, &d.bar, 2)) || (st = dev_alloc(c, &d.vflag, R)) || (st = dev_alloc(c, &d.vslow, 1)) ||
+	   (st = dev_alloc(c, &d.done, d.C_total + 1)) || (st = dev_alloc(c, &d.ddq, 1)) ||
+	   (st = dev_alloc(c, &d.ccheck, 2)))
 		return bail(st);
 	if(hipHostMalloc((void **) &c->h_vslow, sizeof(int), hipHostMallocDefault) != hipSuccess || !c->h_vslow) {
 		c->h_vslow = nullptr;
 		return bail(fail(PP_ERR_NOMEM, "pinned host flag allocation failed"));
 	}
-	// a diagnostic switch: every box trial regroups the particles into the
-	// spare buffer (the form without the fast path, for comparisons)
+	// diagnostic switches for the box trials' energy, which must give the
+	// same trajectories: PP_NPT_FULL_TRIAL=1 regroups the particles into the
+	// spare buffer for every trial; PP_NPT_PARTICLE_TRIAL=1 evaluates fast
+	// trials per particle instead of from the cell records
 	{
-		const char *e = getenv("PP_NPT_FULL_TRIAL");
-		d.npt_full_trial = (e && *e && strcmp(e, "0") != 0) ? 1 : 0;
+		auto on = [](const char *name) {
+			const char *e = getenv(name);
+			return e && *e && strcmp(e, "0") != 0;
+		};
+		d.npt_trial_form = on("PP_NPT_FULL_TRIAL") ? 2 : (on("PP_NPT_PARTICLE_TRIAL") ? 1 : 0);
 	}
+	if(hipDeviceGetAttribute(&c->cus, hipDeviceAttributeMultiprocessorCount, o.device) != hipSuccess || c->cus <= 0)
+		c->cus = 256;
 	{
 		// every buffer a kernel may touch must exist before the first launch
 		const void *need[] = {d.f[0], d.f[1], d.pid[0], d.pid[1], d.reps, d.prep, d.act_off, d.count, d.start,
 		                      d.slot, d.cellid, d.scan_status, d.tile_sum, d.clist, d.ccount, d.pfx, d.rec, d.scan_ticket,
-		                      d.ctr, d.err, d.out_bonds, d.out_ovl, d.bar, d.vflag, d.vslow};
+		                      d.ctr, d.err, d.out_bonds, d.out_ovl, d.bar, d.vflag, d.vslow, d.done, d.ddq, d.ccheck};
 		for(const void *q : need)
 			if(!q) return bail(fail(PP_ERR_NOMEM, "a device buffer was not allocated"));
 	}
@@ -505,6 +527,9 @@ pp_status create_impl(int R, const pp_params *prm, const pp_options *opts, const
 	launch_fill_i32((int *) d.bar, 0, 2, c->stream);
 	launch_fill_i32(d.vflag, 0, R, c->stream);
 	launch_fill_i32(d.vslow, 0, 1, c->stream);
+	launch_fill_i32((int *) d.done, 0, d.C_total + 1, c->stream);
+	launch_fill_i32(d.ddq, 0, 1, c->stream);
+	launch_fill_u64(d.ccheck, 0ull, 2, c->stream);
 	launch_fill_u64(d.ctr, 0ull, (long long) R * NCTR * NSLOT, c->stream);
 
 	// initial energy (main.c:54-61): any overlap is an error
@@ -557,6 +582,14 @@ bool learned_pass(uint64_t seed, uint64_t sw, int k, double frac) {
 	return (double) (z >> 11) * (1.0 / 9007199254740992.0) < frac;
 }
 
+// The live checkerboard checks (pp_colour_check): every replica on the grid
+// of its current lists, added to the running total.
+void colour_check_now(pp_ctx *c) {
+	if(!c->colour_check) return;
+	for(int k = 0; k < c->R; k++) launch_colour_conflicts(c->d, k, c->np[k], nullptr, c->d.ccheck, c->stream);
+	c->colour_checks += c->R;
+}
+
 // Enqueue sweep c->sweep on the context's stream (no synchronisation).
 pp_status enqueue_sweep(pp_ctx *c) {
 	const uint64_t sw = (uint64_t) c->sweep;
@@ -567,11 +600,25 @@ pp_status enqueue_sweep(pp_ctx *c) {
 	if(c->trace_len >= 0 && c->trace_len < c->d.trace_cap) tp = c->trace_len++;
 	launch_rebuild(c->d, REB_SHIFT, sw, tp, c->stream);
 	toc(c, TK_REBUILD, a);
+	colour_check_now(c);
 	if(c->coop_blocks > 0 && !c->net) {
 		hipEvent_t b{};
 		tic(c, TK_PASS, &b);
 		HIPCHECK(launch_sweep_passes(c->d, sw, c->T, c->block, c->coop_blocks, c->stream));
 		toc(c, TK_PASS, b);
+		colour_check_now(c);
+	}
+	else if(c->dd && !c->net) {
+		// a new mark per sweep; when the marks wrap, the old ones are cleared
+		if(++c->dd_mark == 0u) {
+			launch_fill_i32((int *) c->d.done, 0, c->d.C_total + 1, c->stream);
+			c->dd_mark = 1u;
+		}
+		hipEvent_t b{};
+		tic(c, TK_PASS, &b);
+		launch_passes_dd(c->d, sw, c->dd_mark, c->T, c->block, c->stream);
+		toc(c, TK_PASS, b);
+		colour_check_now(c);
 	}
 	else
 		for(int k = 0; k < 8; k++) {
@@ -586,6 +633,7 @@ pp_status enqueue_sweep(pp_ctx *c) {
 				launch_pass(c->d, sw, k, c->T, c->block, c->stream);
 				toc(c, TK_PASS, b);
 			}
+			colour_check_now(c);
 		}
 	// grand-canonical insertions and removals (MC_add_remove, MC.c:342-448)
 	// on the lists the passes leave; box-shape trials after them rebuild
@@ -602,11 +650,15 @@ pp_status enqueue_sweep(pp_ctx *c) {
 			launch_vol_propose(c->d, sw, k, c->stream);
 			// a trial rebuild only when some replica's trial box cannot be
 			// served by its current lists (MC.c:481-485 rebuilds only when
-			// the cells become too small)
+			// the cells become too small); then the trial energies, per
+			// particle and from the cell records, as the trials need them
 			HIPCHECK(hipMemcpyAsync(c->h_vslow, c->d.vslow, sizeof(int), hipMemcpyDeviceToHost, c->stream));
 			HIPCHECK(hipStreamSynchronize(c->stream));
-			if(*c->h_vslow) launch_rebuild(c->d, REB_TRIAL, sw, -1, c->stream);
-			launch_energy(c->d, 1, 0, c->stream);
+			const int need = *c->h_vslow;
+			if(need & NEED_REBUILD) launch_rebuild(c->d, REB_TRIAL, sw, -1, c->stream);
+			if(need & NEED_PARTICLES) launch_energy(c->d, 1, 0, c->stream);
+			else launch_energy_rows_zero(c->d, c->stream);
+			if(need & NEED_RECORDS) launch_energy_rec(c->d, c->cus, c->stream);
 			launch_vol_decide(c->d, sw, k, c->stream);
 			launch_vol_apply(c->d, c->stream);
 			toc(c, TK_VOLUME, v);
@@ -634,7 +686,8 @@ pp_status pp_sweeps(pp_ctx *c, long long nsw) {
 pp_status pp_autotune(pp_ctx *c, int sweeps, int *threads_per_cell, int *block_size, double *us_per_sweep) {
 	if(!c) return fail(PP_ERR_ARG, "ctx is NULL");
 	if(sweeps < 1) return fail(PP_ERR_ARG, "sweeps per candidate must be >= 1");
-	if(c->coop_blocks > 0) return fail(PP_ERR_UNSUPPORTED, "the autotuner works on the per-pass launch form");
+	if(c->coop_blocks > 0 || c->dd)
+		return fail(PP_ERR_UNSUPPORTED, "the autotuner works on the per-pass launch form");
 	HIPCHECK(hipSetDevice(c->device));
 	TRY(collect_timing(c));
 	const int saved_every = c->timing_every;
@@ -1332,6 +1385,40 @@ pp_status pp_get_grid(pp_ctx *c, int rep, int n_side[3], double side[3]) {
 		if(n_side) n_side[a] = rp.g.n[a];
 		if(side) side[a] = rp.g.box[a] / rp.g.n[a];
 	}
+	return PP_OK;
+}
+
+pp_status pp_colour_conflicts(pp_ctx *c, int rep, const int nside[3], long long *count) {
+	if(!c || !count) return fail(PP_ERR_ARG, "bad arguments");
+	if(rep < 0 || rep >= c->R) return fail(PP_ERR_ARG, "replica %d out of range", rep);
+	if(nside)
+		for(int k = 0; k < 3; k++)
+			if(nside[k] < 1) return fail(PP_ERR_ARG, "nside[%d] = %d: at least one cell per axis", k, nside[k]);
+	HIPCHECK(hipSetDevice(c->device));
+	launch_fill_u64(c->d.ccheck + 1, 0ull, 1, c->stream);
+	launch_colour_conflicts(c->d, rep, c->np[rep], nside, c->d.ccheck + 1, c->stream);
+	unsigned long long v = 0;
+	HIPCHECK(hipMemcpyAsync(&v, c->d.ccheck + 1, sizeof(v), hipMemcpyDeviceToHost, c->stream));
+	HIPCHECK(hipStreamSynchronize(c->stream));
+	*count = (long long) v;
+	return check_err(c);
+}
+
+pp_status pp_colour_check(pp_ctx *c, int on, long long *total, long long *checks) {
+	if(!c) return fail(PP_ERR_ARG, "ctx is NULL");
+	if(on < -1 || on > 1) return fail(PP_ERR_ARG, "on must be -1, 0 or 1");
+	HIPCHECK(hipSetDevice(c->device));
+	unsigned long long v = 0;
+	HIPCHECK(hipMemcpyAsync(&v, c->d.ccheck, sizeof(v), hipMemcpyDeviceToHost, c->stream));
+	HIPCHECK(hipStreamSynchronize(c->stream));
+	if(total) *total = (long long) v;
+	if(checks) *checks = c->colour_checks;
+	if(on == 1) {
+		launch_fill_u64(c->d.ccheck, 0ull, 1, c->stream);
+		HIPCHECK(hipStreamSynchronize(c->stream));
+		c->colour_checks = 0;
+	}
+	if(on >= 0) c->colour_check = on == 1;
 	return PP_OK;
 }
 

@@ -79,7 +79,7 @@ struct LearnDev {
 };
 
 // device error bits
-enum Err : int { ERR_CELL_CAP = 1, ERR_BOX = 2, ERR_SCAN = 8, ERR_BARRIER = 16, ERR_GC_FULL = 32 };
+enum Err : int { ERR_CELL_CAP = 1, ERR_BOX = 2, ERR_SCAN = 8, ERR_BARRIER = 16, ERR_GC_FULL = 32, ERR_DEPWAIT = 64 };
 
 // rebuild modes
 enum Reb : int { REB_KEEP = 0, REB_SHIFT = 1, REB_TRIAL = 2 };
@@ -93,6 +93,9 @@ constexpr int SCAN_LOOKBACK_MAX_TILES = 64;
 constexpr int SCAN_SPIN_MAX = 1 << 22;
 // polls of a grid barrier before it reports ERR_BARRIER and lets the workgroup go on
 constexpr int BAR_SPIN_MAX = 1 << 21;
+// polls of a dependency-driven sweep's wait for neighbour cells before it
+// reports ERR_DEPWAIT and lets the team go on
+constexpr int DD_SPIN_MAX = 1 << 20;
 
 // A cell grid over one replica's (periodic, orthorhombic) box.
 struct Grid {
@@ -149,6 +152,11 @@ struct Rep {
 	Grid lg;
 	int lists_ok;
 	int trial_fast;  // the trial in flight is evaluated on the current lists (no trial rebuild)
+	// rec_ok: the cell records of this sweep's passes describe the current
+	// lists (cleared by any other rebuild); a fast trial then takes its
+	// energy from the records, pair by pair (trial_rec)
+	int rec_ok;
+	int trial_rec;
 	int vol_apply;   // an accepted fast trial: positions still to be rescaled in place
 	int lx_ref_rule;  // box-shape moves accepted with exp(+dE/T) as MC.c:562 (not Metropolis)
 	double lx_change_max, lyz_min, lyz_max;
@@ -212,7 +220,14 @@ struct DevState {
 	uint64_t seed;
 	int *vflag;        // [R] a trial energy found an overlap: the replica's other teams stop
 	int *vslow;        // [1] some replica's trial needs a trial rebuild
-	int npt_full_trial;  // every box trial takes the trial rebuild (no fast path)
+	// how box trials find their energy: 0 the fastest path available (cell
+	// records, else the current lists per particle, else a trial rebuild);
+	// 1 never the records; 2 always a trial rebuild (diagnostic forms that
+	// must give the same trajectories)
+	int npt_trial_form;
+	unsigned *done;    // [C_total] dependency-driven sweeps: mark of the sweep whose pass the cell last finished
+	int *ddq;          // [1] dependency-driven sweeps: tickets taken (reset by the rebuild)
+	unsigned long long *ccheck;  // [1] checkerboard conflicts found by the live checks
 };
 
 // launchers (pp_kernels.hip)
@@ -222,6 +237,9 @@ void launch_fill_u64(unsigned long long *p, unsigned long long v, long long n, h
 // `sweep` and records the energy trace entry trace_pos (if >= 0)
 void launch_rebuild(const DevState &d, int mode, uint64_t sweep, long long trace_pos, hipStream_t s);
 void launch_pass(const DevState &d, uint64_t sweep, int pass, int T, int block, hipStream_t s);
+// the 8 colour passes of a sweep in one dependency-driven launch; cells
+// publish `mark` (never 0, different from the previous sweep's) when done
+void launch_passes_dd(const DevState &d, uint64_t sweep, unsigned mark, int T, int block, hipStream_t s);
 // learned proposal (pp_learn.hip)
 void launch_learned_pass(const DevState &d, const NetDev &net, uint64_t sweep, int pass, hipStream_t s);
 void launch_net_forward(const NetDev &net, const __bf16 *x, int m, const int *m_dev, float *y, __bf16 *h,
@@ -249,6 +267,20 @@ void launch_vol_propose(const DevState &d, uint64_t sweep, int k, hipStream_t s)
 void launch_vol_decide(const DevState &d, uint64_t sweep, int k, hipStream_t s);
 // rescale the positions of replicas whose fast trial was accepted
 void launch_vol_apply(const DevState &d, hipStream_t s);
+// the energy of the box trials whose replicas take it from the cell records
+// (trial_rec), one listed cell per team; `cus` sizes the grid
+void launch_energy_rec(const DevState &d, int cus, hipStream_t s);
+// checkerboard invariant (debug): add to *out the particles of replica rep
+// within the bond range of a particle in another cell of the same colour, on
+// the grid of its current lists (nside == nullptr) or on nside cells per axis
+void launch_colour_conflicts(const DevState &d, int rep, int slots, const int *nside, unsigned long long *out,
+                             hipStream_t s);
+// zero the energy rows (C_EBONDS, C_OVL) of every replica
+void launch_energy_rows_zero(const DevState &d, hipStream_t s);
+// the trial-energy paths a box attempt needs, as the host reads them from
+// vslow after k_vol_propose: a trial rebuild, the per-particle trial energy,
+// the per-record trial energy
+enum TrialNeed : int { NEED_REBUILD = 1, NEED_PARTICLES = 2, NEED_RECORDS = 4 };
 // grand-canonical insertions and removals of the sweep (GC replicas only)
 void launch_gc_phase(const DevState &d, uint64_t sweep, hipStream_t s);
 void launch_gram_schmidt(const DevState &d, hipStream_t s);

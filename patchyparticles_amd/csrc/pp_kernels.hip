@@ -133,6 +133,7 @@ __global__ void k_count(DevState d, int mode, uint64_t sweep, long long trace_po
 		rp.lg = rp.g;
 		for(int k = 0; k < 3; k++) rp.lg.shift[k] = sh[k];
 		rp.lists_ok = 1;
+		rp.rec_ok = mode == REB_SHIFT;  // only that rebuild writes the cell records
 	}
 	// a free or removed slot of a grand-canonical replica goes to the
 	// replica's parking cell
@@ -349,9 +350,11 @@ __global__ __launch_bounds__(256) void k_scan(DevState d, int flip, int lists, i
 
 // The layout of the passes' lists, after the scan has counted them:
 // pfx[k][r] = first entry of replica r in the list of pass k (one thread per
-// pass; the kernel boundary makes every workgroup's counts visible).
+// pass; the kernel boundary makes every workgroup's counts visible).  Also
+// resets the ticket queue of the dependency-driven sweep.
 __global__ void k_pass_prefix(DevState d) {
 	const int t = threadIdx.x;
+	if(t == 0 && d.ddq) *d.ddq = 0;  // the dependency-driven sweep's tickets
 	if(t < 8) {
 		int acc = 0;
 		d.pfx[t * (d.R + 1)] = 0;
@@ -475,6 +478,7 @@ struct PassArgs {
 	DevState d;
 	uint64_t sweep;
 	int pass;
+	unsigned mark;  // dependency-driven sweeps: the value a finished cell publishes in done[]
 };
 
 // Per-team LDS (doubles): the stage row of a trial's O' and patches, the
@@ -582,13 +586,64 @@ __device__ __forceinline__ double rot_value(const RotShare &r, double o0, double
 // counts cost about what the trials save.
 constexpr int KEEP_MIN_TRIALS = 4;
 
+// A read of another cell's particle data during a pass.  In the
+// dependency-driven sweep (DD: the passes of a sweep in one launch, each cell
+// starting once its neighbours of earlier passes are done) it is an
+// agent-scope load, served where the writes of every XCD meet, so that no
+// XCD's L2 can return a copy older than the neighbour's finished pass.
+template <bool DD>
+__device__ __forceinline__ double nbld(const double *p) {
+	if constexpr(DD) return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+	else return *p;
+}
+
+// Wait (DD) until every non-empty neighbour cell that an earlier pass of this
+// sweep works on has published `mark` in done[]: lane nb < 26 of the team
+// watches neighbour nb of the cell record (direction m = nb < 13 ? nb : nb +
+// 1, offsets m / 9 - 1, (m / 3) % 3 - 1, m % 3 - 1).  Neighbours of later
+// passes wait for this cell in turn, so none of them writes before this
+// cell's reads.  The wait is bounded: past DD_SPIN_MAX polls the team reports
+// ERR_DEPWAIT and goes on, so the launch always drains.
+template <int T>
+__device__ __forceinline__ void wait_neighbours(const DevState &d, const Rep &rp, const int *nb_tab, int pass, int ix,
+                                                int iy, int iz, int n0, int n1, int n2, unsigned mark, int lane) {
+	bool need = false;
+	const unsigned *w = nullptr;
+	if(lane < 26) {
+		const int m = lane < 13 ? lane : lane + 1;
+		const int cx = wrapi(ix + m / 9 - 1, n0), cy = wrapi(iy + (m / 3) % 3 - 1, n1), cz = wrapi(iz + m % 3 - 1, n2);
+		const int ncol = ((cx & 1) << 2) | ((cy & 1) << 1) | (cz & 1);
+		int np = 0;
+		while(np < 7 && (int) ((rp.col_perm >> (4 * np)) & 0xFu) != ncol) np++;
+		need = np < pass && nb_tab[32 + lane + 1] > nb_tab[32 + lane];
+		w = d.done + rp.cell_off + (cx * n1 + cy) * n2 + cz;
+	}
+	int spins = 0;
+	while(team_any<T>(need)) {
+		if(need) need = __hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != mark;
+		if(++spins > DD_SPIN_MAX) {
+			if(lane == 0) atomicOr(d.err, (int) ERR_DEPWAIT);
+			break;
+		}
+		if(team_any<T>(need)) __builtin_amdgcn_s_sleep(1);
+	}
+	// no load of the neighbours' data moves above the wait
+	__atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
+
 // KEEP (fixed trials per cell, many per member): every member's bonds with
 // the static neighbours are counted up front and kept as it moves, so a trial
 // tests only the new position against them.  Without KEEP (about one trial
 // per member) every trial tests both positions.
-template <int T, int CPL, bool KEEP, bool ONE>
+//
+// DD: the pass is one of several in flight (k_pass_dd).  The cell's own
+// loads and its trials come first; the team then waits for its neighbours of
+// earlier passes, reads their data with agent-scope loads, and finally
+// writes its members back with agent-scope stores, waits for them to
+// complete, and publishes `mark` for its own cell.
+template <int T, int CPL, bool KEEP, bool ONE, bool DD>
 __device__ __forceinline__ void pass_cell(const DevState &d, uint64_t sweep, int pass, int gi, int lane,
-                                          double *blk) {
+                                          double *blk, unsigned mark) {
 	// the team's block, the per-trial data first (short LDS offsets)
 	double *stage = blk;                    // LDS_STAGE doubles
 	double *trials = blk + LDS_STAGE;       // LDS_TRIAL
@@ -615,7 +670,13 @@ __device__ __forceinline__ void pass_cell(const DevState &d, uint64_t sweep, int
 		const int sel = uni<T>(nb_tab[63]);
 		// (a team of T < CELL_CAP lanes holds one member per lane)
 		if(cnt > CELL_CAP || cnt > T) {
-			if(lane == 0) atomicOr(d.err, (int) ERR_CELL_CAP);
+			if(lane == 0) {
+				atomicOr(d.err, (int) ERR_CELL_CAP);
+				// (DD) the neighbours of later passes do not wait for it
+				if constexpr(DD)
+					__hip_atomic_store(d.done + d.reps[r].cell_off + cloc, mark, __ATOMIC_RELAXED,
+					                   __HIP_MEMORY_SCOPE_AGENT);
+			}
 			return;
 		}
 		const Rep &rp = d.reps[r];
@@ -648,24 +709,27 @@ __device__ __forceinline__ void pass_cell(const DevState &d, uint64_t sweep, int
 		// rest (dense neighbourhoods only) are read again at every trial
 		double cxr[CPL], cyr[CPL], czr[CPL];
 		int cqr[CPL];
-#pragma unroll
-		for(int u = 0; u < CPL; u++) {
-			const int i = lane + u * T;
-			cqr[u] = -1;
-			cxr[u] = cyr[u] = czr[u] = 0.;
-			if(i < M) {
-				const int q = nb_slot(i);
-				cqr[u] = q;
-				cxr[u] = F[q];
-				cyr[u] = F[S + q];
-				czr[u] = F[2 * S + q];
-			}
-		}
 		double orv[CPL][9];
+		auto load_static = [&]() {
 #pragma unroll
-		for(int u = 0; u < CPL; u++)
+			for(int u = 0; u < CPL; u++) {
+				const int i = lane + u * T;
+				cqr[u] = -1;
+				cxr[u] = cyr[u] = czr[u] = 0.;
+				if(i < M) {
+					const int q = nb_slot(i);
+					cqr[u] = q;
+					cxr[u] = nbld<DD>(F + q);
+					cyr[u] = nbld<DD>(F + S + q);
+					czr[u] = nbld<DD>(F + 2 * S + q);
+				}
+			}
 #pragma unroll
-			for(int c = 0; c < 9; c++) orv[u][c] = cqr[u] >= 0 ? F[(3 + c) * S + cqr[u]] : 0.;
+			for(int u = 0; u < CPL; u++)
+#pragma unroll
+				for(int c = 0; c < 9; c++) orv[u][c] = cqr[u] >= 0 ? nbld<DD>(F + (3 + c) * S + cqr[u]) : 0.;
+		};
+		if constexpr(!DD) load_static();
 		Trial tr;
 		tr.Lx = unid<T>(rp.g.box[0]);
 		tr.Ly = unid<T>(rp.g.box[1]);
@@ -703,7 +767,7 @@ __device__ __forceinline__ void pass_cell(const DevState &d, uint64_t sweep, int
 		static_assert(MAX_TRIALS_PER_CELL <= 32 && CELL_CAP <= 32, "one lane per trial");
 		const int koff = (threadIdx.x & 63) & ~(T - 1);
 		if(lane < cnt) own_pid[lane] = my_pid;
-		if constexpr(!REG_ORI) {
+		if constexpr(!REG_ORI && !DD) {
 #pragma unroll
 			for(int u = 0; u < CPL; u++)
 #pragma unroll
@@ -725,6 +789,18 @@ __device__ __forceinline__ void pass_cell(const DevState &d, uint64_t sweep, int
 				for(int c = 0; c < 3; c++) w[NCOMP + 3 * i + c] = P[i][c];
 		}
 		team_sync();
+		if constexpr(DD) {
+			// the neighbours' data once their earlier passes are done
+			wait_neighbours<T>(d, rp, nb_tab, pass, ix, iy, iz, n0, n1, n2, mark, lane);
+			load_static();
+			if constexpr(!REG_ORI) {
+#pragma unroll
+				for(int u = 0; u < CPL; u++)
+#pragma unroll
+					for(int c = 0; c < 9; c++) ori[(9 * u + c) * T + lane] = orv[u][c];
+			}
+			team_sync();
+		}
 		// this lane's static neighbours (the surrounding cells, fixed during
 		// the pass): the one in registers and, in dense neighbourhoods, the
 		// rest from HBM
@@ -740,9 +816,9 @@ __device__ __forceinline__ void pass_cell(const DevState &d, uint64_t sweep, int
 				for(int b = T * CPL; b < M; b += T)
 					if(b + lane < M) {
 						const int q = nb_slot(b + lane);
-						f(F[q], F[S + q], F[2 * S + q], [&](double *Oq) {
+						f(nbld<DD>(F + q), nbld<DD>(F + S + q), nbld<DD>(F + 2 * S + q), [&](double *Oq) {
 #pragma unroll
-							for(int c = 0; c < 9; c++) Oq[c] = F[(3 + c) * S + q];
+							for(int c = 0; c < 9; c++) Oq[c] = nbld<DD>(F + (3 + c) * S + q);
 						});
 					}
 		};
@@ -933,7 +1009,7 @@ __device__ __forceinline__ void pass_cell(const DevState &d, uint64_t sweep, int
 				for(int b = T * CPL; b < M; b += T)
 					if(b + lane < M) {
 						const int q = nb_slot(b + lane);
-						ov |= overlaps_new(tr, F[q], F[S + q], F[2 * S + q]);
+						ov |= overlaps_new(tr, nbld<DD>(F + q), nbld<DD>(F + S + q), nbld<DD>(F + 2 * S + q));
 					}
 			if(team_any<T>(ov)) continue;
 
@@ -1061,13 +1137,25 @@ __device__ __forceinline__ void pass_cell(const DevState &d, uint64_t sweep, int
 		if(c_acc > 0 || cnt > 1) {
 			for(int e = lane; e < cnt * NCOMP; e += T) {
 				int m = e / NCOMP, c = e - m * NCOMP;
-				F[c * S + s0 + m] = own[m * OWN_W + c];
+				if constexpr(DD)
+					__hip_atomic_store(F + c * S + s0 + m, own[m * OWN_W + c], __ATOMIC_RELAXED,
+					                   __HIP_MEMORY_SCOPE_AGENT);
+				else F[c * S + s0 + m] = own[m * OWN_W + c];
 			}
 			if(lane < cnt) {
 				int rank = 0;
 				for(int m = 0; m < cnt; m++) rank += own_pid[m] < my_pid;
 				PID[s0 + rank] = my_pid;
 			}
+		}
+		if constexpr(DD) {
+			// the members' stores have completed (the wavefront's memory
+			// counter is zero) before the cell is published as done
+			__atomic_signal_fence(__ATOMIC_SEQ_CST);
+			__builtin_amdgcn_s_waitcnt(0);
+			__atomic_signal_fence(__ATOMIC_SEQ_CST);
+			if(lane == 0)
+				__hip_atomic_store(d.done + rp.cell_off + cloc, mark, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 		}
 		if(lane == 0) {
 			const int slot = gi & (NSLOT - 1);
@@ -1097,7 +1185,42 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
 	if(i >= per_xcd) return;
 	const int gi = uni<T>(((blockIdx.x & 7) * per_xcd + i) * tpb + team);
 	if(gi >= total) return;
-	pass_cell<T, CPL, KEEP, ONE>(a.d, a.sweep, a.pass, gi, lane, lds + team * team_lds<T>());
+	pass_cell<T, CPL, KEEP, ONE, false>(a.d, a.sweep, a.pass, gi, lane, lds + team * team_lds<T>(), 0u);
+}
+
+// The 8 colour passes of a sweep in one launch, driven by dependencies
+// instead of kernel boundaries.  The sweep's work is one queue of tickets:
+// the list of pass 0, then pass 1, ..., each padded to whole wavefronts so
+// that the teams of one wavefront always work on one pass.  Each wavefront
+// takes the next tickets when it starts; each team then runs its cell as the
+// per-pass kernel does, after waiting for its neighbours of earlier passes
+// (wait_neighbours), and publishes its own cell.  A cell's neighbours of
+// earlier passes hold earlier tickets, taken by wavefronts that started
+// before it and are running or done, so the work always progresses whatever
+// is resident; the waits are bounded besides.  The grid covers the largest
+// possible lists; wavefronts whose tickets lie past the sweep's end leave at
+// once.  Results equal the per-pass launches' bit for bit: each cell sees its
+// neighbourhood as the earlier passes left it, and its trials do not depend
+// on timing.
+template <int T, int CPL, bool KEEP, bool ONE>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_pass_dd(PassArgs a) {
+	extern __shared__ double lds[];
+	const DevState &d = a.d;
+	constexpr int TPW = 64 / T;  // teams per wavefront
+	const int lane = threadIdx.x & (T - 1);
+	const int team = threadIdx.x / T;
+	int t0 = 0;
+	if((threadIdx.x & 63) == 0) t0 = atomicAdd(d.ddq, TPW);
+	const int t = __builtin_amdgcn_readfirstlane(t0) + (threadIdx.x & 63) / T;
+	int p = 0, base = 0, n = 0;
+	for(; p < 8; p++) {
+		n = d.pfx[p * (d.R + 1) + d.R];
+		const int padded = (n + TPW - 1) / TPW * TPW;
+		if(t < base + padded) break;
+		base += padded;
+	}
+	if(p == 8 || t - base >= n) return;
+	pass_cell<T, CPL, KEEP, ONE, true>(d, a.sweep, p, t - base, lane, lds + team * team_lds<T>(), a.mark);
 }
 
 // Grid-wide barrier of a cooperative launch (every workgroup resident).
@@ -1145,8 +1268,37 @@ __global__ __launch_bounds__(256) void k_sweep_passes(PassArgs a) {
 	double *own = lds + team * team_lds<T>();
 	for(int k = 0; k < 8; k++) {
 		const int total = uni<T>(d.pfx[k * (d.R + 1) + d.R]);
-		for(int gi = g0; gi < total; gi += nteams) pass_cell<T, CPL, KEEP, false>(d, a.sweep, k, gi, lane, own);
+		for(int gi = g0; gi < total; gi += nteams) pass_cell<T, CPL, KEEP, false, false>(d, a.sweep, k, gi, lane, own, 0u);
 		if(k < 7) grid_barrier(d.bar, d.err);
+	}
+}
+
+template <int T, int CPL, bool KEEP, bool ONE>
+static void pass_dd_T(const DevState &d, uint64_t sweep, unsigned mark, int block, hipStream_t s) {
+	const int tpb = block / T;
+	// every pass's largest list, one team per entry (wavefronts past the
+	// sweep's tickets leave at once)
+	const unsigned nb = (unsigned) std::max(1, 8 * ((d.total_act + tpb - 1) / tpb));
+	size_t lds = (size_t) tpb * team_lds<T>() * sizeof(double);
+	PassArgs a{d, sweep, 0, mark};
+	hipLaunchKernelGGL((k_pass_dd<T, CPL, KEEP, ONE>), dim3(nb), dim3(block), lds, s, a);
+}
+
+template <int T, bool KEEP>
+static void pass_dd_TK(const DevState &d, uint64_t sweep, unsigned mark, int block, hipStream_t s) {
+	if(d.R == 1) pass_dd_T<T, 1, KEEP, true>(d, sweep, mark, block, s);
+	else pass_dd_T<T, 1, KEEP, false>(d, sweep, mark, block, s);
+}
+
+void launch_passes_dd(const DevState &d, uint64_t sweep, unsigned mark, int T, int block, hipStream_t s) {
+	const bool keep = d.trials_per_cell >= KEEP_MIN_TRIALS;
+	if(T == 32) {
+		if(keep) pass_dd_TK<32, true>(d, sweep, mark, block, s);
+		else pass_dd_TK<32, false>(d, sweep, mark, block, s);
+	}
+	else {
+		if(keep) pass_dd_TK<64, true>(d, sweep, mark, block, s);
+		else pass_dd_TK<64, false>(d, sweep, mark, block, s);
 	}
 }
 
@@ -1156,7 +1308,7 @@ static void pass_T(const DevState &d, uint64_t sweep, int pass, int block, hipSt
 	int nb = (d.total_act + tpb - 1) / tpb;
 	nb = (nb + 7) & ~7;  // 8 XCDs x the most workgroups one XCD may need
 	size_t lds = (size_t) tpb * team_lds<T>() * sizeof(double);
-	PassArgs a{d, sweep, pass};
+	PassArgs a{d, sweep, pass, 0u};
 	hipLaunchKernelGGL((k_pass<T, CPL, KEEP, ONE>), dim3(nb), dim3(block), lds, s, a);
 }
 
@@ -1209,7 +1361,7 @@ int sweep_blocks_max(int T, int block, int cus) {
 
 template <int T, int CPL, bool KEEP>
 static hipError_t sweep_T(const DevState &d, uint64_t sweep, int block, int blocks, hipStream_t s) {
-	PassArgs a{d, sweep, 0};
+	PassArgs a{d, sweep, 0, 0u};
 	void *args[] = {&a};
 	return hipLaunchCooperativeKernel((const void *) k_sweep_passes<T, CPL, KEEP>, dim3(blocks), dim3(block), args,
 	                                  (unsigned) pass_lds_bytes(T, block), s);
@@ -1241,6 +1393,41 @@ struct NbLayout {
 // whatever the bonds are: the replica's later teams stop at once.
 // Workgroups reach the XCDs round-robin: each XCD takes a contiguous range of
 // slots (neighbouring cells share its L2).
+// The bond decision of a pair of particles closer than the bond range but
+// not overlapping (MC_would_interact's angular part, MC.c:248-262), seen from
+// p with separation (dx, dy, dz) towards q and p's patches P: the filtered
+// test, and the reference's arithmetic only inside the filter's band or
+// without the filter.  Seen from q the decision is the same (every step is
+// the exact negation of p's).
+template <class LoadO>
+__device__ __forceinline__ bool pair_bonded(double dx, double dy, double dz, double d2, const double *P, double c2,
+                                            double cosmax, LoadO load_q) {
+	bool unsure = c2 < 0.;
+	bool bonded = false;
+	if(!unsure) {
+		const KfBand bd = kf_band(c2, d2);
+		const unsigned fp = kf_fast_p(dx, dy, dz, P, bd);
+		unsure = (fp >> 1) != 0u;
+		if(fp & 1u) {
+			double Oq[9];
+			load_q(Oq);
+			const unsigned fq = kf_fast_q1(dx, dy, dz, kf_band_q(bd), Oq);
+			unsure |= (fq >> 1) != 0u;
+			if(!unsure) bonded = (fq & 1u) != 0u;
+		}
+	}
+	if(unsure) {
+		kf_unit(dx, dy, dz, d2);
+		bonded = false;
+		if(kf_any_p(dx, dy, dz, P, cosmax)) {
+			double Oq[9];
+			load_q(Oq);
+			bonded = kf_any_q(dx, dy, dz, Oq, cosmax);
+		}
+	}
+	return bonded;
+}
+
 template <int T>
 __global__ __launch_bounds__(256) void k_energy(DevState d, int trial, int per_particle) {
 	using NB = NbLayout<T>;
@@ -1253,6 +1440,7 @@ __global__ __launch_bounds__(256) void k_energy(DevState d, int trial, int per_p
 	if(j - rp.p_off >= rp.np) return;  // a free slot (the live particles come first after a rebuild)
 	const bool tr = trial && rp.trial_active;
 	if(trial && !tr) return;
+	if(tr && rp.trial_rec) return;  // k_energy_rec takes this trial
 	if(tr && __hip_atomic_load(&d.vflag[r], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
 	const bool fast = tr && rp.trial_fast;
 	const Grid &G = tr ? (fast ? rp.lg : rp.tg) : rp.g;  // the grid of the lists' membership
@@ -1292,34 +1480,11 @@ __global__ __launch_bounds__(256) void k_energy(DevState d, int trial, int per_p
 			double dz = kf_min_image(F[2 * S + q] * s2 - pz, Lz, iLz);
 			double d2 = dx * dx + dy * dy + dz * dz;
 			if(d2 < 1.) ov = true;
-			else if(d2 < rp.rc2) {
-				// the filtered angular test (kf_fast_p / kf_fast_q1); the
-				// reference's arithmetic only inside the band or without the
-				// filter
-				bool unsure = c2 < 0.;
-				if(!unsure) {
-					const KfBand bd = kf_band(c2, d2);
-					const unsigned fp = kf_fast_p(dx, dy, dz, &P[0][0], bd);
-					unsure = (fp >> 1) != 0u;
-					if(fp & 1u) {
-						double Oq[9];
+			else if(d2 < rp.rc2)
+				nbond += (int) pair_bonded(dx, dy, dz, d2, &P[0][0], c2, rp.cosmax, [&](double *Oq) {
 #pragma unroll
-						for(int c = 0; c < 9; c++) Oq[c] = F[(3 + c) * S + q];
-						const unsigned fq = kf_fast_q1(dx, dy, dz, kf_band_q(bd), Oq);
-						unsure |= (fq >> 1) != 0u;
-						if(!unsure) nbond += (int) (fq & 1u);
-					}
-				}
-				if(unsure) {
-					kf_unit(dx, dy, dz, d2);
-					if(kf_any_p(dx, dy, dz, &P[0][0], rp.cosmax)) {
-						double Oq[9];
-#pragma unroll
-						for(int c = 0; c < 9; c++) Oq[c] = F[(3 + c) * S + q];
-						if(kf_any_q(dx, dy, dz, Oq, rp.cosmax)) nbond++;
-					}
-				}
-			}
+					for(int c = 0; c < 9; c++) Oq[c] = F[(3 + c) * S + q];
+				});
 		}
 	}
 	nbond = team_sum<T>(nbond);
@@ -1394,8 +1559,12 @@ void launch_bond_list(const DevState &d, int max_per, int *count, int *partners,
 // Lanes per particle: 32 where the 27 cells hold many candidates, 8 in dilute
 // systems (fewer than 8 candidates on average over the cell capacity), where
 // most of a 32-lane team would idle.  The result does not depend on it.
-void launch_energy(const DevState &d, int trial, int per_particle, hipStream_t s) {
+void launch_energy_rows_zero(const DevState &d, hipStream_t s) {
 	hipLaunchKernelGGL(k_zero_energy_rows, dim3((d.R * 2 * NSLOT + 255) / 256), dim3(256), 0, s, d.ctr, d.R);
+}
+
+void launch_energy(const DevState &d, int trial, int per_particle, hipStream_t s) {
+	launch_energy_rows_zero(d, s);
 	if(d.Ntot <= 0) return;
 	const int block = 256;
 	const double cand = 27. * (double) d.Ntot / (double) (d.C_total > 0 ? d.C_total : 1);
@@ -1403,6 +1572,170 @@ void launch_energy(const DevState &d, int trial, int per_particle, hipStream_t s
 	auto grid = [&](int per) { return dim3((unsigned) ((((long long) d.Ntot + per - 1) / per + 7) & ~7LL)); };
 	if(cand < 8.) hipLaunchKernelGGL(k_energy<8>, grid(block / 8), dim3(block), 0, s, d, trial, per_particle);
 	else hipLaunchKernelGGL(k_energy<32>, grid(block / 32), dim3(block), 0, s, d, trial, per_particle);
+}
+
+// The energy of box trials on the current lists, from the sweep's cell
+// records instead of per particle: listed cell i (of the 8 passes' lists one
+// after another) pairs its members with each other once and with the
+// members of its neighbours in the 13 directions m > 13 of the record (m =
+// 9 (dx + 1) + 3 (dy + 1) + dz + 1), so every pair of particles in adjacent
+// cells is seen exactly once.  The separation is formed from the cell's
+// member as k_energy forms it from either particle (scaled positions, the
+// trial box) and pair_bonded decides the bond, so a bonded pair adds 2 to
+// the bond rows (the per-particle sum counts it twice) and an overlapping
+// pair adds 1 to the overlap row: k_vol_decide reads the same totals.  Per
+// listed cell this reads its record once, where k_energy reads 27 cell ranges
+// per particle; on dilute systems most of them are empty.
+// TT lanes per listed cell: 8 on dense systems; 1 on dilute ones, where a
+// listed cell holds about one particle and one candidate, so that the lanes
+// stream through the records instead of idling in teams.
+template <int TT>
+__global__ __launch_bounds__(256) void k_energy_rec(DevState d) {
+	const int lane = threadIdx.x & (TT - 1);
+	const int teams = gridDim.x * (blockDim.x / TT);
+	int lens[8];
+	int total = 0;
+#pragma unroll
+	for(int p = 0; p < 8; p++) {
+		lens[p] = d.pfx[p * (d.R + 1) + d.R];
+		total += lens[p];
+	}
+	for(int i0 = xcd_remap(blockIdx.x, gridDim.x) * (blockDim.x / TT) + threadIdx.x / TT; i0 < total; i0 += teams) {
+		int p = 0, gi = i0;
+		while(gi >= lens[p]) gi -= lens[p++];
+		const int *rec = d.rec + ((size_t) p * d.total_act + gi) * REC_W;
+		const int r = rec[62];
+		const Rep &rp = d.reps[r];
+		if(!rp.trial_active || !rp.trial_rec) continue;
+		// (the early exit may read a stale flag: it only saves work)
+		if(__hip_atomic_load(&d.vflag[r], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) continue;
+		const int s0 = rec[59], cnt = rec[60];
+		const int m0 = rec[32 + 13], m2 = rec[58] - m0;  // candidates in the 13 directions
+		const double *F = d.f[rec[63]];
+		const int S = d.stride;
+		const double t0 = rp.trial_s[0], t1 = rp.trial_s[1], t2 = rp.trial_s[2];
+		const double Lx = rp.tg.box[0], Ly = rp.tg.box[1], Lz = rp.tg.box[2];
+		const double iLx = rp.tg.inv_box[0], iLy = rp.tg.inv_box[1], iLz = rp.tg.inv_box[2];
+		const double c2 = rp.cosmax >= KF_FILTER_MIN_COS ? rp.cosmax * rp.cosmax : -1.;
+		int nbond = 0, novl = 0;
+		const int work = cnt * m2 + cnt * cnt;
+		for(int e = lane; e < work; e += TT) {
+			int j, q;
+			if(e < cnt * m2) {
+				j = s0 + e / m2;
+				const int k = e % m2 + m0;  // prefix position among the record's 26 neighbours
+				int nb = 13;
+				while(nb < 25 && rec[32 + nb + 1] <= k) nb++;
+				q = rec[nb] + (k - rec[32 + nb]);
+			}
+			else {
+				const int a = (e - cnt * m2) / cnt, b2 = (e - cnt * m2) % cnt;
+				if(a >= b2) continue;
+				j = s0 + a;
+				q = s0 + b2;
+			}
+			const double px = F[j] * t0, py = F[S + j] * t1, pz = F[2 * S + j] * t2;
+			const double dx = kf_min_image(F[q] * t0 - px, Lx, iLx);
+			const double dy = kf_min_image(F[S + q] * t1 - py, Ly, iLy);
+			const double dz = kf_min_image(F[2 * S + q] * t2 - pz, Lz, iLz);
+			const double d2 = dx * dx + dy * dy + dz * dz;
+			if(d2 < 1.) novl++;
+			else if(d2 < rp.rc2) {
+				double O[9], P[4][3];
+#pragma unroll
+				for(int c = 0; c < 9; c++) O[c] = F[(3 + c) * S + j];
+				kf_patches(O, P);
+				nbond += (int) pair_bonded(dx, dy, dz, d2, &P[0][0], c2, rp.cosmax, [&](double *Oq) {
+#pragma unroll
+					for(int c = 0; c < 9; c++) Oq[c] = F[(3 + c) * S + q];
+				});
+			}
+		}
+		if constexpr(TT > 1) {
+			nbond = team_sum<TT>(nbond);
+			novl = team_sum<TT>(novl);
+		}
+		if(lane == 0) {
+			const int slot = i0 & (NSLOT - 1);
+			if(novl) {
+				__hip_atomic_store(&d.vflag[r], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+				atomicAdd(ctr_at(d.ctr, r, C_OVL, slot), (unsigned long long) novl);
+			}
+			else if(nbond)
+				atomicAdd(ctr_at(d.ctr, r, C_EBONDS, slot), 2ull * (unsigned long long) nbond);
+		}
+	}
+}
+
+void launch_energy_rec(const DevState &d, int cus, hipStream_t s) {
+	// a grid-stride loop over the listed cells: 8 workgroups per CU (a
+	// multiple of 8: one contiguous range of cells per XCD); one lane per
+	// cell when the 27 cells around a particle hold fewer than 8 candidates
+	const int blocks = std::max(8, cus * 8) & ~7;
+	const double cand = 27. * (double) d.Ntot / (double) (d.C_total > 0 ? d.C_total : 1);
+	if(cand < 8.) hipLaunchKernelGGL(k_energy_rec<1>, dim3((unsigned) blocks), dim3(256), 0, s, d);
+	else hipLaunchKernelGGL(k_energy_rec<8>, dim3((unsigned) blocks), dim3(256), 0, s, d);
+}
+
+// ------------------------------------------------------------------------
+// Checkerboard invariant (a debug check; the reference is serial).  The
+// particles of one colour's cells move concurrently, so no particle may lie
+// within the bond range of a particle in another cell of the same colour.
+// Counts the particles of replica `rep` that do, on the grid G (cells by
+// cell_coord, colours by the cells' parities, separations by minimum image
+// in G's box): brute force over the replica's live slots, every colour at
+// once since each particle belongs to one.  One thread per slot.
+__global__ void k_colour_conflicts(DevState d, int rep, int n0, int n1, int n2, unsigned long long *out) {
+	const Rep &rp = d.reps[rep];
+	// the grid of the current cell lists, or n0 x n1 x n2 cells over the box
+	Grid G = rp.lg;
+	if(n0 > 0) {
+		set_grid_box(G, rp.g.box);
+		G.n[0] = n0;
+		G.n[1] = n1;
+		G.n[2] = n2;
+		G.shift[0] = G.shift[1] = G.shift[2] = 0.;
+	}
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if(i >= rp.np_hw) return;
+	const int j = rp.p_off + i;
+	const int *PID = d.pid[rp.sel];
+	if(PID[j] < 0) return;
+	const double *F = d.f[rp.sel];
+	const int S = d.stride;
+	const double x = F[j], y = F[S + j], z = F[2 * S + j];
+	auto cell = [&](double a, double b, double c, int &cx, int &cy, int &cz) {
+		cx = cell_coord(a, G.shift[0], G.inv_box[0], G.n[0]);
+		cy = cell_coord(b, G.shift[1], G.inv_box[1], G.n[1]);
+		cz = cell_coord(c, G.shift[2], G.inv_box[2], G.n[2]);
+	};
+	int ix, iy, iz;
+	cell(x, y, z, ix, iy, iz);
+	const int col = ((ix & 1) << 2) | ((iy & 1) << 1) | (iz & 1);
+	bool hit = false;
+	for(int k = 0; k < rp.np_hw && !hit; k++) {
+		const int q = rp.p_off + k;
+		if(q == j || PID[q] < 0) continue;
+		const double qx = F[q], qy = F[S + q], qz = F[2 * S + q];
+		int cx, cy, cz;
+		cell(qx, qy, qz, cx, cy, cz);
+		if((((cx & 1) << 2) | ((cy & 1) << 1) | (cz & 1)) != col || (cx == ix && cy == iy && cz == iz)) continue;
+		const double dx = kf_min_image(qx - x, G.box[0], G.inv_box[0]);
+		const double dy = kf_min_image(qy - y, G.box[1], G.inv_box[1]);
+		const double dz = kf_min_image(qz - z, G.box[2], G.inv_box[2]);
+		hit = dx * dx + dy * dy + dz * dz < rp.rc2;
+	}
+	if(hit) atomicAdd(out, 1ull);
+}
+
+// nside == nullptr: the grid of the replica's current cell lists; else
+// nside cells per axis over its box, no shift.  `slots`: the replica's slot
+// capacity (the grid covers it; threads past its live slots leave).
+void launch_colour_conflicts(const DevState &d, int rep, int slots, const int *nside, unsigned long long *out,
+                             hipStream_t s) {
+	const int n = std::max(slots, 1);
+	hipLaunchKernelGGL(k_colour_conflicts, dim3((unsigned) ((n + 255) / 256)), dim3(256), 0, s, d, rep,
+	                   nside ? nside[0] : 0, nside ? nside[1] : 0, nside ? nside[2] : 0, out);
 }
 
 // ------------------------------------------------------------------------
@@ -1416,10 +1749,14 @@ void launch_energy(const DevState &d, int trial, int per_particle, hipStream_t s
 // on its own grid into the spare buffer; the host launches that rebuild only
 // when some replica needs it.
 __device__ __forceinline__ void choose_trial_path(const DevState &d, Rep &rp, const double tb[3]) {
-	bool fast = !d.npt_full_trial && rp.lists_ok;
+	bool fast = d.npt_trial_form < 2 && rp.lists_ok;
 	for(int a = 0; a < 3; a++) fast = fast && tb[a] / (double) rp.lg.n[a] >= rp.side_min;
 	rp.trial_fast = fast ? 1 : 0;
-	if(!fast) atomicOr(d.vslow, 1);
+	// a fast trial takes its energy from the cell records while they describe
+	// the lists (the sweep's shifting rebuild wrote them)
+	rp.trial_rec = (fast && rp.rec_ok && d.npt_trial_form == 0) ? 1 : 0;
+	atomicOr(d.vslow, fast ? (rp.trial_rec ? (int) NEED_RECORDS : (int) NEED_PARTICLES)
+	                       : (int) (NEED_REBUILD | NEED_PARTICLES));
 }
 
 __global__ void k_vol_propose(DevState d, uint64_t sweep, int k) {
@@ -1428,6 +1765,7 @@ __global__ void k_vol_propose(DevState d, uint64_t sweep, int k) {
 	Rep &rp = d.reps[r];
 	rp.trial_active = 0;
 	rp.trial_fast = 0;
+	rp.trial_rec = 0;
 	rp.vol_apply = 0;
 	d.vflag[r] = 0;
 	if(rp.ensemble != 5 && !rp.lx_move) return;
@@ -1524,9 +1862,11 @@ __global__ void k_vol_decide(DevState d, uint64_t sweep, int k) {
 		}
 		else {
 			// the spare buffer holds the trial, grouped on the trial grid
+			// (the cell records describe the old lists)
 			rp.sel ^= 1;
 			rp.lg = rp.tg;
 			rp.lists_ok = 1;
+			rp.rec_ok = 0;
 			rp.np_hw = rp.np;
 		}
 		rp.g = rp.tg;
@@ -1604,6 +1944,7 @@ __global__ void k_gc_begin(DevState d, uint64_t sweep) {
 			rp.gc_base = rp.np;
 			rp.np_hw = rp.np + ncol;
 			rp.lists_ok = 0;  // the phase adds and removes particles the lists do not show
+			rp.rec_ok = 0;
 		}
 	}
 }

@@ -168,6 +168,25 @@ class System:
         check(lib().pp_get_layout(self._h, C.byref(cap), C.byref(tile), C.byref(tiles)))
         return cap.value, tile.value, tiles.value
 
+    def colour_conflicts(self, rep=0, nside=None):
+        """Particles of replica rep within the bond range of a particle in
+        another cell of the same colour: on the grid of the current cell lists
+        (nside None) or on nside = (nx, ny, nz) cells over the box (a debug
+        check of the checkerboard invariant; 0 on every grid the sweeps use)."""
+        n = C.c_longlong(0)
+        arg = None if nside is None else (C.c_int * 3)(*[int(v) for v in nside])
+        check(lib().pp_colour_conflicts(self._h, int(rep), arg, C.byref(n)))
+        return n.value
+
+    def colour_check(self, on=None):
+        """Live checkerboard checks after the rebuild and every colour pass of
+        each later sweep: on=True starts them (and resets the totals),
+        on=False stops them, None only reads.  Returns (conflicts, replica
+        checks) since the last reset."""
+        t, k = C.c_longlong(0), C.c_longlong(0)
+        check(lib().pp_colour_check(self._h, -1 if on is None else (1 if on else 0), C.byref(t), C.byref(k)))
+        return t.value, k.value
+
     def set_trace(self, capacity):
         """Record the bonded-pair count after each of the next `capacity` sweeps."""
         self._trace_cap = int(capacity)

@@ -1,7 +1,9 @@
 """Box trials evaluated on the current cell lists (no trial rebuild while
-the lists' cells, rescaled with the trial, stay at least 1 + delta wide)
-against the form that regroups every trial into the spare buffer
-(PP_NPT_FULL_TRIAL=1): the same trajectories bit for bit -- positions,
+the lists' cells, rescaled with the trial, stay at least 1 + delta wide),
+with their energy taken pair by pair from the sweep's cell records (the
+default) or per particle (PP_NPT_PARTICLE_TRIAL=1), against the form that
+regroups every trial into the spare buffer (PP_NPT_FULL_TRIAL=1): the same
+trajectories bit for bit -- positions,
 orientations, boxes, volumes and move counters -- in NPT (MC.c:450-513) and
 with box-shape moves (MC.c:515-585), for one system and for a batch that
 mixes NPT and NVT replicas, whose trials alternate between the two forms as
@@ -19,8 +21,9 @@ def pp():
     return pp
 
 
-def _run(pp, monkeypatch, full, make, steps, lx=None):
-    monkeypatch.setenv("PP_NPT_FULL_TRIAL", "1" if full else "0")
+def _run(pp, monkeypatch, form, make, steps, lx=None):
+    monkeypatch.setenv("PP_NPT_FULL_TRIAL", "1" if form == "full" else "0")
+    monkeypatch.setenv("PP_NPT_PARTICLE_TRIAL", "1" if form == "particles" else "0")
     with make() as s:
         if lx:
             s.set_lx_move(*lx)
@@ -64,9 +67,10 @@ def test_box_trials_on_current_lists_keep_the_trajectory(pp, oracle, monkeypatch
         r, O, box = oracle.generate(400, 0.1, seed=23)
         make = lambda: pp.System(r, O, box, T=0.15, seed=6)
         steps, lx = (400, (2.0, 0.85 * box[1], 1.15 * box[1]))
-    full = _run(pp, monkeypatch, True, make, steps, lx)
-    fast = _run(pp, monkeypatch, False, make, steps, lx)
+    full = _run(pp, monkeypatch, "full", make, steps, lx)
+    fast = _run(pp, monkeypatch, "records", make, steps, lx)
     _same_runs(full, fast)
+    _same_runs(full, _run(pp, monkeypatch, "particles", make, steps, lx))
     c = fast[1]
     assert c["accepted"][:, 1].sum() > 0 and c["accepted"][:, 1].sum() < c["tries"][:, 1].sum()
     for k, (rr, OO, bb) in enumerate(fast[0]):
