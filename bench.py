@@ -9,15 +9,23 @@ every non-empty cell makes --trials-per-cell trials (0: one per member, so a
 step is exactly N attempts).  The metric counts sweeps of N attempted moves
 (MC.c:21): value = attempts / N / second.
 
-Prints one JSON line: sweeps/s (value), the dominant kernel's achieved
-algorithmic bandwidth against the HBM roofline (HIP events on the kernel's
-own stream, inside the timed region), decorrelated samples/s of E/N, and the
-reference's own CPU code timed on this host (cpu_baseline).  Decorrelated
-samples/s = sweeps/s / (2 tau_int[E/N]); every tau_int run (plain, plain with
-one trial per cell, learned) starts from the state the timed region ends in
-and runs the same number of sweeps, untimed for the plain one, and is given
-with its statistical error, its length in units of tau and the drift of E/N
-across it.
+The system is equilibrated (--equil steps, 40000 by default, about 5e5
+sweeps) before the warmup and the timed steps, whatever --warmup is, so the
+headline is timed at the state point.  HIP events on the pass's own stream
+bracket the kernels of every step of a short timed run (256 evenly spaced
+steps of a long one) for the roofline line.
+
+Prints one JSON line: sweeps/s (value), the dominant kernel's launch time and
+bytes against the HBM roofline (bound "latency": each cell's trials are one
+dependent chain), decorrelated samples/s of E/N, and the reference's own CPU
+code timed on this host (cpu_baseline).  Decorrelated samples/s = sweeps/s /
+(2 tau_int[E/N]), with tau_int pooled over 8 independent copies of an
+equilibrated state in one batched context; a run whose tau has a relative
+error above 0.25 or whose E/N drifts by 2 standard errors or more across it
+reports null and "non-stationary" instead.  Three such measurements: the
+headline form at its own state point; the learned proposal against the plain
+move at a stationary state point (rho=0.5, --stat-T, --stat-n particles per
+copy); and the same at the gel state point (N=65536, rho=0.3, T=0.09).
 """
 from __future__ import annotations
 
@@ -37,6 +45,7 @@ sys.path.insert(0, ROOT)
 N, RHO, T, SEED = 32768, 0.5, 0.12, 1
 DELTA, COSMAX, DISP, THETA = 0.12, 0.92, 0.05, 0.1
 HBM_PEAK_GBS = 8000.0
+CLOCK_GHZ = 2.4  # MI355X peak engine clock
 
 
 def algorithmic_bytes_per_move(rho, side, delta=DELTA):
@@ -110,173 +119,135 @@ def measured_traffic(path=os.path.join(ROOT, "profiles", "k_pass_n32768_rho0.5_p
     return None
 
 
-def series_stats(e, steps_per_sweep):
-    """tau_int[E/N] of a per-step E/N series in sweeps, with its statistical
-    error, the run's length in sweeps and in units of tau, and the drift of
-    E/N (second half's mean minus the first's, with the block standard error
-    of that difference): a drift of several standard errors means the run is
-    not stationary and tau is overestimated."""
+def decorrelation(pp, start, prm, *, seed, trials_per_cell, sweeps, replicas=8, net=None, learned_fraction=0.0,
+                  label=""):
+    """tau_int[E/N] of one sampler form from `replicas` independent copies of
+    the state `start` = (r, O, box) in one batched context, each copy with its
+    own random streams: the E/N series of every copy after every step
+    (pp_set_trace), pooled by stats.tau_int_replicas; the drift of E/N across
+    the run from the spread over the copies (stats.replica_drift).  The rate
+    is the batch's own: sweeps of one copy per second (attempts / N / copies
+    / s).  decorrelated_samples_per_s = that rate / (2 tau) is given only when
+    the run passes the stationarity rule (tau's relative error below 0.25 and
+    |drift| below 2 standard errors); otherwise it is null and `status` says
+    "non-stationary" (the drift) or "tau error above 0.25" (a run too short
+    for its tau)."""
     import numpy as np
-    from patchyparticles_amd.stats import block_mean, tau_int_err
-    tau_s, tau_e, _ = tau_int_err(e)
-    tau = tau_s / steps_per_sweep
-    h = len(e) // 2
-    m1, se1 = block_mean(e[:h])
-    m2, se2 = block_mean(e[h:])
-    run = len(e) / steps_per_sweep
-    return {"tau_int_sweeps": round(tau, 2), "tau_int_err_sweeps": round(tau_e / steps_per_sweep, 2),
-            "run_sweeps": round(run), "run_over_tau": round(run / tau, 1), "E_per_N": round(float(e.mean()), 5),
-            "E_drift": round(m2 - m1, 5), "E_drift_se": round(float(np.hypot(se1, se2)), 5)}
-
-
-def tau_leg(pp, r_eq, O_eq, box_eq, opts, trials_per_cell, sweeps, attempts_per_step):
-    """tau_int[E/N] of the plain sampler from the equilibrated state over
-    `sweeps` sweeps (untimed; its rate is the timed region's)."""
-    steps = max(1, int(math.ceil(sweeps * N / attempts_per_step)))
-    with pp.System(r_eq, O_eq, box_eq, seed=SEED + 4, trials_per_cell=trials_per_cell, **opts) as s:
-        s.do_ensemble(50)
+    from patchyparticles_amd.stats import replica_drift, stationary, tau_int_replicas
+    r, O, box = start
+    n = len(r)
+    with pp.System.batch([r] * replicas, [O] * replicas, [box] * replicas, [prm] * replicas, seed=seed,
+                         trials_per_cell=trials_per_cell) as s:
+        if net is not None:
+            s.attach(net, learned_fraction=learned_fraction, theta_max=prm.get("theta_max", THETA))
+        s.do_ensemble(20)
+        c0 = s.counters_all()
+        s.do_ensemble(20)
+        per_step = float((s.counters_all()["tries"][:, 0] - c0["tries"][:, 0]).mean()) / n / 20
+        steps = max(64, int(math.ceil(sweeps / per_step)))
         s.reset_counters()
         s.set_trace(steps)
+        t0 = time.perf_counter()
         s.do_ensemble(steps)
-        c = s.counters()
-        e = s.trace() / N
-    return series_stats(e, N * steps / c["tries"][0])
-
-
-def learned_leg(args, pp, r_eq, O_eq, box_eq, opts):
-    """The learned proposal against the plain move on the same schedule (one
-    trial per non-empty cell and pass), from the equilibrated state: a
-    ProposalNet trained on --train-steps snapshots of the plain sampler (one
-    Adam step per snapshot, --train-batch moves each), then --learned-steps
-    timed steps with --learned-fraction of the colour passes learned and the
-    rest plain; the plain run times the same steps without the network.
-    Decorrelated samples/s = sweeps/s / (2 tau_int[E/N])."""
-    def timed(net):
-        with pp.System(r_eq, O_eq, box_eq, seed=SEED + 2, trials_per_cell=1, **opts) as s:
-            if net is not None:
-                s.attach(net, learned_fraction=args.learned_fraction, theta_max=THETA)
-            s.do_ensemble(50)
-            s.reset_counters()
-            s.set_timing(args.time_every)
-            s.set_trace(args.learned_steps)
-            t0 = time.perf_counter()
-            s.do_ensemble(args.learned_steps)
-            el = time.perf_counter() - t0
-            c = s.counters()
-            e = s.trace() / N
-            nl, ml = s.timing("learned")
-            tl, al = s.learn_counters(0)
-            b, ov, tot, anyo = s.energy()
-            assert not anyo and tot == c["bonds"], "tracked energy disagrees with a full recompute"
-        sw = c["tries"][0] / N / el
-        st = series_stats(e, N * args.learned_steps / c["tries"][0])
-        out = {"sweeps_per_s": round(sw, 2), **st,
-               "decorrelated_samples_per_s": round(sw / (2 * st["tau_int_sweeps"]), 4),
-               "acceptance": round(c["accepted"][0] / max(c["tries"][0], 1), 5)}
+        el = time.perf_counter() - t0
+        c = s.counters_all()
+        X = np.stack([s.trace(k) for k in range(replicas)], 1) / n
+        for k in range(replicas):
+            b, ov, tot, anyo = s.energy(k)
+            assert not anyo and tot == c["bonds"][k], "tracked energy disagrees with a full recompute"
         if net is not None:
-            out["learned_acceptance"] = round(al / max(tl, 1), 5)
-            out["learned_pass_us"] = round(ml / max(nl, 1) * 1e3, 3)
-        return out
+            lt, la = zip(*[s.learn_counters(k) for k in range(replicas)])
+    sweeps_per_step = float(c["tries"][:, 0].mean()) / n / steps
+    tau_s, err_s, _ = tau_int_replicas(X)
+    tau, err = tau_s * sweeps_per_step, err_s * sweeps_per_step
+    drift, drift_se = replica_drift(X)
+    ok = stationary(tau, err, drift, drift_se)
+    still = abs(drift) < 2.0 * drift_se
+    rate = float(c["tries"][:, 0].sum()) / n / replicas / el
+    out = {"form": label, "replicas": replicas, "sweeps_per_s": round(rate, 2), "run_sweeps": round(steps * sweeps_per_step),
+           "tau_int_sweeps": round(tau, 2), "tau_int_err_sweeps": round(err, 2),
+           "run_over_tau": round(steps * sweeps_per_step / tau, 1), "E_per_N": round(float(X.mean()), 5),
+           "E_drift": round(drift, 6), "E_drift_se": round(drift_se, 6),
+           "acceptance": round(float(c["accepted"][:, 0].sum() / max(c["tries"][:, 0].sum(), 1)), 5),
+           "status": "stationary" if ok else ("tau error above 0.25" if still else "non-stationary"),
+           "decorrelated_samples_per_s": round(rate / (2 * tau), 4) if ok else None}
+    if net is not None:
+        out["learned_acceptance"] = round(sum(la) / max(sum(lt), 1), 5)
+    print(f"# tau {label}: {out}", file=sys.stderr, flush=True)
+    return out
 
-    print("# learned proposal: training ...", file=sys.stderr, flush=True)
-    net = pp.ProposalNet(seed=SEED)
-    with pp.System(r_eq, O_eq, box_eq, seed=SEED + 3, trials_per_cell=args.trials_per_cell, **opts) as s:
+
+def ratio(a, b):
+    """a / b of two decorrelation results with its standard error (from the
+    errors of both tau estimates; the rates' own noise is far smaller), or
+    None when either is not stationary."""
+    if not a or not b or a["decorrelated_samples_per_s"] is None or b["decorrelated_samples_per_s"] is None:
+        return None
+    q = a["decorrelated_samples_per_s"] / b["decorrelated_samples_per_s"]
+    rel = math.hypot(a["tau_int_err_sweeps"] / a["tau_int_sweeps"], b["tau_int_err_sweeps"] / b["tau_int_sweeps"])
+    return {"value": round(q, 4), "se": round(q * rel, 4)}
+
+
+def train_net(pp, start, prm, args, seed):
+    """A ProposalNet trained on --train-steps snapshots of the plain sampler
+    at the state point of `start` (one Adam step per snapshot)."""
+    net = pp.ProposalNet(seed=seed)
+    with pp.System(*start, seed=seed + 3, trials_per_cell=8, **prm) as s:
         t0 = time.perf_counter()
         l0, l1 = s.train(net, steps=args.train_steps, batch=args.train_batch, lr=1e-3, pos_noise=0.03,
-                         theta_max=THETA, sweeps_between=1)
+                         theta_max=prm.get("theta_max", THETA), sweeps_between=1)
         t_train = time.perf_counter() - t0
-    print(f"# trained in {t_train:.1f} s (loss {l0:.3f} -> {l1:.3f}); timing learned and plain ...", file=sys.stderr,
+    print(f"# trained in {t_train:.1f} s (loss {l0:.3f} -> {l1:.3f})", file=sys.stderr, flush=True)
+    return net, {"steps": args.train_steps, "batch": args.train_batch, "seconds": round(t_train, 2),
+                 "loss_first": round(l0, 4), "loss_last": round(l1, 4)}
+
+
+def learned_vs_plain(pp, args, n, rho, temp, equil_steps, sweeps, label):
+    """The learned proposal against the plain move at one state point: a
+    seeded start equilibrated by --equil steps of the plain sampler (8 trials
+    per cell), a network trained there, then decorrelation runs of the
+    learned form (--learned-fraction of the passes learned, one move per
+    non-empty cell in each), of the plain move on the same schedule (one trial
+    per non-empty cell and pass) and of the plain move with 16 trials per
+    cell (the fastest plain form), all from the equilibrated state."""
+    r, O, box = pp.generate(n, rho, SEED + 7)
+    prm = dict(T=temp, disp_max=DISP, theta_max=THETA, kf_delta=DELTA, kf_cosmax=COSMAX)
+    with pp.System(r, O, box, seed=SEED + 7, trials_per_cell=8, **prm) as s:
+        for k in range(0, equil_steps, 5000):
+            s.do_ensemble(min(5000, equil_steps - k))
+        start = s.state()
+        e0 = -s.counters()["bonds"] / n
+    print(f"# {label}: N={n} rho={rho} T={temp} equilibrated {equil_steps} steps, E/N={e0:.4f}", file=sys.stderr,
           flush=True)
-    learned = timed(net)
-    plain = timed(None)
+    net, train = train_net(pp, start, prm, args, SEED)
+    res = {"workload": f"NVT N={n}, rho={rho}, T={temp}", "equilibration_steps": equil_steps, "train": train}
+    res["learned"] = decorrelation(pp, start, prm, seed=SEED + 11, trials_per_cell=1, sweeps=sweeps, net=net,
+                                   learned_fraction=args.learned_fraction, label=f"{label} learned")
+    res["plain_same_schedule"] = decorrelation(pp, start, prm, seed=SEED + 12, trials_per_cell=1, sweeps=sweeps,
+                                               label=f"{label} plain 1 trial/cell")
+    res["plain_16"] = decorrelation(pp, start, prm, seed=SEED + 13, trials_per_cell=16, sweeps=sweeps,
+                                    label=f"{label} plain 16 trials/cell")
     net.close()
-    learned["train"] = {"steps": args.train_steps, "batch": args.train_batch, "seconds": round(t_train, 2),
-                        "loss_first": round(l0, 4), "loss_last": round(l1, 4), "pos_noise": 0.03}
-    learned["learned_fraction"] = args.learned_fraction
-    return learned, plain
-
-
-def _jumps(s, sweeps):
-    """Mean squared displacement and mean 1 - cos(angle) per attempted move of
-    replica 0, from states one sweep apart."""
-    import numpy as np
-    tr = rot = 0.0
-    att = 0
-    for _ in range(sweeps):
-        r0, O0, box = s.state(0)
-        c0 = s.counters()["tries"][0]
-        s.do_ensemble(1)
-        r1, O1, _ = s.state(0)
-        d = r1 - r0
-        d -= box * np.rint(d / box)
-        tr += float((d * d).sum())
-        rot += float((1 - (np.einsum("nij,nij->n", O1, O0) - 1) / 2).sum())
-        att += s.counters()["tries"][0] - c0
-    return tr / att, rot / att
+    res["learned_over_plain"] = ratio(res["learned"], res["plain_16"])
+    res["learned_over_plain_same_schedule"] = ratio(res["learned"], res["plain_same_schedule"])
+    res["learned_fraction"] = args.learned_fraction
+    return res
 
 
 def bench_learned(args, pp):
-    """The learned proposal at the gel state point: N=65536, rho=0.3, T=0.09.
-    Plain equilibration (--warmup steps of 8 trials per cell), a network
-    trained on --train-steps snapshots of the plain sampler, then --steps
-    timed steps each of the learned form (--learned-fraction of the passes
-    learned, the rest plain; one trial per non-empty cell and pass) and of
-    the plain move on the same schedule, both from the same state.  value =
-    sweeps/s of the learned form; jumps per attempt and tau_int[E/N] for
-    both, with tau's error, the run's length in units of tau and the drift of
-    E/N across it."""
-    n, rho, temp = 65536, 0.3, 0.09
-    r, O, box = pp.generate(n, rho, SEED)
-    opts = dict(T=temp, disp_max=DISP, theta_max=THETA, kf_delta=DELTA, kf_cosmax=COSMAX)
-    with pp.System(r, O, box, seed=SEED, trials_per_cell=8, **opts) as s:
-        s.do_ensemble(args.warmup)
-        st = s.state(0)
-        print(f"# warmup {args.warmup} steps, E/N={-s.counters()['bonds'] / n:.4f}", file=sys.stderr, flush=True)
-        net = pp.ProposalNet(seed=SEED)
-        t0 = time.perf_counter()
-        l0, l1 = s.train(net, steps=args.train_steps, batch=args.train_batch, lr=1e-3, pos_noise=0.03,
-                         theta_max=THETA, sweeps_between=1)
-        t_train = time.perf_counter() - t0
-    print(f"# trained in {t_train:.1f} s (loss {l0:.3f} -> {l1:.3f})", file=sys.stderr, flush=True)
-    res = {}
-    for name in ("learned", "plain"):
-        with pp.System(*st, seed=SEED + 2, trials_per_cell=1, **opts) as s:
-            if name == "learned":
-                s.attach(net, learned_fraction=args.learned_fraction, theta_max=THETA)
-            s.do_ensemble(20)
-            jt, jr = _jumps(s, 10)
-            s.reset_counters()
-            s.set_timing(args.time_every)
-            s.set_trace(args.steps)
-            t0 = time.perf_counter()
-            s.do_ensemble(args.steps)
-            el = time.perf_counter() - t0
-            c = s.counters()
-            e = s.trace() / n
-            nl, ml = s.timing("learned")
-            b, ov, tot, anyo = s.energy()
-            assert not anyo and tot == c["bonds"], "tracked energy disagrees with a full recompute"
-        sw = c["tries"][0] / n / el
-        st_e = series_stats(e, n * args.steps / c["tries"][0])
-        res[name] = {"sweeps_per_s": round(sw, 2), "acceptance": round(c["accepted"][0] / c["tries"][0], 5),
-                     "jump_sq_per_attempt": round(jt, 8), "one_minus_cos_per_attempt": round(jr, 8), **st_e,
-                     "decorrelated_samples_per_s": round(sw / (2 * st_e["tau_int_sweeps"]), 4)}
-        if name == "learned":
-            res[name]["learned_pass_us"] = round(ml / max(nl, 1) * 1e3, 3)
-        print(f"# {name}: {res[name]}", file=sys.stderr, flush=True)
-    net.close()
+    """The learned proposal at the gel state point alone: N=65536, rho=0.3,
+    T=0.09 (learned_vs_plain with --c4-equil steps of equilibration and
+    --c4-sweeps sweeps per decorrelation run).  value = sweeps/s of the
+    learned form."""
+    res = learned_vs_plain(pp, args, 65536, 0.3, 0.09, args.c4_equil, args.c4_sweeps, "gel")
     out = {
         "metric": "particle sweeps/s + decorrelated samples/s (E/N), learned vs plain, 1 GPU",
-        "value": res["learned"]["sweeps_per_s"], "unit": "sweeps/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": None, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64 geometry, bf16/f32 network", "data": "synthetic (seeded random-insertion start)",
-        "config": {"workload": f"learned-proposal NVT, N={n}, rho={rho}, T={temp}, learned fraction "
-                               f"{args.learned_fraction}, one trial per non-empty cell and pass",
-                   "train": {"steps": args.train_steps, "batch": args.train_batch, "seconds": round(t_train, 2),
-                             "loss_first": round(l0, 4), "loss_last": round(l1, 4)}},
-        "decorrelated_samples_per_s": {"learned": res["learned"]["decorrelated_samples_per_s"],
-                                       "plain": res["plain"]["decorrelated_samples_per_s"]},
-        "learned": res["learned"], "plain": res["plain"],
+        "value": res["learned"]["sweeps_per_s"], "unit": "sweeps/s (per replica of an 8-replica batch)", "n_gpus": 1,
+        "steps": None, "warmup": None, "ms_per_step": None, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64 geometry, bf16/f32 network", "data": "synthetic (seeded random-insertion start)",
+        "config": {"workload": f"learned-proposal NVT, N=65536, rho=0.3, T=0.09, learned fraction "
+                               f"{args.learned_fraction}, one move per non-empty cell and learned pass"},
+        "learned_vs_plain": res,
     }
     print(json.dumps(out), flush=True)
 
@@ -297,12 +268,20 @@ def bench_npt(args, pp):
         bs.append(box)
     prm = [dict(T=temp, disp_max=DISP, theta_max=THETA, kf_delta=DELTA, kf_cosmax=COSMAX, ensemble=pp.NPT, P=P,
                 rescale_factor_max=0.01)] * R
-    s = pp.System.batch(rs, Os, bs, prm, seed=SEED, trials_per_cell=args.trials_per_cell)
+    s = pp.System.batch(rs, Os, bs, prm, seed=SEED, trials_per_cell=args.trials_per_cell,
+                        threads_per_cell=args.tpc, block_size=args.block)
     print(f"# {R} replicas of N={n} created", file=sys.stderr, flush=True)
     for k in range(0, args.warmup, 50):
         s.do_ensemble(min(50, args.warmup - k))
         print(f"# warmup {k + 50} steps, density {float((n / s.counters_all()['volume']).mean()):.4f}",
               file=sys.stderr, flush=True)
+    tuned = None
+    if not args.tpc and not args.block:
+        # the launch shape of the warm state (the trajectory does not depend on it)
+        tpc, block, us = s.autotune(5)
+        tuned = {"threads_per_cell": tpc, "block_size": block}
+        print(f"# autotuned: {tpc} lanes per cell, {block} threads per workgroup ({us:.1f} us/step)", file=sys.stderr,
+              flush=True)
     net = None
     if args.learned_fraction > 0:
         net = pp.ProposalNet(seed=SEED)
@@ -332,6 +311,7 @@ def bench_npt(args, pp):
         "data": "synthetic (seeded random-insertion starts)",
         "config": {"workload": f"{R} NPT replicas of N={n}, P={P}, T={temp}, start rho={rho}, rescale_factor_max 0.01",
                    "trials_per_cell": args.trials_per_cell, "learned_fraction": args.learned_fraction,
+                   "autotuned": tuned,
                    "density_mean": round(float(dens.mean()), 5), "E_per_N_mean": round(float((-c["bonds"] / n).mean()), 5),
                    "volume_acceptance": round(float(c["accepted"][:, 1].sum() / max(c["tries"][:, 1].sum(), 1)), 4)},
     }
@@ -382,36 +362,50 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=4000, help="timed steps (a step: rebuild + 8 colour passes)")
     ap.add_argument("--warmup", type=int, default=None,
-                    help="untimed steps first (equilibration; default 40000 for the single workload, else 6000)")
-    ap.add_argument("--tau-sweeps", type=int, default=24000,
-                    help="sweeps of the plain tau_int run when the learned leg is off (0: no tau_int run)")
+                    help="untimed steps just before the timed ones (default 100; batch and npt workloads: 6000)")
+    ap.add_argument("--equil", type=int, default=40000,
+                    help="equilibration steps of the N=32768 system before the warmup, whatever --warmup is")
+    ap.add_argument("--tau-sweeps", type=int, default=60000,
+                    help="sweeps of each of the 8 copies in the headline form's tau_int run (0: none)")
+    ap.add_argument("--stat-T", type=float, default=0.2,
+                    help="temperature of the stationary state point (rho=0.5) of the learned comparison")
+    ap.add_argument("--stat-n", type=int, default=4096,
+                    help="particles per copy there (tau_int of E/N is intensive; small copies reach many tau)")
+    ap.add_argument("--stat-equil", type=int, default=40000, help="equilibration steps there (8 trials per cell)")
+    ap.add_argument("--stat-sweeps", type=int, default=100000,
+                    help="sweeps of each copy in each decorrelation run there (0: skip the stationary point)")
+    ap.add_argument("--c4-equil", type=int, default=6000,
+                    help="equilibration steps of the gel (N=65536, rho=0.3, T=0.09; 8 trials per cell)")
+    ap.add_argument("--c4-sweeps", type=int, default=10000,
+                    help="sweeps of each copy in each decorrelation run of the gel (0: skip the gel)")
     ap.add_argument("--trials-per-cell", type=int, default=16,
                     help="trials per non-empty cell and pass (0: one per member, a step of exactly N attempts)")
     ap.add_argument("--per-member-steps", type=int, default=2000,
                     help="steps of the one-trial-per-member schedule timed after the main run (0: skip)")
     ap.add_argument("--tpc", type=int, default=0, help="threads per cell (0: default)")
+    ap.add_argument("--launch", type=int, default=0,
+                    help="launch form of the colour passes: 0/1 one launch per pass, 3 dependency-driven")
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--occupancy", type=float, default=0.0)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-timing", action="store_true", help="no HIP events in the timed region")
-    ap.add_argument("--time-every", type=int, default=16,
-                    help="HIP events bracket the kernels of every k-th sweep of the timed region")
-    ap.add_argument("--learned-steps", type=int, default=30000,
-                    help="timed steps of the learned-vs-plain comparison (0: skip it)")
+    ap.add_argument("--time-every", type=int, default=0,
+                    help="HIP events bracket the kernels of every k-th step of the timed region "
+                         "(0: every step of up to 256, else 256 evenly spaced steps)")
     ap.add_argument("--train-steps", type=int, default=10000, help="Adam steps (one sampler snapshot each)")
     ap.add_argument("--train-batch", type=int, default=4096)
     ap.add_argument("--learned-fraction", type=float, default=0.8)
     ap.add_argument("--workload", choices=("single", "batch", "learned", "npt"), default="single",
-                    help="single: the N=32768 system; batch: 256 independent replicas of uneven N, T and rho; "
-                         "learned: the learned proposal at N=65536, rho=0.3, T=0.09; npt: 16 NPT replicas of "
-                         "N=131072")
+                    help="single: the N=32768 system (and the decorrelation legs); batch: 256 independent replicas "
+                         "of uneven N, T and rho; learned: the learned proposal at N=65536, rho=0.3, T=0.09 "
+                         "alone; npt: 16 NPT replicas of N=131072")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only copy a MIB-sized device buffer 5 times (the reference point that scales "
                          "the FETCH_SIZE / WRITE_SIZE counters of a profile)")
     args = ap.parse_args()
     if args.warmup is None:
-        args.warmup = 40000 if args.workload == "single" else 6000
+        args.warmup = 100 if args.workload == "single" else 6000
     if args.calibrate_copy:
         import torch
         n = args.calibrate_copy * (1 << 20) // 4
@@ -438,17 +432,24 @@ def main():
         return bench_npt(args, pp)
 
     r, O, box = pp.generate(N, RHO, SEED)
-    opts = dict(T=T, disp_max=DISP, theta_max=THETA, kf_delta=DELTA, kf_cosmax=COSMAX, threads_per_cell=args.tpc,
-                block_size=args.block, cell_occupancy=args.occupancy)
-    sysm = pp.System(r, O, box, seed=SEED, trials_per_cell=args.trials_per_cell, **opts)
+    opts = dict(T=T, disp_max=DISP, theta_max=THETA, kf_delta=DELTA, kf_cosmax=COSMAX)
+    sysm = pp.System(r, O, box, seed=SEED, trials_per_cell=args.trials_per_cell, threads_per_cell=args.tpc,
+                     block_size=args.block, cell_occupancy=args.occupancy, launch=args.launch, **opts)
     print(f"# start: N={N} box={box[0]:.4f} grid={sysm.grid()}", file=sys.stderr, flush=True)
+    # equilibration (whatever --warmup says): the timed steps run at the
+    # state point, not on the relaxing start
+    for k in range(0, args.equil, 10000):
+        sysm.do_ensemble(min(10000, args.equil - k))
+        print(f"# equilibration {min(k + 10000, args.equil)} steps, E/N={-sysm.counters()['bonds'] / N:.4f}",
+              file=sys.stderr, flush=True)
     sysm.do_ensemble(args.warmup)
     print(f"# warmup {args.warmup} steps done, E/N={-sysm.counters()['bonds'] / N:.4f}", file=sys.stderr, flush=True)
 
     # timed region: K steps; HIP events bracket the kernels of every
-    # --time-every-th step (the per-kernel averages of the roofline line)
+    # time_every-th step (every step of a short run)
+    time_every = args.time_every or max(1, args.steps // 256)
     sysm.reset_counters()
-    sysm.set_timing(0 if args.no_timing else args.time_every)
+    sysm.set_timing(0 if args.no_timing else time_every)
     t0 = time.perf_counter()
     sysm.do_ensemble(args.steps)  # synchronous: returns when the device is done
     t1 = time.perf_counter()
@@ -463,21 +464,25 @@ def main():
     sweeps_per_s = attempts / N / elapsed  # a sweep is N attempted moves (MC.c:21)
     nside, side = sysm.grid()
     bpm = algorithmic_bytes_per_move(RHO, side[0])
-    moves_per_launch = attempts / args.steps / 8  # attempted moves one colour pass makes
+    # one launch makes one colour pass, or all 8 in the dependency-driven form
+    passes_per_launch = 8 if args.launch == 3 else 1
+    moves_per_launch = attempts / args.steps / 8 * passes_per_launch  # attempted moves of one launch
     bytes_per_launch = bpm * moves_per_launch
     avg_pass_s = ms_pass / max(n_pass, 1) / 1e3
     achieved = bytes_per_launch / avg_pass_s / 1e9 if avg_pass_s > 0 else 0.0
+    traffic = measured_traffic() if passes_per_launch == 1 else None  # measured for the per-pass kernel
+    chain = args.trials_per_cell if args.trials_per_cell > 0 else None
 
     # parity spot check of the final snapshot (energy recomputed vs tracked)
     b, ov, tot, anyo = sysm.energy()
     assert not anyo and tot == c["bonds"], "tracked energy disagrees with a full recompute"
-    r_eq, O_eq, box_eq = sysm.state()
+    start = sysm.state()
     sysm.close()
 
     per_member = None
     if args.per_member_steps > 0 and args.trials_per_cell != 0:
         # the same state advanced with one trial per member of each cell
-        with pp.System(r_eq, O_eq, box_eq, seed=SEED + 1, trials_per_cell=0, **opts) as s0:
+        with pp.System(*start, seed=SEED + 1, trials_per_cell=0, **opts) as s0:
             s0.do_ensemble(50)
             s0.reset_counters()
             ta = time.perf_counter()
@@ -487,24 +492,29 @@ def main():
         per_member = {"trials_per_cell": 0, "sweeps_per_s": round(c0["tries"][0] / N / (tb - ta), 2),
                       "steps": args.per_member_steps}
 
-    # decorrelated samples: every tau run starts from this state and runs as
-    # many sweeps as the one-trial-per-cell plain run (or --tau-sweeps)
-    learned = plain_same = None
-    tau_sweeps = args.tau_sweeps
-    if args.learned_steps > 0:
-        learned, plain_same = learned_leg(args, pp, r_eq, O_eq, box_eq, opts)
-        tau_sweeps = plain_same["run_sweeps"]
-    plain_tau = decorrelated = None
-    if tau_sweeps > 0:
-        print(f"# tau_int of the plain run over {tau_sweeps} sweeps ...", file=sys.stderr, flush=True)
-        plain_tau = tau_leg(pp, r_eq, O_eq, box_eq, opts, args.trials_per_cell, tau_sweeps, attempts / args.steps)
-        decorrelated = sweeps_per_s / (2 * plain_tau["tau_int_sweeps"])
+    # decorrelated samples of the headline form at its own state point, from
+    # 8 independent copies of the equilibrated state
+    plain_tau = None
+    if args.tau_sweeps > 0:
+        plain_tau = decorrelation(pp, start, opts, seed=SEED + 4, trials_per_cell=args.trials_per_cell,
+                                  sweeps=args.tau_sweeps, label="N=32768 rho=0.5 T=0.12 plain")
+        # the headline rate is the single system's timed one
+        if plain_tau["decorrelated_samples_per_s"] is not None:
+            plain_tau["decorrelated_samples_per_s_at_value"] = round(sweeps_per_s / (2 * plain_tau["tau_int_sweeps"]), 4)
+    stat_point = None
+    if args.stat_sweeps > 0:
+        stat_point = learned_vs_plain(pp, args, args.stat_n, RHO, args.stat_T, args.stat_equil, args.stat_sweeps,
+                                      "stationary")
+    gel = None
+    if args.c4_sweeps > 0:
+        gel = learned_vs_plain(pp, args, 65536, 0.3, 0.09, args.c4_equil, args.c4_sweeps, "gel")
 
     cpu = None
     if not args.no_cpu:
         print("# cpu baseline ...", file=sys.stderr, flush=True)
-        cpu = cpu_baseline(r_eq, O_eq, box_eq, args.cpu_seconds)
+        cpu = cpu_baseline(*start, args.cpu_seconds)
 
+    avg_launch_us = avg_pass_s * 1e6
     out = {
         "metric": "particle sweeps/s + decorrelated samples/s (E/N), learned vs plain, 1 GPU",
         "value": round(sweeps_per_s, 2),
@@ -524,27 +534,39 @@ def main():
             "grid": nside, "cell_side": round(side[0], 5),
             "threads_per_cell": args.tpc or 32, "trials_per_cell": args.trials_per_cell,
             "step": "rebuild on a randomly shifted grid + 8 colour passes",
+            "equilibration_steps": args.equil,
             "attempts": attempts, "attempts_per_step": round(attempts / args.steps, 1),
             "acceptance": round(c["accepted"][0] / max(attempts, 1), 5),
             "E_per_N": round(float(-c["bonds"] / N), 6),
         },
         "decorrelated_samples_per_s": {
-            "plain": round(decorrelated, 3) if decorrelated else None, "plain_run": plain_tau,
-            "learned": learned["decorrelated_samples_per_s"] if learned else None,
-            "learned_over_plain": round(learned["decorrelated_samples_per_s"] / decorrelated, 4)
-            if learned and decorrelated else None,
-            "learned_run": learned,
-            "plain_one_trial_per_cell": plain_same,
+            "plain": plain_tau["decorrelated_samples_per_s_at_value"]
+            if plain_tau and plain_tau["decorrelated_samples_per_s"] is not None else None,
+            "status": plain_tau["status"] if plain_tau else None,
+            "plain_run": plain_tau,
+            "stationary_point": stat_point,
+            "gel": gel,
         },
         "one_trial_per_member": per_member,
         "roofline": {
-            "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": measured_traffic(),
-            "kernel": "k_pass (colour pass)", "bytes_per_move": round(bpm, 1),
+            # each cell's team runs its trials as one dependent chain: the
+            # pass is bound by that chain's latency, not by bytes
+            "bound": "latency",
+            "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
+            "achieved_is": "algorithmic bytes per launch (attempted moves x bytes per move) / average launch",
+            "hbm_measured_gbs": round(traffic / avg_pass_s / 1e9, 2) if traffic and avg_pass_s > 0 else None,
+            "hbm_measured_frac": round(traffic / avg_pass_s / 1e9 / HBM_PEAK_GBS, 5) if traffic and avg_pass_s > 0 else None,
+            "kernel": "k_pass_dd (the 8 colour passes of a sweep)" if passes_per_launch == 8 else "k_pass (colour pass)",
+            "bytes_per_move": round(bpm, 1),
             "moves_per_launch": round(moves_per_launch, 1),
-            "algorithmic_bytes_per_launch": round(bytes_per_launch), "avg_launch_us": round(avg_pass_s * 1e6, 3),
+            "algorithmic_bytes_per_launch": round(bytes_per_launch), "avg_launch_us": round(avg_launch_us, 3),
+            "trials_per_chain": chain,
+            "us_per_trial_of_chain": round(avg_launch_us / passes_per_launch / chain, 3) if chain else None,
+            "cycles_per_trial_of_chain": round(avg_launch_us / passes_per_launch / chain * CLOCK_GHZ * 1e3)
+            if chain else None,
             "rebuild_us_per_sweep": round(ms_reb / max(n_reb, 1) * 1e3, 3),
-            "timed_sweeps": n_reb,
+            "timed_steps": n_reb, "pass_launches_timed": n_pass,
         },
         "cpu_baseline": cpu,
     }

@@ -28,6 +28,8 @@
  *   pp_download / pp_upload(_n)   the System particle arrays read by
  *                                 output_save (output.c:302-317) / written by
  *                                 the .rrr loader (system.c:213-245)
+ *   pp_snapshot_start / _finish   output_save's read of the particle arrays,
+ *                                 overlapped with the following sweeps
  *   pp_get_counters(_all)         syst->tries/accepted/energy/V/box read by
  *                                 output_print (output.c:119-196)
  *   pp_reset_counters             utils_reset_acceptance_counters (utils.c:276-282)
@@ -273,6 +275,18 @@ pp_status pp_bond_list(pp_ctx *ctx, int rep, int max_per, int32_t *count, int32_
  * seed and trials_per_cell resumes the run bit for bit. */
 pp_status pp_checkpoint_save(pp_ctx *ctx, const char *path);
 pp_status pp_checkpoint_load(pp_ctx *ctx, const char *path);
+
+/* Asynchronous snapshot of replica rep: the state output_save writes
+ * (output.c:302-317, main.c:98-125), taken without holding the sweeps back.
+ * pp_snapshot_start copies the replica's particles on the device into a
+ * staging area of the context and starts their copy to pinned host memory on
+ * a second stream, then returns; sweeps enqueued after it run while the copy
+ * proceeds.  pp_snapshot_finish waits for the copy and writes r (3n), O (9n)
+ * and box as pp_download does, in particle-index order, for the state at the
+ * time of the start.  One snapshot per context at a time: a second start
+ * before the finish is refused. */
+pp_status pp_snapshot_start(pp_ctx *ctx, int rep);
+pp_status pp_snapshot_finish(pp_ctx *ctx, double *r, double *O, double box[3]);
 
 /* Grid of replica rep: cells per side and cell side lengths. */
 pp_status pp_get_grid(pp_ctx *ctx, int rep, int n_side[3], double side[3]);

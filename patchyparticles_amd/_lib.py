@@ -23,7 +23,7 @@ EXPORTS = (
     "pp_net_attach", "pp_net_train", "pp_get_learn_counters", "pp_net_descriptor",
     "pp_checkpoint_save", "pp_checkpoint_load", "pp_bond_list", "pp_set_lx_move",
     "pp_energy_snapshot_noalloc", "pp_set_lx_acceptance", "pp_get_layout", "pp_upload_n",
-    "pp_colour_conflicts", "pp_colour_check",
+    "pp_colour_conflicts", "pp_colour_check", "pp_snapshot_start", "pp_snapshot_finish",
 )
 
 # sizes of the proposal network (PP_NET_* in include/pp.h)
@@ -160,6 +160,10 @@ def lib():
     L.pp_colour_conflicts.restype = st
     L.pp_colour_check.argtypes = [_vp, C.c_int, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]
     L.pp_colour_check.restype = st
+    L.pp_snapshot_start.argtypes = [_vp, C.c_int]
+    L.pp_snapshot_start.restype = st
+    L.pp_snapshot_finish.argtypes = [_vp, _vp, _vp, _vp]
+    L.pp_snapshot_finish.restype = st
     fp = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
     L.pp_net_create.argtypes = [C.c_int, C.c_uint64, C.POINTER(_vp)]
     L.pp_net_create.restype = st

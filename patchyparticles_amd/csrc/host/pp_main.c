@@ -9,7 +9,8 @@
  * cadence (main.c:78-125): at every Print_every step the outputs are written,
  * last.rrr is refreshed and, after step 0, the energy is recomputed from
  * scratch and checked (MC_check_energy, cells_check); at every Save_every
- * step (after 0) conf_<step>.rrr and last.rrr are written.  Between two such
+ * step (after 0) conf_<step>.rrr and last.rrr are written, from a snapshot
+ * copied to the host while the next block of sweeps runs.  Between two such
  * steps the sweeps run on the GPU in one call.  SIGTERM, SIGABRT, SIGINT and
  * SIGUSR2 end the run after the current block of sweeps, with the last
  * configuration saved (main.c:16-24).
@@ -81,6 +82,35 @@ static void save_conf(run *u, long long step, const char *path) {
 	const int n = current_n(u);
 	check(u, pp_download(u->ctx, 0, u->r, u->O, box), "pp_download");
 	if(pprrr_write(path, step, n, box, u->r, u->O) != 0) ppo_exit(&u->out, "File '%s' is not writable\n", path);
+}
+
+/* A save step's configuration in flight: pp_snapshot_start copies the state
+ * on the device and the next block of sweeps runs while it travels to the
+ * host; conf_<step>.rrr and last.rrr are written from it before anything else
+ * happens at the next output step (or at the end of the run). */
+typedef struct pending_save {
+	int on;
+	long long step;
+	int n;
+	char name[1100];
+} pending_save;
+
+static void save_start(run *u, pending_save *p, long long step, const char *name) {
+	p->n = current_n(u);
+	check(u, pp_snapshot_start(u->ctx, 0), "pp_snapshot_start");
+	p->on = 1;
+	p->step = step;
+	snprintf(p->name, sizeof p->name, "%s", name);
+}
+
+static void save_finish(run *u, pending_save *p) {
+	if(!p->on) return;
+	double box[3];
+	p->on = 0;
+	check(u, pp_snapshot_finish(u->ctx, u->r, u->O, box), "pp_snapshot_finish");
+	if(pprrr_write(p->name, p->step, p->n, box, u->r, u->O) != 0) ppo_exit(&u->out, "File '%s' is not writable\n", p->name);
+	if(pprrr_write(u->out.last, p->step, p->n, box, u->r, u->O) != 0)
+		ppo_exit(&u->out, "File '%s' is not writable\n", u->out.last);
 }
 
 /* the checkpoint: the state at the top of an output step, before its energy
@@ -294,8 +324,10 @@ int main(int argc, char *argv[]) {
 	steps += u.out.start_from;
 
 	long long step = u.out.start_from;
+	pending_save pend = {0, 0, 0, {0}};
 	while(step < steps && !g_stop) {
 		const int print = step % u.out.print_every == 0, save = step > 0 && step % u.out.save_every == 0;
+		save_finish(&u, &pend);
 		if(print) {
 			print_step(&u, step);
 			save_checkpoint(&u);
@@ -304,8 +336,7 @@ int main(int argc, char *argv[]) {
 		if(save) {
 			char name[1100];
 			snprintf(name, sizeof name, "%s/conf_%lld.rrr", u.out.folder, step);
-			save_conf(&u, step, name);
-			save_conf(&u, step, u.out.last);
+			save_start(&u, &pend, step, name);
 			if(!print) save_checkpoint(&u);
 		}
 		long long until = next_multiple(step, u.out.print_every);
@@ -316,6 +347,7 @@ int main(int argc, char *argv[]) {
 		check(&u, pp_sweeps(u.ctx, until - step), "pp_sweeps");
 		step = until;
 	}
+	save_finish(&u, &pend);
 
 	save_conf(&u, step, u.out.last);
 	save_checkpoint(&u);

@@ -94,6 +94,18 @@ struct pp_ctx {
 	double learned_fraction = 0.;
 	std::vector<void *> allocs;
 	int *h_vslow = nullptr;  // pinned: the trial-energy paths the box attempt in flight needs (TrialNeed bits)
+	// asynchronous snapshots (pp_snapshot_start / _finish): a device staging
+	// area and pinned host copies of one replica's slots, copied on their own
+	// stream
+	hipStream_t copy_stream = nullptr;
+	hipEvent_t snap_ready = nullptr, snap_done = nullptr;
+	double *snap_dev = nullptr, *snap_host = nullptr;  // NCOMP x snap_cap
+	int *snap_dev_pid = nullptr, *snap_host_pid = nullptr;
+	int snap_cap = 0;
+	bool snap_pending = false;
+	int snap_n = 0;
+	bool snap_gc = false;
+	double snap_box[3] = {0., 0., 0.};
 	int cus = 256;           // compute units of the device
 
 	~pp_ctx() {
@@ -101,8 +113,14 @@ struct pp_ctx {
 			(void) hipEventDestroy(p.second.first);
 			(void) hipEventDestroy(p.second.second);
 		}
+		if(copy_stream) (void) hipStreamSynchronize(copy_stream);
 		for(void *p : allocs) (void) hipFree(p);
 		if(h_vslow) (void) hipHostFree(h_vslow);
+		if(snap_host) (void) hipHostFree(snap_host);
+		if(snap_host_pid) (void) hipHostFree(snap_host_pid);
+		if(snap_ready) (void) hipEventDestroy(snap_ready);
+		if(snap_done) (void) hipEventDestroy(snap_done);
+		if(copy_stream) (void) hipStreamDestroy(copy_stream);
 		if(stream) (void) hipStreamDestroy(stream);
 	}
 };
@@ -843,6 +861,84 @@ pp_status pp_download(pp_ctx *c, int rep, double *r, double *O, double box[3]) {
 	}
 	if(box)
 		for(int a = 0; a < 3; a++) box[a] = rp.g.box[a];
+	return PP_OK;
+}
+
+pp_status pp_snapshot_start(pp_ctx *c, int rep) {
+	if(!c) return fail(PP_ERR_ARG, "ctx is NULL");
+	if(rep < 0 || rep >= c->R) return fail(PP_ERR_ARG, "replica %d out of range", rep);
+	if(c->snap_pending) return fail(PP_ERR_ARG, "a snapshot is still pending (pp_snapshot_finish)");
+	HIPCHECK(hipSetDevice(c->device));
+	if(!c->copy_stream) {
+		// the staging area holds the largest replica
+		int cap = 1;
+		for(int k = 0; k < c->R; k++) cap = std::max(cap, c->np[k]);
+		if(hipStreamCreateWithFlags(&c->copy_stream, hipStreamNonBlocking) != hipSuccess) {
+			c->copy_stream = nullptr;
+			return fail(PP_ERR_HIP, "snapshot stream creation failed");
+		}
+		HIPCHECK(hipEventCreateWithFlags(&c->snap_ready, hipEventDisableTiming));
+		HIPCHECK(hipEventCreateWithFlags(&c->snap_done, hipEventDisableTiming));
+		TRY(dev_alloc(c, &c->snap_dev, (size_t) NCOMP * cap));
+		TRY(dev_alloc(c, &c->snap_dev_pid, (size_t) cap));
+		if(hipHostMalloc((void **) &c->snap_host, sizeof(double) * NCOMP * cap, hipHostMallocDefault) != hipSuccess ||
+		   hipHostMalloc((void **) &c->snap_host_pid, sizeof(int) * cap, hipHostMallocDefault) != hipSuccess)
+			return fail(PP_ERR_NOMEM, "pinned snapshot buffers");
+		c->snap_cap = cap;
+	}
+	if(c->any_gc) {
+		// grand-canonical replicas: their live particles first (a rebuild)
+		rebuild_now(c);
+	}
+	TRY(pull_reps(c));
+	const Rep &rp = c->reps[rep];
+	const int n = rp.np, off = rp.p_off, S = c->d.stride, cap = c->snap_cap;
+	// on the sweeps' stream: the slots into the staging area, then the copy
+	// to the host on the snapshot stream once they are there
+	for(int a = 0; a < NCOMP && n; a++)
+		HIPCHECK(hipMemcpyAsync(c->snap_dev + (size_t) a * cap, c->d.f[rp.sel] + (size_t) a * S + off,
+		                        sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream));
+	if(n)
+		HIPCHECK(hipMemcpyAsync(c->snap_dev_pid, c->d.pid[rp.sel] + off, sizeof(int) * n, hipMemcpyDeviceToDevice,
+		                        c->stream));
+	HIPCHECK(hipEventRecord(c->snap_ready, c->stream));
+	HIPCHECK(hipStreamWaitEvent(c->copy_stream, c->snap_ready, 0));
+	if(n) {
+		HIPCHECK(hipMemcpyAsync(c->snap_host, c->snap_dev, sizeof(double) * NCOMP * cap, hipMemcpyDeviceToHost,
+		                        c->copy_stream));
+		HIPCHECK(hipMemcpyAsync(c->snap_host_pid, c->snap_dev_pid, sizeof(int) * n, hipMemcpyDeviceToHost,
+		                        c->copy_stream));
+	}
+	HIPCHECK(hipEventRecord(c->snap_done, c->copy_stream));
+	c->snap_n = n;
+	c->snap_gc = rp.gc != 0;
+	for(int a = 0; a < 3; a++) c->snap_box[a] = rp.g.box[a];
+	c->snap_pending = true;
+	return PP_OK;
+}
+
+pp_status pp_snapshot_finish(pp_ctx *c, double *r, double *O, double box[3]) {
+	if(!c) return fail(PP_ERR_ARG, "ctx is NULL");
+	if(!c->snap_pending) return fail(PP_ERR_ARG, "no snapshot pending (pp_snapshot_start)");
+	HIPCHECK(hipSetDevice(c->device));
+	c->snap_pending = false;
+	HIPCHECK(hipEventSynchronize(c->snap_done));
+	const int n = c->snap_n, cap = c->snap_cap;
+	std::vector<int> sorted(c->snap_host_pid, c->snap_host_pid + n);
+	std::sort(sorted.begin(), sorted.end());
+	for(int i = 0; i < n; i++)
+		if(sorted[i] < 0 || (i > 0 && sorted[i] == sorted[i - 1]) || (!c->snap_gc && sorted[i] != i))
+			return fail(PP_ERR_HIP, "corrupt particle index %d", sorted[i]);
+	for(int j = 0; j < n; j++) {
+		const int id = c->snap_host_pid[j];
+		const int i = c->snap_gc ? rank_of(sorted, id) : id;
+		if(r)
+			for(int a = 0; a < 3; a++) r[3 * i + a] = c->snap_host[(size_t) a * cap + j];
+		if(O)
+			for(int a = 0; a < 9; a++) O[9 * i + a] = c->snap_host[(size_t) (3 + a) * cap + j];
+	}
+	if(box)
+		for(int a = 0; a < 3; a++) box[a] = c->snap_box[a];
 	return PP_OK;
 }
 

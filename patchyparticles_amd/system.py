@@ -128,6 +128,21 @@ class System:
         check(lib().pp_download(self._h, rep, _ptr(r), _ptr(O), _ptr(box)))
         return r[:3 * n].reshape(n, 3), O[:9 * n].reshape(n, 3, 3), box
 
+    def snapshot_start(self, rep=0):
+        """Start an asynchronous copy of replica rep's state; later sweeps run
+        while it proceeds.  snapshot_finish() returns that state."""
+        self._snap_n = self._n(rep)
+        check(lib().pp_snapshot_start(self._h, int(rep)))
+
+    def snapshot_finish(self):
+        """(r, O, box) of the replica as it was at snapshot_start."""
+        n = getattr(self, "_snap_n", 0)
+        r = np.zeros(3 * max(n, 1))
+        O = np.zeros(9 * max(n, 1))
+        box = np.zeros(3)
+        check(lib().pp_snapshot_finish(self._h, _ptr(r), _ptr(O), _ptr(box)))
+        return r[:3 * n].reshape(n, 3), O[:9 * n].reshape(n, 3, 3), box
+
     def upload(self, rep, r, O, box=None):
         """Replace replica rep's particles (a grand-canonical replica takes
         any number up to its gc_n_max)."""
