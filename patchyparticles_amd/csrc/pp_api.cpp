@@ -93,7 +93,11 @@ struct pp_ctx {
 	pp_net *net = nullptr;
 	double learned_fraction = 0.;
 	std::vector<void *> allocs;
-	int *h_vslow = nullptr;  // pinned: the trial-energy paths the box attempt in flight needs (TrialNeed bits)
+	// pinned: (sequence number << 4) | the trial-energy paths the box attempt
+	// in flight needs (TrialNeed bits), written by the device (k_vol_publish)
+	int *h_vslow = nullptr;
+	int *d_vslow_host = nullptr;  // its device address
+	unsigned vol_seq = 0;
 	// asynchronous snapshots (pp_snapshot_start / _finish): a device staging
 	// area and pinned host copies of one replica's slots, copied on their own
 	// stream
@@ -490,10 +494,11 @@ pp_status create_impl(int R, const pp_params *prm, const pp_options *opts, const
 	   (st = dev_alloc(c, &d.done, d.C_total + 1)) || (st = dev_alloc(c, &d.ddq, 1)) ||
 	   (st = dev_alloc(c, &d.ccheck, 2)))
 		return bail(st);
-	if(hipHostMalloc((void **) &c->h_vslow, sizeof(int), hipHostMallocDefault) != hipSuccess || !c->h_vslow) {
-		c->h_vslow = nullptr;
+	if(hipHostMalloc((void **) &c->h_vslow, sizeof(int), hipHostMallocDefault) != hipSuccess) c->h_vslow = nullptr;
+	if(!c->h_vslow || hipHostGetDevicePointer((void **) &c->d_vslow_host, c->h_vslow, 0) != hipSuccess ||
+	   !c->d_vslow_host)
 		return bail(fail(PP_ERR_NOMEM, "pinned host flag allocation failed"));
-	}
+	*c->h_vslow = 0;
 	// diagnostic switches for the box trials' energy, which must give the
 	// same trajectories: PP_NPT_FULL_TRIAL=1 regroups the particles into the
 	// spare buffer for every trial; PP_NPT_PARTICLE_TRIAL=1 evaluates fast
@@ -608,6 +613,26 @@ void colour_check_now(pp_ctx *c) {
 	c->colour_checks += c->R;
 }
 
+// Wait for the device to publish attempt `seq`'s trial paths in the pinned
+// word (k_vol_publish): a spin on host memory, with the stream polled now and
+// then so that a failed launch is reported instead of waited for.
+pp_status wait_published(pp_ctx *c, unsigned seq, int *need) {
+	for(long long spin = 0;; spin++) {
+		const int v = __atomic_load_n(c->h_vslow, __ATOMIC_ACQUIRE);
+		if(((unsigned) v >> 4) == seq) {
+			*need = v & 15;
+			return PP_OK;
+		}
+		if((spin & 1023) == 1023) {
+			const hipError_t q = hipStreamQuery(c->stream);
+			if(q != hipSuccess && q != hipErrorNotReady) return fail(PP_ERR_HIP, "HIP error: %s", hipGetErrorString(q));
+			if(q == hipSuccess && ((unsigned) __atomic_load_n(c->h_vslow, __ATOMIC_ACQUIRE) >> 4) != seq)
+				return fail(PP_ERR_HIP, "the box attempt's trial paths were not published");
+		}
+		__builtin_ia32_pause();
+	}
+}
+
 // Enqueue sweep c->sweep on the context's stream (no synchronisation).
 pp_status enqueue_sweep(pp_ctx *c) {
 	const uint64_t sw = (uint64_t) c->sweep;
@@ -670,9 +695,10 @@ pp_status enqueue_sweep(pp_ctx *c) {
 			// served by its current lists (MC.c:481-485 rebuilds only when
 			// the cells become too small); then the trial energies, per
 			// particle and from the cell records, as the trials need them
-			HIPCHECK(hipMemcpyAsync(c->h_vslow, c->d.vslow, sizeof(int), hipMemcpyDeviceToHost, c->stream));
-			HIPCHECK(hipStreamSynchronize(c->stream));
-			const int need = *c->h_vslow;
+			const unsigned seq = (++c->vol_seq) & 0x7FFFFFFu;
+			launch_vol_publish(c->d, c->d_vslow_host, seq, c->stream);
+			int need = 0;
+			TRY(wait_published(c, seq, &need));
 			if(need & NEED_REBUILD) launch_rebuild(c->d, REB_TRIAL, sw, -1, c->stream);
 			if(need & NEED_PARTICLES) launch_energy(c->d, 1, 0, c->stream);
 			else launch_energy_rows_zero(c->d, c->stream);

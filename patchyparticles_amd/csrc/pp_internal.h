@@ -264,6 +264,9 @@ void launch_energy(const DevState &d, int trial, int per_particle, hipStream_t s
 void launch_bond_list(const DevState &d, int max_per, int *count, int *partners, hipStream_t s);
 // volume-move attempt k of sweep `sweep` (NPT replicas only)
 void launch_vol_propose(const DevState &d, uint64_t sweep, int k, hipStream_t s);
+// write (seq << 4) | the TrialNeed bits of the attempt just proposed into the
+// host's pinned word (host_word: its device address); seq < 2^27
+void launch_vol_publish(const DevState &d, int *host_word, unsigned seq, hipStream_t s);
 void launch_vol_decide(const DevState &d, uint64_t sweep, int k, hipStream_t s);
 // rescale the positions of replicas whose fast trial was accepted
 void launch_vol_apply(const DevState &d, hipStream_t s);

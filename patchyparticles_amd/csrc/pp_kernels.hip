@@ -1826,6 +1826,22 @@ __global__ void k_vol_propose(DevState d, uint64_t sweep, int k) {
 	choose_trial_path(d, rp, tb);
 }
 
+// The trial-energy paths the box attempt in flight needs (TrialNeed bits in
+// vslow, gathered by k_vol_propose), written straight into the host's pinned
+// word with the attempt's sequence number above them: the host waits for
+// exactly this attempt's value without a copy or a stream synchronisation.
+__global__ void k_vol_publish(DevState d, int *host_word, unsigned seq) {
+	if(threadIdx.x == 0) {
+		const int need = __hip_atomic_load(d.vslow, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+		__hip_atomic_store(host_word, (int) ((seq << 4) | (unsigned) need), __ATOMIC_RELAXED,
+		                   __HIP_MEMORY_SCOPE_SYSTEM);
+	}
+}
+
+void launch_vol_publish(const DevState &d, int *host_word, unsigned seq, hipStream_t s) {
+	hipLaunchKernelGGL(k_vol_publish, dim3(1), dim3(64), 0, s, d, host_word, seq);
+}
+
 void launch_vol_propose(const DevState &d, uint64_t sweep, int k, hipStream_t s) {
 	hipLaunchKernelGGL(k_vol_propose, dim3((d.R + 63) / 64), dim3(64), 0, s, d, sweep, k);
 }

@@ -16,7 +16,8 @@ executables.  Outputs are data only (inputs and expected outputs):
                     block statistics over the runs
   host_runs.json    the lines the reference program prints at step 0
                     (stdout, energy.dat, acceptance.dat, density.dat) on the
-                    reference generator's N=1000 rho=0.1 start, NVT and NPT
+                    reference generator's N=1000 rho=0.1 start, NVT, NPT and
+                    grand canonical
   bonds_start.rrr,  the reference program's Print_bonds file at step 0 of a
   bonds_0_ref.dat,  run from the snap_rho05_T01 liquid (N=500) and the keys
   bonds_keys.json   it ran with
@@ -270,7 +271,8 @@ def host_runs():
     subprocess.run([os.path.join(ROOT, "oracle", "_ref", "generator"), "1000", "0.1"], cwd=d, check=True,
                    capture_output=True)
     out = {}
-    for name, extra in (("nvt", {}), ("npt", {"Ensemble": 5, "P": 0.05, "rescale_factor_max": 0.05})):
+    for name, extra in (("nvt", {}), ("npt", {"Ensemble": 5, "P": 0.05, "rescale_factor_max": 0.05}),
+                        ("gc", {"Ensemble": 1, "Activity": 0.1, "GC_N_max": 3000})):
         keys = {"Dynamics": 0, "Ensemble": 0, "Seed": 17, "Temperature": 0.1, "Disp_max": 0.05, "Theta_max": 0.1,
                 "KF_delta": 0.12, "KF_cosmax": 0.92, "Initial_conditions_file": "generated.rrr", "Steps": 1,
                 "Print_every": 1, "Save_every": 1000, "Configuration_folder": "."}

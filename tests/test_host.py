@@ -179,7 +179,7 @@ def test_driver_usage_and_input_errors(host_built, tmp_path):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("ensemble", ["nvt", "npt"])
+@pytest.mark.parametrize("ensemble", ["nvt", "npt", "gc"])
 def test_driver_run_matches_reference_outputs(host_built, oracle, tmp_path, ensemble):
     """The reference's generated start, its input keys: the step-0 lines equal
     the reference program's byte for byte; later print steps, conf_<step>.rrr
@@ -202,9 +202,12 @@ def test_driver_run_matches_reference_outputs(host_built, oracle, tmp_path, ense
     assert len(out) == 4 and out[3].split()[0] == "300"
     for line in acc[1:]:
         assert 0.2 < float(line.split()[1]) < 1.0
-    if ensemble == "npt":
+    if ensemble != "nvt":
         dens = (tmp_path / "density.dat").read_text().splitlines(keepends=True)
         assert dens[0] == g["density0"] and len(dens) == 3
+    if ensemble == "gc":
+        # insertion and removal acceptances after the rototranslation column
+        assert all(len(line.split()) == 4 for line in acc)
     for step in (100, 200):
         s, box, r, O = oracle.rrr_load(str(tmp_path / "confs" / f"conf_{step}.rrr"))
         assert s == step
@@ -212,7 +215,7 @@ def test_driver_run_matches_reference_outputs(host_built, oracle, tmp_path, ense
         e_printed = float(energy[step // 100].split()[1])
         # the file keeps 6 decimals of the orientation rows: a pair at a
         # threshold may flip when it is read back
-        assert not ov.any() and abs(-tot / 1000 - e_printed) <= 3e-3
+        assert not ov.any() and abs(-tot / len(r) - e_printed) <= 3e-3
     s, box, r, O = oracle.rrr_load(str(tmp_path / "last.rrr"))
     assert s == 300
 
