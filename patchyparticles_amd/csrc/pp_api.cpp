@@ -494,6 +494,13 @@ pp_status create_impl(int R, const pp_params *prm, const pp_options *opts, const
 	   (st = dev_alloc(c, &d.done, d.C_total + 1)) || (st = dev_alloc(c, &d.ddq, 1)) ||
 	   (st = dev_alloc(c, &d.ccheck, 2)))
 		return bail(st);
+	if(c->any_npt || c->any_lx) {
+		// the close-pair list of the box trials: 8 pairs per slot (a
+		// hard-sphere particle has at most about 14 neighbours within 1.1
+		// bond ranges; a fuller list overflows to the cell records)
+		d.pl_cap = 8 * d.stride;
+		if((st = dev_alloc(c, &d.pl, (size_t) d.pl_cap)) || (st = dev_alloc(c, &d.pl_n, 2))) return bail(st);
+	}
 	if(hipHostMalloc((void **) &c->h_vslow, sizeof(int), hipHostMallocDefault) != hipSuccess) c->h_vslow = nullptr;
 	if(!c->h_vslow || hipHostGetDevicePointer((void **) &c->d_vslow_host, c->h_vslow, 0) != hipSuccess ||
 	   !c->d_vslow_host)
@@ -502,13 +509,15 @@ pp_status create_impl(int R, const pp_params *prm, const pp_options *opts, const
 	// diagnostic switches for the box trials' energy, which must give the
 	// same trajectories: PP_NPT_FULL_TRIAL=1 regroups the particles into the
 	// spare buffer for every trial; PP_NPT_PARTICLE_TRIAL=1 evaluates fast
-	// trials per particle instead of from the cell records
+	// trials per particle instead of from the cell records;
+	// PP_NPT_RECORD_TRIAL=1 takes them from the records, never from the
+	// close-pair list
 	{
 		auto on = [](const char *name) {
 			const char *e = getenv(name);
 			return e && *e && strcmp(e, "0") != 0;
 		};
-		d.npt_trial_form = on("PP_NPT_FULL_TRIAL") ? 2 : (on("PP_NPT_PARTICLE_TRIAL") ? 1 : 0);
+		d.npt_trial_form = on("PP_NPT_FULL_TRIAL") ? 2 : (on("PP_NPT_PARTICLE_TRIAL") ? 1 : (on("PP_NPT_RECORD_TRIAL") ? 3 : 0));
 	}
 	if(hipDeviceGetAttribute(&c->cus, hipDeviceAttributeMultiprocessorCount, o.device) != hipSuccess || c->cus <= 0)
 		c->cus = 256;
@@ -553,6 +562,7 @@ pp_status create_impl(int R, const pp_params *prm, const pp_options *opts, const
 	launch_fill_i32((int *) d.done, 0, d.C_total + 1, c->stream);
 	launch_fill_i32(d.ddq, 0, 1, c->stream);
 	launch_fill_u64(d.ccheck, 0ull, 2, c->stream);
+	if(d.pl_n) launch_fill_i32(d.pl_n, 0, 2, c->stream);
 	launch_fill_u64(d.ctr, 0ull, (long long) R * NCTR * NSLOT, c->stream);
 
 	// initial energy (main.c:54-61): any overlap is an error
@@ -687,6 +697,8 @@ pp_status enqueue_sweep(pp_ctx *c) {
 		// non-empty cell a step makes about s x (non-empty cells) <= s N
 		// attempts, and s volume attempts
 		const int nvol = c->d.trials_per_cell > 0 ? c->d.trials_per_cell : 1;
+		// the close pairs of this sweep's state, for the trials it serves
+		if(c->d.pl) launch_pairs_build(c->d, c->cus, c->stream);
 		for(int k = 0; k < nvol; k++) {
 			hipEvent_t v{};
 			tic(c, TK_VOLUME, &v);
@@ -703,6 +715,7 @@ pp_status enqueue_sweep(pp_ctx *c) {
 			if(need & NEED_PARTICLES) launch_energy(c->d, 1, 0, c->stream);
 			else launch_energy_rows_zero(c->d, c->stream);
 			if(need & NEED_RECORDS) launch_energy_rec(c->d, c->cus, c->stream);
+			if(need & NEED_PAIRS) launch_energy_pairs(c->d, c->cus, c->stream);
 			launch_vol_decide(c->d, sw, k, c->stream);
 			launch_vol_apply(c->d, c->stream);
 			toc(c, TK_VOLUME, v);
@@ -1359,6 +1372,13 @@ pp_status pp_set_lx_move(pp_ctx *c, int rep, double lx_change_max, double lyz_mi
 	}
 	c->any_lx = false;
 	for(const Rep &rp : c->reps) c->any_lx = c->any_lx || rp.lx_move;
+	if(c->any_lx && !c->d.pl) {
+		// the close-pair list of the box trials (as created for NPT contexts)
+		c->d.pl_cap = 8 * c->d.stride;
+		TRY(dev_alloc(c, &c->d.pl, (size_t) c->d.pl_cap));
+		TRY(dev_alloc(c, &c->d.pl_n, 2));
+		launch_fill_i32(c->d.pl_n, 0, 2, c->stream);
+	}
 	return push_reps(c);
 }
 

@@ -157,6 +157,14 @@ struct Rep {
 	// energy from the records, pair by pair (trial_rec)
 	int rec_ok;
 	int trial_rec;
+	// pl_ok: the sweep's list of close pairs (built from the records at the
+	// start of the box trials) holds this replica's pairs; pl_scale: the
+	// product of the axis scalings accepted since; a fast trial whose
+	// smallest cumulative scaling keeps every pair beyond the list radius
+	// outside the bond range takes its energy from that list (trial_pl)
+	int pl_ok;
+	int trial_pl;
+	double pl_scale[3];
 	int vol_apply;   // an accepted fast trial: positions still to be rescaled in place
 	int lx_ref_rule;  // box-shape moves accepted with exp(+dE/T) as MC.c:562 (not Metropolis)
 	double lx_change_max, lyz_min, lyz_max;
@@ -220,14 +228,18 @@ struct DevState {
 	uint64_t seed;
 	int *vflag;        // [R] a trial energy found an overlap: the replica's other teams stop
 	int *vslow;        // [1] some replica's trial needs a trial rebuild
-	// how box trials find their energy: 0 the fastest path available (cell
-	// records, else the current lists per particle, else a trial rebuild);
-	// 1 never the records; 2 always a trial rebuild (diagnostic forms that
-	// must give the same trajectories)
+	// how box trials find their energy: 0 the fastest path available (the
+	// close-pair list, else the cell records, else the current lists per
+	// particle, else a trial rebuild); 3 never the pair list; 1 neither the
+	// pair list nor the records; 2 always a trial rebuild (diagnostic forms
+	// that must give the same trajectories)
 	int npt_trial_form;
 	unsigned *done;    // [C_total] dependency-driven sweeps: mark of the sweep whose pass the cell last finished
 	int *ddq;          // [1] dependency-driven sweeps: tickets taken (reset by the rebuild)
 	unsigned long long *ccheck;  // [1] checkerboard conflicts found by the live checks
+	int2 *pl;          // [pl_cap] the close pairs (slot j, slot q) of the box-trial replicas, this sweep
+	int *pl_n;         // [2] pairs listed, and 1 if the list overflowed
+	int pl_cap;
 };
 
 // launchers (pp_kernels.hip)
@@ -283,7 +295,14 @@ void launch_energy_rows_zero(const DevState &d, hipStream_t s);
 // the trial-energy paths a box attempt needs, as the host reads them from
 // vslow after k_vol_propose: a trial rebuild, the per-particle trial energy,
 // the per-record trial energy
-enum TrialNeed : int { NEED_REBUILD = 1, NEED_PARTICLES = 2, NEED_RECORDS = 4 };
+enum TrialNeed : int { NEED_REBUILD = 1, NEED_PARTICLES = 2, NEED_RECORDS = 4, NEED_PAIRS = 8 };
+// the close-pair list keeps the pairs of adjacent cells closer than
+// PAIR_LIST_FACTOR bond ranges at its build
+constexpr double PAIR_LIST_FACTOR = 1.1;
+// build the sweep's close-pair list (and reset the replicas' pair state)
+void launch_pairs_build(const DevState &d, int cus, hipStream_t s);
+// the energy of the box trials that take it from the close-pair list
+void launch_energy_pairs(const DevState &d, int cus, hipStream_t s);
 // grand-canonical insertions and removals of the sweep (GC replicas only)
 void launch_gc_phase(const DevState &d, uint64_t sweep, hipStream_t s);
 void launch_gram_schmidt(const DevState &d, hipStream_t s);

@@ -1322,6 +1322,16 @@ size_t pass_lds_bytes(int T, int block) {
 // for every team) or several.
 template <int T, bool KEEP>
 static void pass_TK(const DevState &d, uint64_t sweep, int pass, int block, hipStream_t s) {
+	if constexpr(T == 16) {
+		// dilute systems (fewer than 16 candidates in a cell's 27 cells on
+		// average): one register neighbour per lane, the rare rest from memory
+		const double cand = 27. * (double) d.Ntot / (double) std::max(d.C_total, 1);
+		if(cand < 16.) {
+			if(d.R == 1) pass_T<16, 1, KEEP, true>(d, sweep, pass, block, s);
+			else pass_T<16, 1, KEEP, false>(d, sweep, pass, block, s);
+			return;
+		}
+	}
 	constexpr int CPL = T < 32 ? 32 / T : 1;  // register neighbours per lane
 	if(d.R == 1) pass_T<T, CPL, KEEP, true>(d, sweep, pass, block, s);
 	else pass_T<T, CPL, KEEP, false>(d, sweep, pass, block, s);
@@ -1440,7 +1450,7 @@ __global__ __launch_bounds__(256) void k_energy(DevState d, int trial, int per_p
 	if(j - rp.p_off >= rp.np) return;  // a free slot (the live particles come first after a rebuild)
 	const bool tr = trial && rp.trial_active;
 	if(trial && !tr) return;
-	if(tr && rp.trial_rec) return;  // k_energy_rec takes this trial
+	if(tr && (rp.trial_rec || rp.trial_pl)) return;  // k_energy_rec / k_energy_pairs take this trial
 	if(tr && __hip_atomic_load(&d.vflag[r], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
 	const bool fast = tr && rp.trial_fast;
 	const Grid &G = tr ? (fast ? rp.lg : rp.tg) : rp.g;  // the grid of the lists' membership
@@ -1678,6 +1688,178 @@ void launch_energy_rec(const DevState &d, int cus, hipStream_t s) {
 }
 
 // ------------------------------------------------------------------------
+// Close pairs of the box trials.  At the start of a sweep's box trials the
+// records give every pair of particles in adjacent cells once, as
+// k_energy_rec enumerates them; of each replica that makes box trials, the
+// pairs closer than PAIR_LIST_FACTOR bond ranges (minimum image in the
+// current box) are listed.  A trial scales every separation by the axis
+// factors accepted since the list was built (pl_scale) times its own.  While
+// the smallest of those products s keeps s x PAIR_LIST_FACTOR above 1 (with a
+// margin for the roundings of the list's test), an unlisted pair of adjacent
+// cells stays beyond the bond range, and a fast trial already holds every
+// pair within the bond range in adjacent cells: the trial's energy is the sum
+// over the listed pairs, pair by pair as k_energy_rec takes it.
+
+// The pair of work item e of a record (k_energy_rec's enumeration): the
+// cell's member j with candidate k of the 13 directions m > 13, or two
+// members; false for the half of the member pairs that is skipped.
+__device__ __forceinline__ bool rec_pair(const int *rec, int s0, int cnt, int m0, int m2, int e, int &j, int &q) {
+	if(e < cnt * m2) {
+		j = s0 + e / m2;
+		const int k = e % m2 + m0;
+		int nb = 13;
+		while(nb < 25 && rec[32 + nb + 1] <= k) nb++;
+		q = rec[nb] + (k - rec[32 + nb]);
+		return true;
+	}
+	const int a = (e - cnt * m2) / cnt, b2 = (e - cnt * m2) % cnt;
+	j = s0 + a;
+	q = s0 + b2;
+	return a < b2;
+}
+
+// Which replicas list their pairs this sweep (box trials on current lists
+// whose records were written by this sweep's rebuild), and an empty list.
+__global__ void k_pairs_reset(DevState d) {
+	const int r = blockIdx.x * blockDim.x + threadIdx.x;
+	if(r == 0) {
+		d.pl_n[0] = 0;
+		d.pl_n[1] = 0;
+	}
+	if(r >= d.R) return;
+	Rep &rp = d.reps[r];
+	rp.pl_ok = ((rp.ensemble == 5 || rp.lx_move) && rp.rec_ok && rp.lists_ok) ? 1 : 0;
+	rp.pl_scale[0] = rp.pl_scale[1] = rp.pl_scale[2] = 1.;
+}
+
+// One lane per listed cell, the wavefront's lanes in step: each counts its
+// close pairs, one atomic per wavefront reserves their places (in no fixed
+// order: the trial energies are integer sums), then each writes them.  A
+// list that would pass pl_cap is marked as overflowed; the trials then take
+// their energy from the records.
+__global__ __launch_bounds__(256) void k_pairs_build(DevState d) {
+	const int lane = threadIdx.x & 63;
+	const int wave = (xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + (int) threadIdx.x) >> 6;
+	const int waves = gridDim.x * (blockDim.x >> 6);
+	int lens[8];
+	int total = 0;
+#pragma unroll
+	for(int p = 0; p < 8; p++) {
+		lens[p] = d.pfx[p * (d.R + 1) + d.R];
+		total += lens[p];
+	}
+	for(int w0 = wave * 64; w0 < total; w0 += waves * 64) {
+		const int i0 = w0 + lane;
+		const int *rec = d.rec;
+		bool on = false;
+		int s0 = 0, cnt = 0, m0 = 0, m2 = 0;
+		const double *F = d.f[0];
+		double L[3] = {1., 1., 1.}, iL[3] = {1., 1., 1.}, rl2 = 0.;
+		if(i0 < total) {
+			int p = 0, gi = i0;
+			while(gi >= lens[p]) gi -= lens[p++];
+			rec = d.rec + ((size_t) p * d.total_act + gi) * REC_W;
+			const Rep &rp = d.reps[rec[62]];
+			on = rp.pl_ok != 0;
+			if(on) {
+				s0 = rec[59];
+				cnt = rec[60];
+				m0 = rec[32 + 13];
+				m2 = rec[58] - m0;
+				F = d.f[rec[63]];
+				for(int a = 0; a < 3; a++) {
+					L[a] = rp.g.box[a];
+					iL[a] = rp.g.inv_box[a];
+				}
+				rl2 = rp.rc2 * (PAIR_LIST_FACTOR * PAIR_LIST_FACTOR);
+			}
+		}
+		const int S = d.stride;
+		auto close = [&](int j, int q) {
+			const double dx = kf_min_image(F[q] - F[j], L[0], iL[0]);
+			const double dy = kf_min_image(F[S + q] - F[S + j], L[1], iL[1]);
+			const double dz = kf_min_image(F[2 * S + q] - F[2 * S + j], L[2], iL[2]);
+			return dx * dx + dy * dy + dz * dz < rl2;
+		};
+		const int work = on ? cnt * m2 + cnt * cnt : 0;
+		int n = 0;
+		for(int e = 0; e < work; e++) {
+			int j, q;
+			if(rec_pair(rec, s0, cnt, m0, m2, e, j, q) && close(j, q)) n++;
+		}
+		const int incl = wave_incl_scan(n);
+		const int tot = __builtin_amdgcn_readlane(incl, 63);
+		int base = 0;
+		if(tot > 0) {
+			if(lane == 0) base = atomicAdd(d.pl_n, tot);
+			base = __builtin_amdgcn_readfirstlane(base);
+		}
+		if(base + tot > d.pl_cap) {
+			if(lane == 0) d.pl_n[1] = 1;
+			continue;
+		}
+		int pos = base + incl - n;
+		for(int e = 0; e < work; e++) {
+			int j, q;
+			if(rec_pair(rec, s0, cnt, m0, m2, e, j, q) && close(j, q)) d.pl[pos++] = make_int2(j, q);
+		}
+	}
+}
+
+void launch_pairs_build(const DevState &d, int cus, hipStream_t s) {
+	hipLaunchKernelGGL(k_pairs_reset, dim3((unsigned) ((d.R + 255) / 256)), dim3(256), 0, s, d);
+	const int blocks = std::max(8, cus * 8) & ~7;
+	hipLaunchKernelGGL(k_pairs_build, dim3((unsigned) blocks), dim3(256), 0, s, d);
+}
+
+// The trial energies from the close-pair list: one lane per pair, the
+// separation and the bond decision as k_energy_rec forms them; a bonded pair
+// adds 2 to the bond rows, an overlapping pair 1 to the overlap row.
+__global__ __launch_bounds__(256) void k_energy_pairs(DevState d) {
+	const int n = min(d.pl_n[0], d.pl_cap);
+	const int stride = gridDim.x * blockDim.x;
+	for(int k = xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x; k < n; k += stride) {
+		const int2 pq = d.pl[k];
+		const int j = pq.x, q = pq.y;
+		const int r = d.prep[j];
+		const Rep &rp = d.reps[r];
+		if(!rp.trial_active || !rp.trial_pl) continue;
+		// (the early exit may read a stale flag: it only saves work)
+		if(__hip_atomic_load(&d.vflag[r], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) continue;
+		const double *F = d.f[rp.sel];
+		const int S = d.stride;
+		const double t0 = rp.trial_s[0], t1 = rp.trial_s[1], t2 = rp.trial_s[2];
+		const double px = F[j] * t0, py = F[S + j] * t1, pz = F[2 * S + j] * t2;
+		const double dx = kf_min_image(F[q] * t0 - px, rp.tg.box[0], rp.tg.inv_box[0]);
+		const double dy = kf_min_image(F[S + q] * t1 - py, rp.tg.box[1], rp.tg.inv_box[1]);
+		const double dz = kf_min_image(F[2 * S + q] * t2 - pz, rp.tg.box[2], rp.tg.inv_box[2]);
+		const double d2 = dx * dx + dy * dy + dz * dz;
+		const int slot = k & (NSLOT - 1);
+		if(d2 < 1.) {
+			__hip_atomic_store(&d.vflag[r], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+			atomicAdd(ctr_at(d.ctr, r, C_OVL, slot), 1ull);
+		}
+		else if(d2 < rp.rc2) {
+			const double c2 = rp.cosmax >= KF_FILTER_MIN_COS ? rp.cosmax * rp.cosmax : -1.;
+			double O[9], P[4][3];
+#pragma unroll
+			for(int c = 0; c < 9; c++) O[c] = F[(3 + c) * S + j];
+			kf_patches(O, P);
+			if(pair_bonded(dx, dy, dz, d2, &P[0][0], c2, rp.cosmax, [&](double *Oq) {
+#pragma unroll
+				   for(int c = 0; c < 9; c++) Oq[c] = F[(3 + c) * S + q];
+			   }))
+				atomicAdd(ctr_at(d.ctr, r, C_EBONDS, slot), 2ull);
+		}
+	}
+}
+
+void launch_energy_pairs(const DevState &d, int cus, hipStream_t s) {
+	const int blocks = std::max(8, cus * 8) & ~7;
+	hipLaunchKernelGGL(k_energy_pairs, dim3((unsigned) blocks), dim3(256), 0, s, d);
+}
+
+// ------------------------------------------------------------------------
 // Checkerboard invariant (a debug check; the reference is serial).  The
 // particles of one colour's cells move concurrently, so no particle may lie
 // within the bond range of a particle in another cell of the same colour.
@@ -1749,13 +1931,20 @@ void launch_colour_conflicts(const DevState &d, int rep, int slots, const int *n
 // on its own grid into the spare buffer; the host launches that rebuild only
 // when some replica needs it.
 __device__ __forceinline__ void choose_trial_path(const DevState &d, Rep &rp, const double tb[3]) {
-	bool fast = d.npt_trial_form < 2 && rp.lists_ok;
+	bool fast = d.npt_trial_form != 2 && rp.lists_ok;
 	for(int a = 0; a < 3; a++) fast = fast && tb[a] / (double) rp.lg.n[a] >= rp.side_min;
 	rp.trial_fast = fast ? 1 : 0;
 	// a fast trial takes its energy from the cell records while they describe
 	// the lists (the sweep's shifting rebuild wrote them)
-	rp.trial_rec = (fast && rp.rec_ok && d.npt_trial_form == 0) ? 1 : 0;
-	atomicOr(d.vslow, fast ? (rp.trial_rec ? (int) NEED_RECORDS : (int) NEED_PARTICLES)
+	rp.trial_rec = (fast && rp.rec_ok && (d.npt_trial_form == 0 || d.npt_trial_form == 3)) ? 1 : 0;
+	// the close-pair list while the cumulative scaling keeps unlisted pairs
+	// beyond the bond range (1e-9: the roundings of the list's test)
+	double smin = 1e300;
+	for(int a = 0; a < 3; a++) smin = fmin(smin, rp.pl_scale[a] * rp.trial_s[a]);
+	rp.trial_pl = (rp.trial_rec && d.npt_trial_form == 0 && rp.pl_ok && d.pl_n && d.pl_n[1] == 0 &&
+	               smin * PAIR_LIST_FACTOR >= 1. + 1e-9) ? 1 : 0;
+	if(rp.trial_pl) rp.trial_rec = 0;
+	atomicOr(d.vslow, fast ? (rp.trial_pl ? (int) NEED_PAIRS : (rp.trial_rec ? (int) NEED_RECORDS : (int) NEED_PARTICLES))
 	                       : (int) (NEED_REBUILD | NEED_PARTICLES));
 }
 
@@ -1766,6 +1955,7 @@ __global__ void k_vol_propose(DevState d, uint64_t sweep, int k) {
 	rp.trial_active = 0;
 	rp.trial_fast = 0;
 	rp.trial_rec = 0;
+	rp.trial_pl = 0;
 	rp.vol_apply = 0;
 	d.vflag[r] = 0;
 	if(rp.ensemble != 5 && !rp.lx_move) return;
@@ -1873,6 +2063,7 @@ __global__ void k_vol_decide(DevState d, uint64_t sweep, int k) {
 			// the positions are rescaled in place (k_vol_apply); the lists keep
 			// their members, their grid is rescaled with the positions
 			rp.vol_apply = 1;
+			for(int a = 0; a < 3; a++) rp.pl_scale[a] *= rp.trial_s[a];
 			set_grid_box(rp.lg, rp.tg.box);
 			for(int a = 0; a < 3; a++) rp.lg.shift[a] *= rp.trial_s[a];
 		}
@@ -1883,6 +2074,7 @@ __global__ void k_vol_decide(DevState d, uint64_t sweep, int k) {
 			rp.lg = rp.tg;
 			rp.lists_ok = 1;
 			rp.rec_ok = 0;
+			rp.pl_ok = 0;
 			rp.np_hw = rp.np;
 		}
 		rp.g = rp.tg;

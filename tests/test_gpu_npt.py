@@ -1,9 +1,10 @@
 """Box trials evaluated on the current cell lists (no trial rebuild while
 the lists' cells, rescaled with the trial, stay at least 1 + delta wide),
-with their energy taken pair by pair from the sweep's cell records (the
-default) or per particle (PP_NPT_PARTICLE_TRIAL=1), against the form that
-regroups every trial into the spare buffer (PP_NPT_FULL_TRIAL=1): the same
-trajectories bit for bit -- positions,
+with their energy taken from the sweep's close-pair list (the default), pair
+by pair from the cell records (PP_NPT_RECORD_TRIAL=1) or per particle
+(PP_NPT_PARTICLE_TRIAL=1), against the form that regroups every trial into
+the spare buffer (PP_NPT_FULL_TRIAL=1): the same trajectories bit for bit --
+positions,
 orientations, boxes, volumes and move counters -- in NPT (MC.c:450-513) and
 with box-shape moves (MC.c:515-585), for one system and for a batch that
 mixes NPT and NVT replicas, whose trials alternate between the two forms as
@@ -24,6 +25,7 @@ def pp():
 def _run(pp, monkeypatch, form, make, steps, lx=None):
     monkeypatch.setenv("PP_NPT_FULL_TRIAL", "1" if form == "full" else "0")
     monkeypatch.setenv("PP_NPT_PARTICLE_TRIAL", "1" if form == "particles" else "0")
+    monkeypatch.setenv("PP_NPT_RECORD_TRIAL", "1" if form == "records" else "0")
     with make() as s:
         if lx:
             s.set_lx_move(*lx)
@@ -44,7 +46,7 @@ def _same_runs(a, b):
 NPT = dict(ensemble=5)
 
 
-@pytest.mark.parametrize("case", ["npt_dense", "npt_dilute", "mixed_batch", "box_shape"])
+@pytest.mark.parametrize("case", ["npt_dense", "npt_dilute", "mixed_batch", "box_shape", "coincident_paths"])
 def test_box_trials_on_current_lists_keep_the_trajectory(pp, oracle, monkeypatch, case):
     if case == "npt_dense":
         r, O, box = oracle.generate(500, 0.3, seed=21)
@@ -63,13 +65,24 @@ def test_box_trials_on_current_lists_keep_the_trajectory(pp, oracle, monkeypatch
         make = lambda: pp.System.batch([x[0] for x in starts], [x[1] for x in starts], [x[2] for x in starts], prm,
                                        seed=5, trials_per_cell=2)
         steps, lx = 300, None
+    elif case == "coincident_paths":
+        # in every attempt one replica's large steps need trial rebuilds (and
+        # the per-particle energy) while the others' small ones take the
+        # close-pair list or the records: each trial counted by one path only
+        starts = [oracle.generate(n, rho, seed=50 + k) for k, (n, rho) in enumerate([(500, 0.4), (500, 0.4), (700, 0.3)])]
+        prm = [dict(T=0.2, P=0.3, rescale_factor_max=0.6, **NPT), dict(T=0.2, P=0.3, rescale_factor_max=0.01, **NPT),
+               dict(T=0.15, P=0.2, rescale_factor_max=0.02, **NPT)]
+        make = lambda: pp.System.batch([x[0] for x in starts], [x[1] for x in starts], [x[2] for x in starts], prm,
+                                       seed=8, trials_per_cell=4)
+        steps, lx = 250, None
     else:
         r, O, box = oracle.generate(400, 0.1, seed=23)
         make = lambda: pp.System(r, O, box, T=0.15, seed=6)
         steps, lx = (400, (2.0, 0.85 * box[1], 1.15 * box[1]))
     full = _run(pp, monkeypatch, "full", make, steps, lx)
-    fast = _run(pp, monkeypatch, "records", make, steps, lx)
+    fast = _run(pp, monkeypatch, "pairs", make, steps, lx)
     _same_runs(full, fast)
+    _same_runs(full, _run(pp, monkeypatch, "records", make, steps, lx))
     _same_runs(full, _run(pp, monkeypatch, "particles", make, steps, lx))
     c = fast[1]
     assert c["accepted"][:, 1].sum() > 0 and c["accepted"][:, 1].sum() < c["tries"][:, 1].sum()
