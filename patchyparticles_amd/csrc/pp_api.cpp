@@ -13,6 +13,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/pp.h"
@@ -628,6 +629,10 @@ void colour_check_now(pp_ctx *c) {
 // then so that a failed launch is reported instead of waited for.
 pp_status wait_published(pp_ctx *c, unsigned seq, int *need) {
 	for(long long spin = 0;; spin++) {
+		// (after the first few thousand polls, about 0.1 ms, the thread
+		// yields between polls: a long pass ahead of the attempt need not
+		// hold a core)
+		if(spin > 4096) std::this_thread::yield();
 		const int v = __atomic_load_n(c->h_vslow, __ATOMIC_ACQUIRE);
 		if(((unsigned) v >> 4) == seq) {
 			*need = v & 15;
@@ -908,21 +913,27 @@ pp_status pp_snapshot_start(pp_ctx *c, int rep) {
 	if(rep < 0 || rep >= c->R) return fail(PP_ERR_ARG, "replica %d out of range", rep);
 	if(c->snap_pending) return fail(PP_ERR_ARG, "a snapshot is still pending (pp_snapshot_finish)");
 	HIPCHECK(hipSetDevice(c->device));
-	if(!c->copy_stream) {
-		// the staging area holds the largest replica
+	if(!c->snap_cap) {
+		// the staging area holds the largest replica; each piece is kept as it
+		// is made (the destructor frees it), and snap_cap is set only once
+		// all exist, so a failed first call is retried whole
 		int cap = 1;
 		for(int k = 0; k < c->R; k++) cap = std::max(cap, c->np[k]);
-		if(hipStreamCreateWithFlags(&c->copy_stream, hipStreamNonBlocking) != hipSuccess) {
+		if(!c->copy_stream && hipStreamCreateWithFlags(&c->copy_stream, hipStreamNonBlocking) != hipSuccess) {
 			c->copy_stream = nullptr;
 			return fail(PP_ERR_HIP, "snapshot stream creation failed");
 		}
-		HIPCHECK(hipEventCreateWithFlags(&c->snap_ready, hipEventDisableTiming));
-		HIPCHECK(hipEventCreateWithFlags(&c->snap_done, hipEventDisableTiming));
-		TRY(dev_alloc(c, &c->snap_dev, (size_t) NCOMP * cap));
-		TRY(dev_alloc(c, &c->snap_dev_pid, (size_t) cap));
-		if(hipHostMalloc((void **) &c->snap_host, sizeof(double) * NCOMP * cap, hipHostMallocDefault) != hipSuccess ||
+		if(!c->snap_ready) HIPCHECK(hipEventCreateWithFlags(&c->snap_ready, hipEventDisableTiming));
+		if(!c->snap_done) HIPCHECK(hipEventCreateWithFlags(&c->snap_done, hipEventDisableTiming));
+		if(!c->snap_dev) TRY(dev_alloc(c, &c->snap_dev, (size_t) NCOMP * cap));
+		if(!c->snap_dev_pid) TRY(dev_alloc(c, &c->snap_dev_pid, (size_t) cap));
+		if(!c->snap_host &&
+		   hipHostMalloc((void **) &c->snap_host, sizeof(double) * NCOMP * cap, hipHostMallocDefault) != hipSuccess)
+			c->snap_host = nullptr;
+		if(!c->snap_host_pid &&
 		   hipHostMalloc((void **) &c->snap_host_pid, sizeof(int) * cap, hipHostMallocDefault) != hipSuccess)
-			return fail(PP_ERR_NOMEM, "pinned snapshot buffers");
+			c->snap_host_pid = nullptr;
+		if(!c->snap_host || !c->snap_host_pid) return fail(PP_ERR_NOMEM, "pinned snapshot buffers");
 		c->snap_cap = cap;
 	}
 	if(c->any_gc) {

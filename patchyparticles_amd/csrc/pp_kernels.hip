@@ -1821,6 +1821,7 @@ __global__ __launch_bounds__(256) void k_energy_pairs(DevState d) {
 	for(int k = xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x; k < n; k += stride) {
 		const int2 pq = d.pl[k];
 		const int j = pq.x, q = pq.y;
+		if((unsigned) j >= (unsigned) d.Ntot || (unsigned) q >= (unsigned) d.Ntot) continue;
 		const int r = d.prep[j];
 		const Rep &rp = d.reps[r];
 		if(!rp.trial_active || !rp.trial_pl) continue;
