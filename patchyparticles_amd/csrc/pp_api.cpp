@@ -499,7 +499,7 @@ pp_status create_impl(int R, const pp_params *prm, const pp_options *opts, const
 		// the close-pair list of the box trials: 8 pairs per slot (a
 		// hard-sphere particle has at most about 14 neighbours within 1.1
 		// bond ranges; a fuller list overflows to the cell records)
-		d.pl_cap = 8 * d.stride;
+		d.pl_cap = (int) std::min<long long>(8LL * d.stride, 0x7fffff00LL);
 		if((st = dev_alloc(c, &d.pl, (size_t) d.pl_cap)) || (st = dev_alloc(c, &d.pl_n, 2))) return bail(st);
 	}
 	if(hipHostMalloc((void **) &c->h_vslow, sizeof(int), hipHostMallocDefault) != hipSuccess) c->h_vslow = nullptr;
@@ -1385,7 +1385,7 @@ pp_status pp_set_lx_move(pp_ctx *c, int rep, double lx_change_max, double lyz_mi
 	for(const Rep &rp : c->reps) c->any_lx = c->any_lx || rp.lx_move;
 	if(c->any_lx && !c->d.pl) {
 		// the close-pair list of the box trials (as created for NPT contexts)
-		c->d.pl_cap = 8 * c->d.stride;
+		c->d.pl_cap = (int) std::min<long long>(8LL * c->d.stride, 0x7fffff00LL);
 		TRY(dev_alloc(c, &c->d.pl, (size_t) c->d.pl_cap));
 		TRY(dev_alloc(c, &c->d.pl_n, 2));
 		launch_fill_i32(c->d.pl_n, 0, 2, c->stream);
