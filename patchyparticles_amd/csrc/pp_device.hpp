@@ -20,6 +20,18 @@ __device__ __forceinline__ int xcd_remap(int b, int nb) {
 	return (b & 7) * per + (b >> 3);
 }
 
+// Buffer b (0 or 1) of the double-buffered slot arrays.  A team-uniform b
+// that is not wavefront-uniform (a selector read from a cell record) would
+// index the kernel argument with a vector load; later loads through the
+// pointer then wait on every load issued before them.  Selecting between the
+// two scalar base pointers keeps them in registers.
+template <class P>
+__device__ __forceinline__ P *pick_buf(P *const (&buf)[2], int b) {
+	const unsigned long long p0 = (unsigned long long) buf[0], p1 = (unsigned long long) buf[1];
+	const unsigned lo = b ? (unsigned) p1 : (unsigned) p0, hi = b ? (unsigned) (p1 >> 32) : (unsigned) (p0 >> 32);
+	return (P *) (((unsigned long long) hi << 32) | lo);
+}
+
 // DPP controls (gfx9 encoding)
 constexpr int DPP_QUAD_1032 = 0xB1;      // quad_perm [1,0,3,2]
 constexpr int DPP_QUAD_2301 = 0x4E;      // quad_perm [2,3,0,1]

@@ -138,8 +138,8 @@ __global__ void k_count(DevState d, int mode, uint64_t sweep, long long trace_po
 	// a free or removed slot of a grand-canonical replica goes to the
 	// replica's parking cell
 	int c = rp.cell_off + rp.cell_cap;
-	if(j - rp.p_off < rp.np_hw && d.pid[rp.sel][j] >= 0) {
-		const double *F = d.f[rp.sel];
+	if(j - rp.p_off < rp.np_hw && pick_buf(d.pid, rp.sel)[j] >= 0) {
+		const double *F = pick_buf(d.f, rp.sel);
 		const int S = d.stride;
 		double p[3] = {F[j], F[S + j], F[2 * S + j]};
 		if(tr)
@@ -431,8 +431,8 @@ __global__ void k_scatter(DevState d, int mode, uint64_t sweep) {
 	// the buffer this rebuild switches to holds the live particles first
 	if(mode != REB_TRIAL && j == rp.p_off) d.reps[r].np_hw = rp.np;
 	const int sb = tr ? rp.sel : (rp.sel ^ 1);
-	const double *F = d.f[sb];
-	double *G = d.f[sb ^ 1];
+	const double *F = pick_buf(d.f, sb);
+	double *G = pick_buf(d.f, sb ^ 1);
 	const int S = d.stride;
 	const int dst = d.start[d.cellid[j]] + d.slot[j];
 #pragma unroll
@@ -441,7 +441,7 @@ __global__ void k_scatter(DevState d, int mode, uint64_t sweep) {
 		if(tr && c < 3) v *= rp.trial_s[c];
 		G[c * S + dst] = v;
 	}
-	d.pid[sb ^ 1][dst] = d.pid[sb][j];
+	pick_buf(d.pid, sb ^ 1)[dst] = pick_buf(d.pid, sb)[j];
 }
 
 void launch_rebuild(const DevState &d, int mode, uint64_t sweep, long long trace_pos, hipStream_t s) {
@@ -680,8 +680,8 @@ __device__ __forceinline__ void pass_cell(const DevState &d, uint64_t sweep, int
 			return;
 		}
 		const Rep &rp = d.reps[r];
-		double *F = d.f[sel];
-		int *PID = d.pid[sel];
+		double *F = pick_buf(d.f, sel);
+		int *PID = pick_buf(d.pid, sel);
 
 		// 2. every load of the cell at once: the members, the static
 		// neighbours' positions and the replica's geometry
@@ -1457,7 +1457,7 @@ __global__ __launch_bounds__(256) void k_energy(DevState d, int trial, int per_p
 	const Grid &B = tr ? rp.tg : rp.g;                    // the box of the separations
 	const double s0 = fast ? rp.trial_s[0] : 1., s1 = fast ? rp.trial_s[1] : 1., s2 = fast ? rp.trial_s[2] : 1.;
 	const int b = (tr && !fast) ? (rp.sel ^ 1) : rp.sel;
-	const double *F = d.f[b];
+	const double *F = pick_buf(d.f, b);
 	const int S = d.stride;
 	const double ux = F[j], uy = F[S + j], uz = F[2 * S + j];
 	double O[9];
@@ -1503,7 +1503,7 @@ __global__ __launch_bounds__(256) void k_energy(DevState d, int trial, int per_p
 		if(tr && any) __hip_atomic_store(&d.vflag[r], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 		int res = any ? 0 : nbond;
 		if(per_particle) {
-			int i = rp.p_off + d.pid[b][j];
+			int i = rp.p_off + pick_buf(d.pid, b)[j];
 			d.out_bonds[i] = res;
 			d.out_ovl[i] = any ? 1 : 0;
 		}
@@ -1525,8 +1525,8 @@ __global__ void k_bond_list(DevState d, int max_per, int *count, int *partners) 
 	const Rep &rp = d.reps[r];
 	if(j - rp.p_off >= rp.np) return;
 	const Grid &G = rp.g;
-	const double *F = d.f[rp.sel];
-	const int *PID = d.pid[rp.sel];
+	const double *F = pick_buf(d.f, rp.sel);
+	const int *PID = pick_buf(d.pid, rp.sel);
 	const int S = d.stride;
 	const double px = F[j], py = F[S + j], pz = F[2 * S + j];
 	double O[9];
@@ -1621,7 +1621,7 @@ __global__ __launch_bounds__(256) void k_energy_rec(DevState d) {
 		if(__hip_atomic_load(&d.vflag[r], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) continue;
 		const int s0 = rec[59], cnt = rec[60];
 		const int m0 = rec[32 + 13], m2 = rec[58] - m0;  // candidates in the 13 directions
-		const double *F = d.f[rec[63]];
+		const double *F = pick_buf(d.f, rec[63]);
 		const int S = d.stride;
 		const double t0 = rp.trial_s[0], t1 = rp.trial_s[1], t2 = rp.trial_s[2];
 		const double Lx = rp.tg.box[0], Ly = rp.tg.box[1], Lz = rp.tg.box[2];
@@ -1766,7 +1766,7 @@ __global__ __launch_bounds__(256) void k_pairs_build(DevState d) {
 				cnt = rec[60];
 				m0 = rec[32 + 13];
 				m2 = rec[58] - m0;
-				F = d.f[rec[63]];
+				F = pick_buf(d.f, rec[63]);
 				for(int a = 0; a < 3; a++) {
 					L[a] = rp.g.box[a];
 					iL[a] = rp.g.inv_box[a];
@@ -1827,7 +1827,7 @@ __global__ __launch_bounds__(256) void k_energy_pairs(DevState d) {
 		if(!rp.trial_active || !rp.trial_pl) continue;
 		// (the early exit may read a stale flag: it only saves work)
 		if(__hip_atomic_load(&d.vflag[r], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) continue;
-		const double *F = d.f[rp.sel];
+		const double *F = pick_buf(d.f, rp.sel);
 		const int S = d.stride;
 		const double t0 = rp.trial_s[0], t1 = rp.trial_s[1], t2 = rp.trial_s[2];
 		const double px = F[j] * t0, py = F[S + j] * t1, pz = F[2 * S + j] * t2;
@@ -1882,9 +1882,9 @@ __global__ void k_colour_conflicts(DevState d, int rep, int n0, int n1, int n2, 
 	const int i = blockIdx.x * blockDim.x + threadIdx.x;
 	if(i >= rp.np_hw) return;
 	const int j = rp.p_off + i;
-	const int *PID = d.pid[rp.sel];
+	const int *PID = pick_buf(d.pid, rp.sel);
 	if(PID[j] < 0) return;
-	const double *F = d.f[rp.sel];
+	const double *F = pick_buf(d.f, rp.sel);
 	const int S = d.stride;
 	const double x = F[j], y = F[S + j], z = F[2 * S + j];
 	auto cell = [&](double a, double b, double c, int &cx, int &cy, int &cz) {
@@ -2101,7 +2101,7 @@ __global__ void k_vol_apply(DevState d) {
 	if(j >= d.Ntot) return;
 	const Rep &rp = d.reps[d.prep[j]];
 	if(!rp.vol_apply) return;
-	double *F = d.f[rp.sel];
+	double *F = pick_buf(d.f, rp.sel);
 	const int S = d.stride;
 	for(int c = 0; c < 3; c++) F[(size_t) c * S + j] *= rp.trial_s[c];
 }
@@ -2140,7 +2140,7 @@ __global__ void k_gc_begin(DevState d, uint64_t sweep) {
 	Rep &rp = d.reps[r];
 	if(!rp.gc) return;
 	const int loc = j - rp.p_off;
-	if(loc < rp.np) d.pid[rp.sel][j] = loc;
+	if(loc < rp.np) pick_buf(d.pid, rp.sel)[j] = loc;
 	if(loc == 0) {
 		const u32x4 v = rng(d.seed, r, sweep, 0, TAG_GC, 0, 0);
 		rp.gc_colour = (int) (v.x & 7u);
@@ -2237,8 +2237,8 @@ __global__ __launch_bounds__(256) void k_gc_pass(DevState d, uint64_t sweep) {
 	const int cloc = (ix * n1 + iy) * n2 + iz;
 	const int c = rp.cell_off + cloc;
 	const int s0 = d.start[c], cnt = d.start[c + 1] - s0;
-	double *F = d.f[rp.sel];
-	int *PID = d.pid[rp.sel];
+	double *F = pick_buf(d.f, rp.sel);
+	int *PID = pick_buf(d.pid, rp.sel);
 	const int S = d.stride;
 	const u32x4 u = rng(d.seed, r, sweep, (uint32_t) cloc, TAG_GC, (uint32_t) col, 1);
 	const u32x4 w = rng(d.seed, r, sweep, (uint32_t) cloc, TAG_GC, (uint32_t) col, 2);
@@ -2313,7 +2313,7 @@ __global__ void k_gram_schmidt(DevState d) {
 	if(j >= d.Ntot) return;
 	const Rep &rp = d.reps[d.prep[j]];
 	if(j - rp.p_off >= rp.np) return;
-	double *F = d.f[rp.sel];
+	double *F = pick_buf(d.f, rp.sel);
 	const int S = d.stride;
 	double O[9];
 #pragma unroll

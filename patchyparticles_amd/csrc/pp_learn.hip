@@ -687,8 +687,8 @@ __global__ __launch_bounds__(64 * LP_WAVES) void k_learn_pass(DevState d, NetDev
 			L[k] = rp.g.box[k];
 			iL[k] = rp.g.inv_box[k];
 		}
-		F = d.f[v.sel];
-		const int *PID = d.pid[v.sel];
+		F = pick_buf(d.f, v.sel);
+		const int *PID = pick_buf(d.pid, v.sel);
 		const u32x4 ra = rng(d.seed, v.r, sweep, (uint32_t) v.cloc, TAG_LEARN_A, (uint32_t) pass, 0);
 		const int target = (int) __umulhi(ra.x, (uint32_t) v.cnt);
 		bool hit = false;
@@ -809,7 +809,7 @@ __global__ __launch_bounds__(64 * LP_WAVES) void k_learn_pass(DevState d, NetDev
 	}
 	if(!inside) return;
 	const Rep &rp = d.reps[v.r];
-	double *Fw = d.f[v.sel];
+	double *Fw = pick_buf(d.f, v.sel);
 	const double *pp = S.prop[cell];
 	double p[3], O[9], np[3], On[9];
 	for(int k = 0; k < 3; k++) p[k] = Fw[(size_t) k * SS + j];
@@ -964,7 +964,7 @@ __global__ __launch_bounds__(64 * TS_WAVES) void k_train_samples(DevState d, uin
 		// chosen with probability proportional to its size
 		j = (int) __umulhi(r0.x, (uint32_t) d.Ntot);
 		const Rep &rp = d.reps[d.prep[j]];
-		F = d.f[rp.sel];
+		F = pick_buf(d.f, rp.sel);
 		for(int k = 0; k < 3; k++) {
 			L[k] = rp.g.box[k];
 			iL[k] = rp.g.inv_box[k];
