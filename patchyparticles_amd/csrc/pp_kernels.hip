@@ -705,18 +705,22 @@ __device__ __forceinline__ void pass_cell(const DevState &d, uint64_t sweep, int
 			}
 			return nb_tab[lo2] + (i - nb_tab[32 + lo2]);
 		};
-		// the first T * CPL static neighbours' positions into registers; the
-		// rest (dense neighbourhoods only) are read again at every trial
+		// Register slot lane + u T (u < CPL) of the team holds one neighbour:
+		// slots below cnt the cell's members (member = lane, kept current
+		// from the cell's LDS block at every trial), then the first static
+		// neighbours; the rest (dense neighbourhoods only) are read again at
+		// every trial.  A lane thus tests one neighbour per trial whether the
+		// cell holds one member or several.
 		double cxr[CPL], cyr[CPL], czr[CPL];
-		int cqr[CPL];
+		int cqr[CPL];  // the static neighbour's slot, -1: a member or none
 		double orv[CPL][9];
 		auto load_static = [&]() {
 #pragma unroll
 			for(int u = 0; u < CPL; u++) {
-				const int i = lane + u * T;
+				const int i = lane + u * T - cnt;
 				cqr[u] = -1;
 				cxr[u] = cyr[u] = czr[u] = 0.;
-				if(i < M) {
+				if(i >= 0 && i < M) {
 					const int q = nb_slot(i);
 					cqr[u] = q;
 					cxr[u] = nbld<DD>(F + q);
@@ -801,19 +805,25 @@ __device__ __forceinline__ void pass_cell(const DevState &d, uint64_t sweep, int
 			}
 			team_sync();
 		}
-		// this lane's static neighbours (the surrounding cells, fixed during
-		// the pass): the one in registers and, in dense neighbourhoods, the
-		// rest from HBM
+		// (orientations in LDS) a member slot's orientation, kept current by
+		// the trials that move it
+		if constexpr(!REG_ORI)
+			if(lane < cnt) {
+#pragma unroll
+				for(int c = 0; c < 9; c++) ori[c * T + lane] = own[lane * OWN_W + 3 + c];
+			}
+		// the orientation of the neighbour in register slot u
 		auto reg_o = [&](int u, double *Oq) {
 #pragma unroll
 			for(int c = 0; c < 9; c++) Oq[c] = REG_ORI ? orv[u][c] : ori[(9 * u + c) * T + lane];
 		};
 		// whether any team of the wavefront has neighbours beyond the
-		// register-resident ones (a scalar branch around those loops)
-		const bool wave_far = __ballot(M > T * CPL) != 0ull;
+		// register slots (a scalar branch around those loops)
+		const bool wave_far = __ballot(M + cnt > T * CPL) != 0ull;
+		// the static neighbours beyond the register slots, from HBM
 		auto each_far = [&](auto &&f) {
 			if(wave_far)
-				for(int b = T * CPL; b < M; b += T)
+				for(int b = T * CPL - cnt; b < M; b += T)
 					if(b + lane < M) {
 						const int q = nb_slot(b + lane);
 						f(nbld<DD>(F + q), nbld<DD>(F + S + q), nbld<DD>(F + 2 * S + q), [&](double *Oq) {
@@ -934,14 +944,21 @@ __device__ __forceinline__ void pass_cell(const DevState &d, uint64_t sweep, int
 			double tv[TRIAL_W];
 #pragma unroll
 			for(int e = 0; e < TRIAL_W; e++) tv[e] = tp[e];
-			const bool has_m = lane < cnt && lane != k;
-			const double *wl = own + (lane < cnt ? lane : 0) * OWN_W;
-			double mx = 0., my = 0., mz = 0.;
-			if(has_m) {
-				mx = wl[0];
-				my = wl[1];
-				mz = wl[2];
+			// a member slot: the member's current state (every member but the
+			// moving one is a neighbour of the trial)
+			const bool is_mem = lane < cnt;
+			const bool has_m = is_mem && lane != k;
+			if(is_mem) {
+				const double *wl = own + lane * OWN_W;
+				cxr[0] = wl[0];
+				cyr[0] = wl[1];
+				czr[0] = wl[2];
+				if constexpr(REG_ORI)
+#pragma unroll
+					for(int c = 0; c < 9; c++) orv[0][c] = wl[3 + c];
 			}
+			// register slot u holds a neighbour of this trial
+			auto has_slot = [&](int u) { return u == 0 && is_mem ? has_m : cqr[u] >= 0; };
 			const double *me = own + k * OWN_W;
 			double ux = 0., uy = 0., uz = 0.;  // (kept form) the member's u
 			if constexpr(KEEP) {
@@ -996,17 +1013,12 @@ __device__ __forceinline__ void pass_cell(const DevState &d, uint64_t sweep, int
 			double sx[CPL], sy[CPL], sz[CPL], s2[CPL];
 #pragma unroll
 			for(int u = 0; u < CPL; u++)
-				if(cqr[u] >= 0) {
+				if(has_slot(u)) {
 					s2[u] = sep_new(tr, cxr[u], cyr[u], czr[u], sx[u], sy[u], sz[u]);
 					ov |= s2[u] < 1.;
 				}
-			double msx = 0., msy = 0., msz = 0., ms2 = 0.;
-			if(has_m) {
-				ms2 = sep_new(tr, mx, my, mz, msx, msy, msz);
-				ov |= ms2 < 1.;
-			}
 			if(wave_far)
-				for(int b = T * CPL; b < M; b += T)
+				for(int b = T * CPL - cnt; b < M; b += T)
 					if(b + lane < M) {
 						const int q = nb_slot(b + lane);
 						ov |= overlaps_new(tr, nbld<DD>(F + q), nbld<DD>(F + S + q), nbld<DD>(F + 2 * S + q));
@@ -1034,35 +1046,49 @@ __device__ __forceinline__ void pass_cell(const DevState &d, uint64_t sweep, int
 			for(int c = 0; c < 12; c++) Pn[c] = stage[9 + c];
 
 			// phase 2: bonds at the old and the new position, over this lane's
-			// neighbours: the static ones (KEEP: new position only, the old
-			// count is the member's kept one) and one member of the cell
-			auto member_o = [&](double *Oq) {
-#pragma unroll
-				for(int c = 0; c < 9; c++) Oq[c] = wl[3 + c];
-			};
+			// neighbours: its register slots (a member or static neighbours;
+			// KEEP: new position only, the old count is the member's kept one)
+			// and the static neighbours beyond them
 			int bo, bn, s_new = 0, bnm = 0;
 			if constexpr(KEEP) {
 				int bns = 0;
 				const int bom = has_m ? (int) ((mbits >> k) & 1u) : 0;
+				// a register slot's bond: with a member (slot 0 of a member
+				// lane) or with a static neighbour
+				auto add_slot = [&](int u, int b) {
+					if(u == 0) {
+						bnm += is_mem ? b : 0;
+						bns += is_mem ? 0 : b;
+					}
+					else bns += b;
+				};
 				bool band = tr.c2 < 0.;  // no filter: every lane takes the exact form
 				if(!band) {
 #pragma unroll
 					for(int u = 0; u < CPL; u++)
-						if(cqr[u] >= 0)
-							bonds_new_sep(tr, Pn, sx[u], sy[u], sz[u], s2[u], [&](double *Oq) { reg_o(u, Oq); }, bns,
+						if(has_slot(u)) {
+							int b = 0;
+							bonds_new_sep(tr, Pn, sx[u], sy[u], sz[u], s2[u], [&](double *Oq) { reg_o(u, Oq); }, b,
 							              band);
+							add_slot(u, b);
+						}
 					each_far([&](double qx, double qy, double qz, auto load_o) {
 						bonds_new(tr, Pn, qx, qy, qz, load_o, bns, band);
 					});
-					if(has_m) bonds_new_sep(tr, Pn, msx, msy, msz, ms2, member_o, bnm, band);
 				}
 				if(__ballot(band) != 0ull && team_any<T>(band) && band) {
 					bns = 0;
 					bnm = 0;
-					each_static([&](double qx, double qy, double qz, auto load_o) {
+#pragma unroll
+					for(int u = 0; u < CPL; u++)
+						if(has_slot(u)) {
+							int b = 0;
+							bonds_new_exact(tr, Pn, cxr[u], cyr[u], czr[u], [&](double *Oq) { reg_o(u, Oq); }, b);
+							add_slot(u, b);
+						}
+					each_far([&](double qx, double qy, double qz, auto load_o) {
 						bonds_new_exact(tr, Pn, qx, qy, qz, load_o, bns);
 					});
-					if(has_m) bonds_new_exact(tr, Pn, mx, my, mz, member_o, bnm);
 				}
 				// three counts below 1024 in one reduction
 				const int packed = team_sum<T>(bns | (bnm << 10) | (bom << 20));
@@ -1075,8 +1101,10 @@ __device__ __forceinline__ void pass_cell(const DevState &d, uint64_t sweep, int
 				bn = 0;
 				bool band = tr.c2 < 0.;
 				auto each_neighbour = [&](auto &&f) {
-					each_static(f);
-					if(has_m) f(mx, my, mz, member_o);
+#pragma unroll
+					for(int u = 0; u < CPL; u++)
+						if(has_slot(u)) f(cxr[u], cyr[u], czr[u], [&](double *Oq) { reg_o(u, Oq); });
+					each_far(f);
 				};
 				if(!band)
 					each_neighbour([&](double qx, double qy, double qz, auto load_o) {
@@ -1113,6 +1141,9 @@ __device__ __forceinline__ void pass_cell(const DevState &d, uint64_t sweep, int
 				if(lane < NW) wk[3 + lane] = rv;
 				if constexpr(T < NW)
 					if(lane + T < NW) wk[3 + lane + T] = rv2;
+				// (orientations in LDS) and into member k's register slot
+				if constexpr(!REG_ORI)
+					if(lane < 9) ori[lane * T + k] = rv;
 				if(lane == 0) {
 					wk[0] = tr.nx;
 					wk[1] = tr.ny;

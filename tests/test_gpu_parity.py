@@ -206,16 +206,23 @@ def _fcc(cells, a, seed):
     return r, q, np.full(3, cells * a)
 
 
+@pytest.mark.parametrize("lattice", ["rho0.89", "rho0.70"])
 @pytest.mark.parametrize("tpc", [16, 32, 64])
-def test_dense_neighbourhood_beyond_registers(pp, oracle, tpc):
+def test_dense_neighbourhood_beyond_registers(pp, oracle, tpc, lattice):
     """FCC at rho=0.89 in a box of 4^3 sweep cells of side 1.65: the 26 cells
     around a sweep cell hold ~104 particles, more than one team keeps in
-    registers (32 or 64), so the colour pass reads the rest from memory.  Overlaps
-    and bonds must still be seen: the run stays overlap-free, the tracked
-    energy equals a full recompute, and the bonds equal the oracle's."""
-    r, O, box = _fcc(4, 1.65, 4)
+    registers (32 or 64), so the colour pass reads the rest from memory.  FCC
+    at rho=0.70 in 12^3 cells of side 1.19: the 27 cells hold ~32 particles,
+    so with 32 register slots per team (16 lanes x 2, or 32 lanes) the cell's
+    members and its first static neighbours fill them and about half the
+    cells read their last few static neighbours from memory, while 64-lane
+    teams keep all of them.  Overlaps and bonds must still be seen: the run
+    stays overlap-free, the tracked energy equals a full recompute, the bonds
+    equal the oracle's, and another team width gives the same trajectory."""
+    cells, a, nside = (4, 1.65, 4) if lattice == "rho0.89" else (8, (4 / 0.7) ** (1 / 3), 12)
+    r, O, box = _fcc(cells, a, 4)
     with pp.System(r, O, box, T=0.12, seed=21, threads_per_cell=tpc) as s:
-        assert s.grid()[0] == [4, 4, 4]
+        assert s.grid()[0] == [nside] * 3
         s.do_ensemble(40)
         rr, OO, bb = s.state()
         b, ov, tot, anyo = s.energy()
@@ -224,7 +231,7 @@ def test_dense_neighbourhood_beyond_registers(pp, oracle, tpc):
     assert not anyo and not oov.any()
     assert np.array_equal(b, ob) and c["bonds"] == tot == otot
     assert c["accepted"][0] > 0.2 * c["tries"][0]
-    with pp.System(r, O, box, T=0.12, seed=21, threads_per_cell=64 if tpc == 32 else 32) as s:
+    with pp.System(r, O, box, T=0.12, seed=21, threads_per_cell=64 if tpc != 64 else 32) as s:
         s.do_ensemble(40)
         other = s.state()
     for x, y in zip((rr, OO, bb), other):
