@@ -62,6 +62,45 @@ __device__ __forceinline__ int team_sum(int v) {
 	return v;
 }
 
+// Sum of a double over a team of 16 lanes (DPP butterflies within a row, on
+// the two halves of the value).  Lanes may round differently: for bounds
+// only.
+__device__ __forceinline__ double dpp_add_f64_quad1032(double x) {
+	const long long b = __double_as_longlong(x);
+	const int lo = __builtin_amdgcn_update_dpp(0, (int) b, DPP_QUAD_1032, 0xf, 0xf, false);
+	const int hi = __builtin_amdgcn_update_dpp(0, (int) (b >> 32), DPP_QUAD_1032, 0xf, 0xf, false);
+	return x + __longlong_as_double(((long long) hi << 32) | (unsigned) lo);
+}
+__device__ __forceinline__ double dpp_add_f64_quad2301(double x) {
+	const long long b = __double_as_longlong(x);
+	const int lo = __builtin_amdgcn_update_dpp(0, (int) b, DPP_QUAD_2301, 0xf, 0xf, false);
+	const int hi = __builtin_amdgcn_update_dpp(0, (int) (b >> 32), DPP_QUAD_2301, 0xf, 0xf, false);
+	return x + __longlong_as_double(((long long) hi << 32) | (unsigned) lo);
+}
+__device__ __forceinline__ double dpp_add_f64_half_mirror(double x) {
+	const long long b = __double_as_longlong(x);
+	const int lo = __builtin_amdgcn_update_dpp(0, (int) b, DPP_HALF_MIRROR, 0xf, 0xf, false);
+	const int hi = __builtin_amdgcn_update_dpp(0, (int) (b >> 32), DPP_HALF_MIRROR, 0xf, 0xf, false);
+	return x + __longlong_as_double(((long long) hi << 32) | (unsigned) lo);
+}
+__device__ __forceinline__ double dpp_add_f64_row_mirror(double x) {
+	const long long b = __double_as_longlong(x);
+	const int lo = __builtin_amdgcn_update_dpp(0, (int) b, DPP_ROW_MIRROR, 0xf, 0xf, false);
+	const int hi = __builtin_amdgcn_update_dpp(0, (int) (b >> 32), DPP_ROW_MIRROR, 0xf, 0xf, false);
+	return x + __longlong_as_double(((long long) hi << 32) | (unsigned) lo);
+}
+__device__ __forceinline__ double team16_sum_f64(double x) {
+	x = dpp_add_f64_quad1032(x);
+	x = dpp_add_f64_quad2301(x);
+	x = dpp_add_f64_half_mirror(x);
+	return dpp_add_f64_row_mirror(x);
+}
+
+// Lanes below this one whose bit is set in mask m.
+__device__ __forceinline__ int lanes_below(unsigned long long m) {
+	return (int) __builtin_amdgcn_mbcnt_hi((unsigned) (m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned) m, 0u));
+}
+
 // Inclusive prefix sum over a whole wavefront (every lane active):
 // Hillis-Steele steps within each 16-lane row (row_shr 1, 2, 4, 8, the
 // lanes shifted in from outside the row reading 0), then row 0's total into

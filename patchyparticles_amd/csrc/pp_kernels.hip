@@ -586,6 +586,10 @@ __device__ __forceinline__ double rot_value(const RotShare &r, double o0, double
 // counts cost about what the trials save.
 constexpr int KEEP_MIN_TRIALS = 4;
 
+// Margin of the reach filter over rounding in positions and distances
+// (below 1e-12 for coordinates of a few hundred).
+constexpr double REACH_EPS = 1e-9;
+
 // A read of another cell's particle data during a pass.  In the
 // dependency-driven sweep (DD: the passes of a sweep in one launch, each cell
 // starting once its neighbours of earlier passes are done) it is an
@@ -641,7 +645,7 @@ __device__ __forceinline__ void wait_neighbours(const DevState &d, const Rep &rp
 // earlier passes, reads their data with agent-scope loads, and finally
 // writes its members back with agent-scope stores, waits for them to
 // complete, and publishes `mark` for its own cell.
-template <int T, int CPL, bool KEEP, bool ONE, bool DD>
+template <int T, int CPL, bool KEEP, bool ONE, bool DD, bool REACH = false>
 __device__ __forceinline__ void pass_cell(const DevState &d, uint64_t sweep, int pass, int gi, int lane,
                                           double *blk, unsigned mark) {
 	// the team's block, the per-trial data first (short LDS offsets)
@@ -652,9 +656,10 @@ __device__ __forceinline__ void pass_cell(const DevState &d, uint64_t sweep, int
 	int *sbond = (int *) (own + LDS_OWN + LDS_PID);  // member m's bonds with the static neighbours
 	int *nb_tab = (int *) (own + LDS_OWN + LDS_PID + LDS_SB);  // the cell's record (build_record)
 	// the orientations of this lane's register neighbours: in registers where
-	// they allow (scalar replica constants), else in LDS, component c of
-	// neighbour u at (9 u + c) * T + lane
-	constexpr bool REG_ORI = (ONE || T == 64) && CPL == 1;
+	// they allow (scalar replica constants, and one slot per lane or a reach
+	// filter that leaves the second slot almost always empty), else in LDS,
+	// component c of neighbour u at (9 u + c) * T + lane
+	constexpr bool REG_ORI = (ONE || T == 64) && (CPL == 1 || REACH);
 	double *ori = blk + LDS_FIXED;
 	const int S = d.stride;
 	{
@@ -756,10 +761,12 @@ __device__ __forceinline__ void pass_cell(const DevState &d, uint64_t sweep, int
 		// (MC.c:332), so a trial decides without a table read.  Lane t keeps
 		// trial t's rank.
 		int my_k = 0;
+		double my_dl = 0.;  // (REACH) the length of this lane's trial displacement
 		if(lane < ntrial) {
 			double tv[TRIAL_W];
 			const int col = ((ix & 1) << 2) | ((iy & 1) << 1) | (iz & 1);
 			make_trial(d, rp, r, sweep, cloc, col, lane, cnt, tv);
+			if constexpr(REACH) my_dl = __builtin_sqrt(tv[1] * tv[1] + tv[2] * tv[2] + tv[3] * tv[3]);
 			unsigned am = 0;
 #pragma unroll
 			for(int j = 1; j < ACC_TAB; j++) am |= (unsigned) (tv[13] < rp.acc_tab[j]) << j;
@@ -771,7 +778,7 @@ __device__ __forceinline__ void pass_cell(const DevState &d, uint64_t sweep, int
 		static_assert(MAX_TRIALS_PER_CELL <= 32 && CELL_CAP <= 32, "one lane per trial");
 		const int koff = (threadIdx.x & 63) & ~(T - 1);
 		if(lane < cnt) own_pid[lane] = my_pid;
-		if constexpr(!REG_ORI && !DD) {
+		if constexpr(!REG_ORI && !DD && !REACH) {
 #pragma unroll
 			for(int u = 0; u < CPL; u++)
 #pragma unroll
@@ -793,6 +800,89 @@ __device__ __forceinline__ void pass_cell(const DevState &d, uint64_t sweep, int
 				for(int c = 0; c < 3; c++) w[NCOMP + 3 * i + c] = P[i][c];
 		}
 		team_sync();
+		if constexpr(REACH) {
+			// Keep only the static neighbours a member can reach.  No member
+			// travels further during the pass than S, the summed lengths of
+			// the cell's trial displacements (triangle inequality on the
+			// periodic box), so a neighbour further than 1 + delta + S from
+			// every member's start can neither overlap nor bond with any
+			// trial position: dropping it changes no decision.  The kept ones
+			// are compacted into the register slots after the members (most
+			// teams then need one slot per lane) through the orientation
+			// block of LDS, 8 values then 5 values per slot.
+			const double S = team16_sum_f64(my_dl);
+			const double reach = __builtin_sqrt(tr.rc2) + S + REACH_EPS;
+			const double reach2 = reach * reach;
+			bool keep_u[CPL];
+#pragma unroll
+			for(int u = 0; u < CPL; u++) {
+				double dmin2 = reach2;
+				if(cqr[u] >= 0)
+					for(int m = 0; m < cnt; m++) {
+						const double *wm = own + m * OWN_W;
+						const double dx = kf_min_image(cxr[u] - wm[0], tr.Lx, tr.iLx);
+						const double dy = kf_min_image(cyr[u] - wm[1], tr.Ly, tr.iLy);
+						const double dz = kf_min_image(czr[u] - wm[2], tr.Lz, tr.iLz);
+						dmin2 = __builtin_fmin(dmin2, dx * dx + dy * dy + dz * dz);
+					}
+				keep_u[u] = cqr[u] >= 0 && dmin2 < reach2;
+			}
+			static_assert(!REACH || (T == 16 && CPL == 2), "reach filter: 16-lane teams, two slots per lane");
+			const unsigned long long tmask = ((1ull << T) - 1ull) << koff;
+			const unsigned long long k0 = __ballot(keep_u[0]) & tmask, k1 = __ballot(keep_u[CPL - 1]) & tmask;
+			const int n0 = __popcll(k0), nr = n0 + __popcll(k1);
+			const int dst[2] = {cnt + lanes_below(k0), cnt + n0 + lanes_below(k1)};
+			double *stg = ori;  // value v of slot j at stg[32 v + j]
+			double stale[CPL][5];
+#pragma unroll
+			for(int u = 0; u < CPL; u++) {
+#pragma unroll
+				for(int c = 0; c < 5; c++) stale[u][c] = orv[u][4 + c];
+				if(keep_u[u]) {
+					const int j = dst[u];
+					stg[j] = (double) cqr[u];
+					stg[32 + j] = cxr[u];
+					stg[64 + j] = cyr[u];
+					stg[96 + j] = czr[u];
+#pragma unroll
+					for(int c = 0; c < 4; c++) stg[32 * (4 + c) + j] = orv[u][c];
+				}
+			}
+			team_sync();
+#pragma unroll
+			for(int u = 0; u < CPL; u++) {
+				const int j = lane + u * T;
+				const bool in = j >= cnt && j < cnt + nr;
+				cqr[u] = in ? (int) stg[j] : -1;
+				cxr[u] = in ? stg[32 + j] : 0.;
+				cyr[u] = in ? stg[64 + j] : 0.;
+				czr[u] = in ? stg[96 + j] : 0.;
+#pragma unroll
+				for(int c = 0; c < 4; c++) orv[u][c] = in ? stg[32 * (4 + c) + j] : 0.;
+			}
+			team_sync();
+#pragma unroll
+			for(int u = 0; u < CPL; u++)
+				if(keep_u[u])
+#pragma unroll
+					for(int c = 0; c < 5; c++) stg[32 * c + dst[u]] = stale[u][c];
+			team_sync();
+#pragma unroll
+			for(int u = 0; u < CPL; u++) {
+				const int j = lane + u * T;
+				const bool in = j >= cnt && j < cnt + nr;
+#pragma unroll
+				for(int c = 0; c < 5; c++) orv[u][4 + c] = in ? stg[32 * c + j] : 0.;
+			}
+			team_sync();
+			if constexpr(!REG_ORI) {
+#pragma unroll
+				for(int u = 0; u < CPL; u++)
+#pragma unroll
+					for(int c = 0; c < 9; c++) ori[(9 * u + c) * T + lane] = orv[u][c];
+			}
+			team_sync();
+		}
 		if constexpr(DD) {
 			// the neighbours' data once their earlier passes are done
 			wait_neighbours<T>(d, rp, nb_tab, pass, ix, iy, iz, n0, n1, n2, mark, lane);
@@ -842,29 +932,48 @@ __device__ __forceinline__ void pass_cell(const DevState &d, uint64_t sweep, int
 		// 5. KEEP: every member's bonds with the static neighbours
 		unsigned mbits = 0;
 		if constexpr(KEEP) {
-			for(int m = 0; m < cnt; m++) {
-				const double *wm = own + m * OWN_W;
-				Trial tm = tr;
-				tm.nx = unid<T>(wm[0]);
-				tm.ny = unid<T>(wm[1]);
-				tm.nz = unid<T>(wm[2]);
-				double Pm[12];
+			// two members at a time (independent chains), one reduction for both
+			for(int m = 0; m < cnt; m += 2) {
+				const bool two = m + 1 < cnt;
+				const double *w0 = own + m * OWN_W, *w1 = own + (two ? m + 1 : m) * OWN_W;
+				Trial t0 = tr, t1 = tr;
+				t0.nx = unid<T>(w0[0]);
+				t0.ny = unid<T>(w0[1]);
+				t0.nz = unid<T>(w0[2]);
+				t1.nx = unid<T>(w1[0]);
+				t1.ny = unid<T>(w1[1]);
+				t1.nz = unid<T>(w1[2]);
+				double P0[12], P1[12];
 #pragma unroll
-				for(int c = 0; c < 12; c++) Pm[c] = wm[NCOMP + c];
-				int b = 0;
-				bool band = tr.c2 < 0.;
-				if(!band)
+				for(int c = 0; c < 12; c++) {
+					P0[c] = w0[NCOMP + c];
+					P1[c] = w1[NCOMP + c];
+				}
+				int b0 = 0, b1 = 0;
+				bool band0 = tr.c2 < 0., band1 = band0;
+				if(!band0)
 					each_static([&](double qx, double qy, double qz, auto load_o) {
-						bonds_new(tm, Pm, qx, qy, qz, load_o, b, band);
+						bonds_new(t0, P0, qx, qy, qz, load_o, b0, band0);
+						if(two) bonds_new(t1, P1, qx, qy, qz, load_o, b1, band1);
 					});
-				if(__ballot(band) != 0ull && team_any<T>(band) && band) {
-					b = 0;
+				if(__ballot(band0) != 0ull && team_any<T>(band0) && band0) {
+					b0 = 0;
 					each_static([&](double qx, double qy, double qz, auto load_o) {
-						bonds_new_exact(tm, Pm, qx, qy, qz, load_o, b);
+						bonds_new_exact(t0, P0, qx, qy, qz, load_o, b0);
 					});
 				}
-				b = team_sum<T>(b);
-				if(lane == 0) sbond[m] = b;
+				if(__ballot(band1) != 0ull && team_any<T>(band1) && band1) {
+					b1 = 0;
+					each_static([&](double qx, double qy, double qz, auto load_o) {
+						bonds_new_exact(t1, P1, qx, qy, qz, load_o, b1);
+					});
+				}
+				// (each count is below the team's neighbour count, M < 65536)
+				const int b = team_sum<T>(b0 | (b1 << 16));
+				if(lane == 0) {
+					sbond[m] = b & 0xffff;
+					if(two) sbond[m + 1] = b >> 16;
+				}
 			}
 			// and the members' bonds among themselves: lane m < cnt keeps bit
 			// j of mbits set iff members m and j are bonded (exact form; the
@@ -945,10 +1054,11 @@ __device__ __forceinline__ void pass_cell(const DevState &d, uint64_t sweep, int
 #pragma unroll
 			for(int e = 0; e < TRIAL_W; e++) tv[e] = tp[e];
 			// a member slot: the member's current state (every member but the
-			// moving one is a neighbour of the trial)
+			// moving one is a neighbour of the trial; a cell of one member
+			// reads nothing here)
 			const bool is_mem = lane < cnt;
 			const bool has_m = is_mem && lane != k;
-			if(is_mem) {
+			if(has_m) {
 				const double *wl = own + lane * OWN_W;
 				cxr[0] = wl[0];
 				cyr[0] = wl[1];
@@ -1216,7 +1326,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
 	if(i >= per_xcd) return;
 	const int gi = uni<T>(((blockIdx.x & 7) * per_xcd + i) * tpb + team);
 	if(gi >= total) return;
-	pass_cell<T, CPL, KEEP, ONE, false>(a.d, a.sweep, a.pass, gi, lane, lds + team * team_lds<T>(), 0u);
+	// 16-lane teams with two slots per lane keep the reachable neighbours only
+	pass_cell<T, CPL, KEEP, ONE, false, T == 16 && CPL == 2 && KEEP>(a.d, a.sweep, a.pass, gi, lane,
+	                                                                  lds + team * team_lds<T>(), 0u);
 }
 
 // The 8 colour passes of a sweep in one launch, driven by dependencies
