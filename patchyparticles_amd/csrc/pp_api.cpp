@@ -493,7 +493,8 @@ pp_status create_impl(int R, const pp_params *prm, const pp_options *opts, const
 	   (st = dev_alloc(c, &d.out_bonds, d.stride)) || (st = dev_alloc(c, &d.out_ovl, d.stride)) ||
 	   (st = dev_alloc(c, &d.bar, 2)) || (st = dev_alloc(c, &d.vflag, R)) || (st = dev_alloc(c, &d.vslow, 1)) ||
 	   (st = dev_alloc(c, &d.done, d.C_total + 1)) || (st = dev_alloc(c, &d.ddq, 1)) ||
-	   (st = dev_alloc(c, &d.ccheck, 2)))
+	   (st = dev_alloc(c, &d.ccheck, 2)) || (st = dev_alloc(c, &d.ovf, std::max(d.total_act, 1))) ||
+	   (st = dev_alloc(c, &d.ovf_n, 8)))
 		return bail(st);
 	if(c->any_npt || c->any_lx) {
 		// the close-pair list of the box trials: 8 pairs per slot (a
@@ -519,6 +520,12 @@ pp_status create_impl(int R, const pp_params *prm, const pp_options *opts, const
 			return e && *e && strcmp(e, "0") != 0;
 		};
 		d.npt_trial_form = on("PP_NPT_FULL_TRIAL") ? 2 : (on("PP_NPT_PARTICLE_TRIAL") ? 1 : (on("PP_NPT_RECORD_TRIAL") ? 3 : 0));
+		// PP_SMALL_CELL_CAP=k (1 to the small-cell pass's capacity): defer
+		// cells of more than k members to the general pass (a diagnostic
+		// that must give the same trajectories)
+		d.small_cap = small_cell_cap();
+		if(const char *e = getenv("PP_SMALL_CELL_CAP"))
+			if(*e) d.small_cap = std::max(1, std::min(small_cell_cap(), atoi(e)));
 	}
 	if(hipDeviceGetAttribute(&c->cus, hipDeviceAttributeMultiprocessorCount, o.device) != hipSuccess || c->cus <= 0)
 		c->cus = 256;
@@ -526,7 +533,8 @@ pp_status create_impl(int R, const pp_params *prm, const pp_options *opts, const
 		// every buffer a kernel may touch must exist before the first launch
 		const void *need[] = {d.f[0], d.f[1], d.pid[0], d.pid[1], d.reps, d.prep, d.act_off, d.count, d.start,
 		                      d.slot, d.cellid, d.scan_status, d.tile_sum, d.clist, d.ccount, d.pfx, d.rec, d.scan_ticket,
-		                      d.ctr, d.err, d.out_bonds, d.out_ovl, d.bar, d.vflag, d.vslow, d.done, d.ddq, d.ccheck};
+		                      d.ctr, d.err, d.out_bonds, d.out_ovl, d.bar, d.vflag, d.vslow, d.done, d.ddq, d.ccheck,
+		                      d.ovf, d.ovf_n};
 		for(const void *q : need)
 			if(!q) return bail(fail(PP_ERR_NOMEM, "a device buffer was not allocated"));
 	}
@@ -562,6 +570,7 @@ pp_status create_impl(int R, const pp_params *prm, const pp_options *opts, const
 	launch_fill_i32(d.vslow, 0, 1, c->stream);
 	launch_fill_i32((int *) d.done, 0, d.C_total + 1, c->stream);
 	launch_fill_i32(d.ddq, 0, 1, c->stream);
+	launch_fill_i32(d.ovf_n, 0, 8, c->stream);
 	launch_fill_u64(d.ccheck, 0ull, 2, c->stream);
 	if(d.pl_n) launch_fill_i32(d.pl_n, 0, 2, c->stream);
 	launch_fill_u64(d.ctr, 0ull, (long long) R * NCTR * NSLOT, c->stream);

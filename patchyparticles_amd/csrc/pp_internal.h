@@ -237,12 +237,16 @@ struct DevState {
 	unsigned *done;    // [C_total] dependency-driven sweeps: mark of the sweep whose pass the cell last finished
 	int *ddq;          // [1] dependency-driven sweeps: tickets taken (reset by the rebuild)
 	unsigned long long *ccheck;  // [1] checkerboard conflicts found by the live checks
+	int *ovf;          // [total_act] list entries of a pass deferred by the small-cell pass to the general one
+	int *ovf_n;        // [8] entries deferred per pass of the sweep (reset by the rebuild)
+	int small_cap;     // members from which the small-cell pass defers a cell (at most its LDS capacity)
 	int2 *pl;          // [pl_cap] the close pairs (slot j, slot q) of the box-trial replicas, this sweep
 	int *pl_n;         // [2] pairs listed, and 1 if the list overflowed
 	int pl_cap;
 };
 
 // launchers (pp_kernels.hip)
+int small_cell_cap();  // members a small-cell pass team stages in LDS
 void launch_fill_i32(int *p, int v, long long n, hipStream_t s);
 void launch_fill_u64(unsigned long long *p, unsigned long long v, long long n, hipStream_t s);
 // regroup slots by cell (mode REB_*); REB_SHIFT draws the grid shift of

@@ -356,6 +356,7 @@ __global__ void k_pass_prefix(DevState d) {
 	const int t = threadIdx.x;
 	if(t == 0 && d.ddq) *d.ddq = 0;  // the dependency-driven sweep's tickets
 	if(t < 8) {
+		d.ovf_n[t] = 0;  // the small-cell pass's deferred cells
 		int acc = 0;
 		d.pfx[t * (d.R + 1)] = 0;
 		for(int r = 0; r < d.R; r++) {
@@ -481,30 +482,41 @@ struct PassArgs {
 	unsigned mark;  // dependency-driven sweeps: the value a finished cell publishes in done[]
 };
 
-// Per-team LDS (doubles): the stage row of a trial's O' and patches, the
-// cell's trials, its members ranked by particle index (x, y, z, O and the 4
-// patch vectors; the kept form holds the member's fractional cell
-// coordinates in the first three patch slots), their particle indices, their
-// kept static bond counts and the cell's record (surrounding slot ranges).
+// Per-team LDS (doubles) for cells of up to CAP members: the stage row of a
+// trial's O' and patches, the cell's trials, its members ranked by particle
+// index (x, y, z, O and the 4 patch vectors; the kept form holds the
+// member's fractional cell coordinates in the first three patch slots),
+// their particle indices, their kept static bond counts and the cell's
+// record (surrounding slot ranges).
 constexpr int OWN_W = NCOMP + 12;   // x, y, z, O[9], P[12]
-constexpr int LDS_OWN = CELL_CAP * OWN_W;
-constexpr int LDS_PID = (CELL_CAP + 1) / 2;  // CELL_CAP ints
-constexpr int LDS_SB = (CELL_CAP + 1) / 2;   // CELL_CAP ints: each member's bonds with the static neighbours
-constexpr int LDS_TRIAL = CELL_CAP * TRIAL_W;
-constexpr int LDS_NB = REC_W / 2;      // the cell record, REC_W ints
-constexpr int LDS_STAGE = 22;  // a trial's O' (9) and patches (12), formed one value per lane
-constexpr int LDS_FIXED = LDS_OWN + LDS_PID + LDS_SB + LDS_TRIAL + LDS_NB + LDS_STAGE;
-// + the orientations of the T register-resident neighbours, component-major
-template <int T>
+constexpr int LDS_NB = REC_W / 2;   // the cell record, REC_W ints
+constexpr int LDS_STAGE = 22;       // a trial's O' (9) and patches (12), formed one value per lane
+template <int CAP>
+struct TeamLds {
+	static constexpr int OWN = CAP * OWN_W;
+	static constexpr int PID = (CAP + 1) / 2;  // CAP ints
+	static constexpr int SB = (CAP + 1) / 2;   // CAP ints: each member's bonds with the static neighbours
+	// a trial per member, or a fixed number per cell
+	static constexpr int TRIAL = (CAP > MAX_TRIALS_PER_CELL ? CAP : MAX_TRIALS_PER_CELL) * TRIAL_W;
+	static constexpr int FIXED = OWN + PID + SB + TRIAL + LDS_NB + LDS_STAGE;
+	static_assert((LDS_STAGE + TRIAL + OWN + PID + SB) % 2 == 0 && FIXED % 2 == 0 && (LDS_STAGE + TRIAL) % 2 == 0 &&
+	                  TRIAL_W % 2 == 0 && OWN_W % 2 == 0,
+	              "16-byte aligned trials, members, record and team blocks");
+};
+// Cells of the small-cell pass (dilute systems, 16-lane teams with one
+// register slot per lane): about half the LDS of a general team, so twice
+// the cells stay resident; fuller cells are deferred to the general pass.
+constexpr int SMALL_CAP = 8;
+// + the orientations of the register neighbours, component-major (two
+// slots per lane for 16-lane teams, one per lane for the small-cell pass)
+template <int T, int CAP = CELL_CAP>
 constexpr int team_lds() {
-	return LDS_FIXED + 9 * (T < 32 ? 32 : T);
+	return TeamLds<CAP>::FIXED + 9 * (CAP < CELL_CAP ? T : (T < 32 ? 32 : T));
 }
-static_assert((LDS_STAGE + LDS_TRIAL + LDS_OWN + LDS_PID + LDS_SB) % 2 == 0 && LDS_FIXED % 2 == 0 &&
-                  (LDS_STAGE + LDS_TRIAL) % 2 == 0 && TRIAL_W % 2 == 0 && OWN_W % 2 == 0,
-              "16-byte aligned trials, members, record and team blocks");
 // 16 teams of 32 lanes (four 128-lane workgroups, the default shape) share a
-// CU's 160 KiB
+// CU's 160 KiB; 32 small-cell teams of 16 lanes do
 static_assert(2 * 8 * team_lds<32>() * sizeof(double) <= 160 * 1024, "two pass workgroups per CU");
+static_assert(4 * 8 * team_lds<16, SMALL_CAP>() * sizeof(double) <= 160 * 1024, "four small-cell workgroups per CU");
 
 // The same decisions through the filtered tests (no square root, no
 // divisions, no branch per patch).  A pair inside a filter band sets `band`;
@@ -645,22 +657,26 @@ __device__ __forceinline__ void wait_neighbours(const DevState &d, const Rep &rp
 // earlier passes, reads their data with agent-scope loads, and finally
 // writes its members back with agent-scope stores, waits for them to
 // complete, and publishes `mark` for its own cell.
-template <int T, int CPL, bool KEEP, bool ONE, bool DD, bool REACH = false>
+//
+// CAP < CELL_CAP (the small-cell pass): a cell of more than CAP members is
+// not worked on here but listed for the general pass that follows.
+template <int T, int CPL, bool KEEP, bool ONE, bool DD, bool REACH = false, int CAP = CELL_CAP>
 __device__ __forceinline__ void pass_cell(const DevState &d, uint64_t sweep, int pass, int gi, int lane,
                                           double *blk, unsigned mark) {
+	using L = TeamLds<CAP>;
 	// the team's block, the per-trial data first (short LDS offsets)
 	double *stage = blk;                    // LDS_STAGE doubles
-	double *trials = blk + LDS_STAGE;       // LDS_TRIAL
-	double *own = trials + LDS_TRIAL;       // LDS_OWN
-	int *own_pid = (int *) (own + LDS_OWN);
-	int *sbond = (int *) (own + LDS_OWN + LDS_PID);  // member m's bonds with the static neighbours
-	int *nb_tab = (int *) (own + LDS_OWN + LDS_PID + LDS_SB);  // the cell's record (build_record)
+	double *trials = blk + LDS_STAGE;       // L::TRIAL
+	double *own = trials + L::TRIAL;        // L::OWN
+	int *own_pid = (int *) (own + L::OWN);
+	int *sbond = (int *) (own + L::OWN + L::PID);  // member m's bonds with the static neighbours
+	int *nb_tab = (int *) (own + L::OWN + L::PID + L::SB);  // the cell's record (build_record)
 	// the orientations of this lane's register neighbours: in registers where
 	// they allow (scalar replica constants, and one slot per lane or a reach
 	// filter that leaves the second slot almost always empty), else in LDS,
 	// component c of neighbour u at (9 u + c) * T + lane
 	constexpr bool REG_ORI = (ONE || T == 64) && (CPL == 1 || REACH);
-	double *ori = blk + LDS_FIXED;
+	double *ori = blk + L::FIXED;
 	const int S = d.stride;
 	{
 		// 1. the cell's record: surrounding slot ranges, own range, replica
@@ -673,6 +689,12 @@ __device__ __forceinline__ void pass_cell(const DevState &d, uint64_t sweep, int
 		const int cloc = uni<T>(nb_tab[61]);
 		const int r = ONE ? 0 : uni<T>(nb_tab[62]);
 		const int sel = uni<T>(nb_tab[63]);
+		if constexpr(CAP < CELL_CAP)
+			if(cnt > d.small_cap) {
+				// (entries < total_act: each cell is listed at most once a pass)
+				if(lane == 0) d.ovf[atomicAdd(d.ovf_n + pass, 1)] = gi;
+				return;
+			}
 		// (a team of T < CELL_CAP lanes holds one member per lane)
 		if(cnt > CELL_CAP || cnt > T) {
 			if(lane == 0) {
@@ -1314,7 +1336,7 @@ __device__ __forceinline__ void pass_cell(const DevState &d, uint64_t sweep, int
 // list is in cell order, so XCD x takes the x-th eighth of the entries
 // actually listed (a slab of the box, whose shared neighbour reads its L2
 // then serves); the surplus workgroups of each XCD leave at once.
-template <int T, int CPL, bool KEEP, bool ONE>
+template <int T, int CPL, bool KEEP, bool ONE, int CAP = CELL_CAP>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_pass(PassArgs a) {
 	extern __shared__ double lds[];
 	const int lane = threadIdx.x & (T - 1);
@@ -1327,8 +1349,22 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
 	const int gi = uni<T>(((blockIdx.x & 7) * per_xcd + i) * tpb + team);
 	if(gi >= total) return;
 	// 16-lane teams with two slots per lane keep the reachable neighbours only
-	pass_cell<T, CPL, KEEP, ONE, false, T == 16 && CPL == 2 && KEEP>(a.d, a.sweep, a.pass, gi, lane,
-	                                                                  lds + team * team_lds<T>(), 0u);
+	pass_cell<T, CPL, KEEP, ONE, false, T == 16 && CPL == 2 && KEEP, CAP>(a.d, a.sweep, a.pass, gi, lane,
+	                                                                       lds + team * team_lds<T, CAP>(), 0u);
+}
+
+// The cells the small-cell pass deferred (more than SMALL_CAP members), by
+// the general 32-lane form: teams take the listed entries in turn.
+template <bool KEEP, bool ONE>
+__global__ __launch_bounds__(128) void k_pass_ovf(PassArgs a) {
+	extern __shared__ double lds[];
+	constexpr int T = 32;
+	const int lane = threadIdx.x & (T - 1);
+	const int team = threadIdx.x / T;
+	const int tpb = blockDim.x / T;
+	const int n = a.d.ovf_n[a.pass];
+	for(int e = blockIdx.x * tpb + team; e < n; e += gridDim.x * tpb)
+		pass_cell<T, 1, KEEP, ONE, false>(a.d, a.sweep, a.pass, a.d.ovf[e], lane, lds + team * team_lds<T>(), 0u);
 }
 
 // The 8 colour passes of a sweep in one launch, driven by dependencies
@@ -1445,15 +1481,23 @@ void launch_passes_dd(const DevState &d, uint64_t sweep, unsigned mark, int T, i
 	}
 }
 
-template <int T, int CPL, bool KEEP, bool ONE>
+template <int T, int CPL, bool KEEP, bool ONE, int CAP = CELL_CAP>
 static void pass_T(const DevState &d, uint64_t sweep, int pass, int block, hipStream_t s) {
 	const int tpb = block / T;
 	int nb = (d.total_act + tpb - 1) / tpb;
 	nb = (nb + 7) & ~7;  // 8 XCDs x the most workgroups one XCD may need
-	size_t lds = (size_t) tpb * team_lds<T>() * sizeof(double);
+	size_t lds = (size_t) tpb * team_lds<T, CAP>() * sizeof(double);
 	PassArgs a{d, sweep, pass, 0u};
-	hipLaunchKernelGGL((k_pass<T, CPL, KEEP, ONE>), dim3(nb), dim3(block), lds, s, a);
+	hipLaunchKernelGGL((k_pass<T, CPL, KEEP, ONE, CAP>), dim3(nb), dim3(block), lds, s, a);
+	if constexpr(CAP < CELL_CAP) {
+		// the deferred cells (usually none: the workgroups leave at once)
+		constexpr int OB = 128;
+		hipLaunchKernelGGL((k_pass_ovf<KEEP, ONE>), dim3(256), dim3(OB), (size_t) (OB / 32) * team_lds<32>() * sizeof(double),
+		                   s, a);
+	}
 }
+
+int small_cell_cap() { return SMALL_CAP; }
 
 size_t pass_lds_bytes(int T, int block) {
 	return (size_t) (block / T) * (T == 16 ? team_lds<16>() : (T == 32 ? team_lds<32>() : team_lds<64>())) *
@@ -1470,8 +1514,8 @@ static void pass_TK(const DevState &d, uint64_t sweep, int pass, int block, hipS
 		// average): one register neighbour per lane, the rare rest from memory
 		const double cand = 27. * (double) d.Ntot / (double) std::max(d.C_total, 1);
 		if(cand < 16.) {
-			if(d.R == 1) pass_T<16, 1, KEEP, true>(d, sweep, pass, block, s);
-			else pass_T<16, 1, KEEP, false>(d, sweep, pass, block, s);
+			if(d.R == 1) pass_T<16, 1, KEEP, true, SMALL_CAP>(d, sweep, pass, block, s);
+			else pass_T<16, 1, KEEP, false, SMALL_CAP>(d, sweep, pass, block, s);
 			return;
 		}
 	}

@@ -242,6 +242,39 @@ def test_dense_neighbourhood_beyond_registers(pp, oracle, tpc, lattice, trials):
         assert np.array_equal(x, y)
 
 
+@pytest.mark.parametrize("trials", [0, 16])
+def test_small_cell_pass_defers_full_cells(pp, oracle, monkeypatch, trials):
+    """Dilute systems (fewer than 16 candidates in a cell's 27 cells) take the
+    small-cell pass with 16-lane teams: half the LDS of a general team, and a
+    cell of more than 8 members is deferred to the general 32-lane pass that
+    follows.  With the threshold forced down to one member
+    (PP_SMALL_CELL_CAP=1), every cell of two or more members takes the
+    deferred path.  N=3000 at rho=0.15: the trajectory is the same bit for bit
+    with the threshold at 8, at 1 and with 32-lane teams, and the final bonds
+    equal the oracle's."""
+    r, O, box = _start(oracle, 3000, 0.15, 12)
+
+    def run(tpc, cap):
+        if cap:
+            monkeypatch.setenv("PP_SMALL_CELL_CAP", str(cap))
+        else:
+            monkeypatch.delenv("PP_SMALL_CELL_CAP", raising=False)
+        with pp.System(r, O, box, T=0.12, seed=31, threads_per_cell=tpc, trials_per_cell=trials) as s:
+            s.do_ensemble(30)
+            return s.state(), s.counters(), s.energy()
+
+    ref = run(32, 0)
+    for tpc, cap in ((16, 0), (16, 1)):
+        got = run(tpc, cap)
+        for x, y in zip(ref[0], got[0]):
+            assert np.array_equal(x, y), (tpc, cap)
+        assert got[1] == ref[1], (tpc, cap)
+    rr, OO, bb = ref[0]
+    b, ov, tot, anyo = ref[2]
+    ob, oov, otot = oracle.energy(rr, OO, bb)
+    assert not anyo and np.array_equal(b, ob) and tot == otot == ref[1]["bonds"]
+
+
 @pytest.mark.parametrize("trials_per_cell", [1, 3])
 def test_fixed_trials_per_cell(pp, oracle, trials_per_cell):
     """s trials per non-empty cell and pass: the tracked energy equals a full
