@@ -96,8 +96,9 @@ struct pp_ctx {
 	std::vector<void *> allocs;
 	// pinned: (sequence number << 4) | the trial-energy paths the box attempt
 	// in flight needs (TrialNeed bits), written by the device (k_vol_publish)
-	int *h_vslow = nullptr;
+	int *h_vslow = nullptr;       // [2] pinned: an attempt's trial paths; where a run-ahead sequence stopped
 	int *d_vslow_host = nullptr;  // its device address
+	bool vol_run_ahead = true;    // box attempts enqueued on the close-pair list, one host wait per step
 	unsigned vol_seq = 0;
 	// asynchronous snapshots (pp_snapshot_start / _finish): a device staging
 	// area and pinned host copies of one replica's slots, copied on their own
@@ -491,7 +492,7 @@ pp_status create_impl(int R, const pp_params *prm, const pp_options *opts, const
 	   (st = dev_alloc(c, &d.rec, (size_t) 8 * std::max(d.total_act, 1) * REC_W)) ||
 	   (st = dev_alloc(c, &d.ctr, (size_t) R * NCTR * NSLOT)) || (st = dev_alloc(c, &d.err, 1)) ||
 	   (st = dev_alloc(c, &d.out_bonds, d.stride)) || (st = dev_alloc(c, &d.out_ovl, d.stride)) ||
-	   (st = dev_alloc(c, &d.bar, 2)) || (st = dev_alloc(c, &d.vflag, R)) || (st = dev_alloc(c, &d.vslow, 1)) ||
+	   (st = dev_alloc(c, &d.bar, 2)) || (st = dev_alloc(c, &d.vflag, R)) || (st = dev_alloc(c, &d.vslow, 2)) ||
 	   (st = dev_alloc(c, &d.done, d.C_total + 1)) || (st = dev_alloc(c, &d.ddq, 1)) ||
 	   (st = dev_alloc(c, &d.ccheck, 2)) || (st = dev_alloc(c, &d.ovf, std::max(d.total_act, 1))) ||
 	   (st = dev_alloc(c, &d.ovf_n, 8)))
@@ -503,11 +504,12 @@ pp_status create_impl(int R, const pp_params *prm, const pp_options *opts, const
 		d.pl_cap = (int) std::min<long long>(8LL * d.stride, 0x7fffff00LL);
 		if((st = dev_alloc(c, &d.pl, (size_t) d.pl_cap)) || (st = dev_alloc(c, &d.pl_n, 2))) return bail(st);
 	}
-	if(hipHostMalloc((void **) &c->h_vslow, sizeof(int), hipHostMallocDefault) != hipSuccess) c->h_vslow = nullptr;
+	if(hipHostMalloc((void **) &c->h_vslow, 2 * sizeof(int), hipHostMallocDefault) != hipSuccess) c->h_vslow = nullptr;
 	if(!c->h_vslow || hipHostGetDevicePointer((void **) &c->d_vslow_host, c->h_vslow, 0) != hipSuccess ||
 	   !c->d_vslow_host)
 		return bail(fail(PP_ERR_NOMEM, "pinned host flag allocation failed"));
-	*c->h_vslow = 0;
+	c->h_vslow[0] = 0;
+	c->h_vslow[1] = 0;
 	// diagnostic switches for the box trials' energy, which must give the
 	// same trajectories: PP_NPT_FULL_TRIAL=1 regroups the particles into the
 	// spare buffer for every trial; PP_NPT_PARTICLE_TRIAL=1 evaluates fast
@@ -520,6 +522,9 @@ pp_status create_impl(int R, const pp_params *prm, const pp_options *opts, const
 			return e && *e && strcmp(e, "0") != 0;
 		};
 		d.npt_trial_form = on("PP_NPT_FULL_TRIAL") ? 2 : (on("PP_NPT_PARTICLE_TRIAL") ? 1 : (on("PP_NPT_RECORD_TRIAL") ? 3 : 0));
+		// PP_NPT_STEPWISE=1: the host reads every box attempt's trial paths
+		// before enqueueing its energy (no run-ahead; the same trajectories)
+		c->vol_run_ahead = d.npt_trial_form == 0 && !on("PP_NPT_STEPWISE");
 		// PP_SMALL_CELL_CAP=k (1 to the small-cell pass's capacity): defer
 		// cells of more than k members to the general pass (a diagnostic
 		// that must give the same trajectories)
@@ -568,6 +573,7 @@ pp_status create_impl(int R, const pp_params *prm, const pp_options *opts, const
 	launch_fill_i32((int *) d.bar, 0, 2, c->stream);
 	launch_fill_i32(d.vflag, 0, R, c->stream);
 	launch_fill_i32(d.vslow, 0, 1, c->stream);
+	launch_fill_i32(d.vslow + 1, -1, 1, c->stream);
 	launch_fill_i32((int *) d.done, 0, d.C_total + 1, c->stream);
 	launch_fill_i32(d.ddq, 0, 1, c->stream);
 	launch_fill_i32(d.ovf_n, 0, 8, c->stream);
@@ -633,25 +639,27 @@ void colour_check_now(pp_ctx *c) {
 	c->colour_checks += c->R;
 }
 
-// Wait for the device to publish attempt `seq`'s trial paths in the pinned
-// word (k_vol_publish): a spin on host memory, with the stream polled now and
-// then so that a failed launch is reported instead of waited for.
-pp_status wait_published(pp_ctx *c, unsigned seq, int *need) {
+// Wait for the device to publish `seq` in pinned word `w` (k_vol_publish:
+// an attempt's trial paths in the low 4 bits; k_vol_report: where a
+// run-ahead sequence stopped in the low 8): a spin on host memory, with the
+// stream polled now and then so that a failed launch is reported instead of
+// waited for.
+pp_status wait_published(pp_ctx *c, int w, int bits, unsigned seq, int *payload) {
 	for(long long spin = 0;; spin++) {
 		// (after the first few thousand polls, about 0.1 ms, the thread
 		// yields between polls: a long pass ahead of the attempt need not
 		// hold a core)
 		if(spin > 4096) std::this_thread::yield();
-		const int v = __atomic_load_n(c->h_vslow, __ATOMIC_ACQUIRE);
-		if(((unsigned) v >> 4) == seq) {
-			*need = v & 15;
+		const int v = __atomic_load_n(c->h_vslow + w, __ATOMIC_ACQUIRE);
+		if(((unsigned) v >> bits) == seq) {
+			*payload = v & ((1 << bits) - 1);
 			return PP_OK;
 		}
 		if((spin & 1023) == 1023) {
 			const hipError_t q = hipStreamQuery(c->stream);
 			if(q != hipSuccess && q != hipErrorNotReady) return fail(PP_ERR_HIP, "HIP error: %s", hipGetErrorString(q));
-			if(q == hipSuccess && ((unsigned) __atomic_load_n(c->h_vslow, __ATOMIC_ACQUIRE) >> 4) != seq)
-				return fail(PP_ERR_HIP, "the box attempt's trial paths were not published");
+			if(q == hipSuccess && ((unsigned) __atomic_load_n(c->h_vslow + w, __ATOMIC_ACQUIRE) >> bits) != seq)
+				return fail(PP_ERR_HIP, "the box attempts' trial paths were not published");
 		}
 		__builtin_ia32_pause();
 	}
@@ -713,10 +721,35 @@ pp_status enqueue_sweep(pp_ctx *c) {
 		const int nvol = c->d.trials_per_cell > 0 ? c->d.trials_per_cell : 1;
 		// the close pairs of this sweep's state, for the trials it serves
 		if(c->d.pl) launch_pairs_build(c->d, c->cus, c->stream);
-		for(int k = 0; k < nvol; k++) {
+		int k0 = 0;
+		bool resume = false;
+		if(c->vol_run_ahead && c->d.pl) {
+			// every attempt enqueued on the close-pair list, which serves
+			// every fast trial in the common case; the host then learns once
+			// where, if anywhere, an attempt needed more (k_vol_gate), and
+			// goes on from that attempt's standing proposal as below
+			for(int k = 0; k < nvol; k++) {
+				hipEvent_t v{};
+				tic(c, TK_VOLUME, &v);
+				launch_vol_propose(c->d, sw, k, c->stream);
+				launch_vol_gate(c->d, k, c->stream);
+				launch_energy_rows_zero(c->d, c->stream);
+				launch_energy_pairs(c->d, c->cus, c->stream);
+				launch_vol_decide(c->d, sw, k, c->stream);
+				launch_vol_apply(c->d, c->stream);
+				toc(c, TK_VOLUME, v);
+			}
+			const unsigned seq = (++c->vol_seq) & 0x7FFFFFu;
+			launch_vol_report(c->d, c->d_vslow_host + 1, seq, c->stream);
+			int stop = 0;
+			TRY(wait_published(c, 1, 8, seq, &stop));
+			k0 = stop > 0 ? stop - 1 : nvol;
+			resume = stop > 0;
+		}
+		for(int k = k0; k < nvol; k++) {
 			hipEvent_t v{};
 			tic(c, TK_VOLUME, &v);
-			launch_vol_propose(c->d, sw, k, c->stream);
+			if(!(resume && k == k0)) launch_vol_propose(c->d, sw, k, c->stream);
 			// a trial rebuild only when some replica's trial box cannot be
 			// served by its current lists (MC.c:481-485 rebuilds only when
 			// the cells become too small); then the trial energies, per
@@ -724,7 +757,7 @@ pp_status enqueue_sweep(pp_ctx *c) {
 			const unsigned seq = (++c->vol_seq) & 0x7FFFFFFu;
 			launch_vol_publish(c->d, c->d_vslow_host, seq, c->stream);
 			int need = 0;
-			TRY(wait_published(c, seq, &need));
+			TRY(wait_published(c, 0, 4, seq, &need));
 			if(need & NEED_REBUILD) launch_rebuild(c->d, REB_TRIAL, sw, -1, c->stream);
 			if(need & NEED_PARTICLES) launch_energy(c->d, 1, 0, c->stream);
 			else launch_energy_rows_zero(c->d, c->stream);

@@ -227,7 +227,8 @@ struct DevState {
 	int trace_cap;
 	uint64_t seed;
 	int *vflag;        // [R] a trial energy found an overlap: the replica's other teams stop
-	int *vslow;        // [1] some replica's trial needs a trial rebuild
+	int *vslow;        // [2] the trial paths the attempt's replicas need (TrialNeed bits); the attempt
+	                   // at which a run-ahead sequence of attempts stopped, -1 while none did
 	// how box trials find their energy: 0 the fastest path available (the
 	// close-pair list, else the cell records, else the current lists per
 	// particle, else a trial rebuild); 3 never the pair list; 1 neither the
@@ -296,6 +297,11 @@ void launch_colour_conflicts(const DevState &d, int rep, int slots, const int *n
                              hipStream_t s);
 // zero the energy rows (C_EBONDS, C_OVL) of every replica
 void launch_energy_rows_zero(const DevState &d, hipStream_t s);
+// run-ahead box attempts: stop the sequence at attempt k when it needs more
+// than the close-pair list; report the stop (attempt + 1, 0 for none) with
+// `seq` into the host's pinned word and clear it
+void launch_vol_gate(const DevState &d, int k, hipStream_t s);
+void launch_vol_report(const DevState &d, int *host_word, unsigned seq, hipStream_t s);
 // the trial-energy paths a box attempt needs, as the host reads them from
 // vslow after k_vol_propose: a trial rebuild, the per-particle trial energy,
 // the per-record trial energy

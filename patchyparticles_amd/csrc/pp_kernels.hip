@@ -47,8 +47,10 @@ void launch_fill_u64(unsigned long long *p, unsigned long long v, long long n, h
 	hipLaunchKernelGGL(k_fill_u64, dim3((unsigned) ((n + 255) / 256)), dim3(256), 0, s, p, v, n);
 }
 
-// Zero the energy-kernel rows (C_EBONDS, C_OVL) of every replica.
-__global__ void k_zero_energy_rows(unsigned long long *ctr, int R) {
+// Zero the energy-kernel rows (C_EBONDS, C_OVL) of every replica (nothing
+// once a run-ahead sequence of box attempts has stopped: *stop >= 0).
+__global__ void k_zero_energy_rows(unsigned long long *ctr, int R, const int *stop) {
+	if(stop && *stop >= 0) return;
 	int i = blockIdx.x * blockDim.x + threadIdx.x;
 	if(i >= R * 2 * NSLOT) return;
 	int r = i / (2 * NSLOT), k = i % (2 * NSLOT);
@@ -356,7 +358,7 @@ __global__ void k_pass_prefix(DevState d) {
 	const int t = threadIdx.x;
 	if(t == 0 && d.ddq) *d.ddq = 0;  // the dependency-driven sweep's tickets
 	if(t < 8) {
-		d.ovf_n[t] = 0;  // the small-cell pass's deferred cells
+		if(d.ovf_n) d.ovf_n[t] = 0;  // the small-cell pass's deferred cells
 		int acc = 0;
 		d.pfx[t * (d.R + 1)] = 0;
 		for(int r = 0; r < d.R; r++) {
@@ -1753,13 +1755,14 @@ void launch_bond_list(const DevState &d, int max_per, int *count, int *partners,
 		hipLaunchKernelGGL(k_bond_list, dim3((d.Ntot + 255) / 256), dim3(256), 0, s, d, max_per, count, partners);
 }
 
+void launch_energy_rows_zero(const DevState &d, hipStream_t s) {
+	hipLaunchKernelGGL(k_zero_energy_rows, dim3((d.R * 2 * NSLOT + 255) / 256), dim3(256), 0, s, d.ctr, d.R,
+	                   (const int *) (d.vslow ? d.vslow + 1 : nullptr));
+}
+
 // Lanes per particle: 32 where the 27 cells hold many candidates, 8 in dilute
 // systems (fewer than 8 candidates on average over the cell capacity), where
 // most of a 32-lane team would idle.  The result does not depend on it.
-void launch_energy_rows_zero(const DevState &d, hipStream_t s) {
-	hipLaunchKernelGGL(k_zero_energy_rows, dim3((d.R * 2 * NSLOT + 255) / 256), dim3(256), 0, s, d.ctr, d.R);
-}
-
 void launch_energy(const DevState &d, int trial, int per_particle, hipStream_t s) {
 	launch_energy_rows_zero(d, s);
 	if(d.Ntot <= 0) return;
@@ -2003,6 +2006,7 @@ void launch_pairs_build(const DevState &d, int cus, hipStream_t s) {
 // separation and the bond decision as k_energy_rec forms them; a bonded pair
 // adds 2 to the bond rows, an overlapping pair 1 to the overlap row.
 __global__ __launch_bounds__(256) void k_energy_pairs(DevState d) {
+	if(d.vslow[1] >= 0) return;  // a run-ahead sequence of attempts stopped
 	const int n = min(d.pl_n[0], d.pl_cap);
 	const int stride = gridDim.x * blockDim.x;
 	for(int k = xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x; k < n; k += stride) {
@@ -2138,7 +2142,7 @@ __device__ __forceinline__ void choose_trial_path(const DevState &d, Rep &rp, co
 
 __global__ void k_vol_propose(DevState d, uint64_t sweep, int k) {
 	int r = blockIdx.x * blockDim.x + threadIdx.x;
-	if(r >= d.R) return;
+	if(r >= d.R || d.vslow[1] >= 0) return;
 	Rep &rp = d.reps[r];
 	rp.trial_active = 0;
 	rp.trial_fast = 0;
@@ -2220,13 +2224,43 @@ void launch_vol_publish(const DevState &d, int *host_word, unsigned seq, hipStre
 	hipLaunchKernelGGL(k_vol_publish, dim3(1), dim3(64), 0, s, d, host_word, seq);
 }
 
+// Run-ahead box attempts (enqueued without the host reading each attempt's
+// paths): attempt k proceeds on the close-pair list alone, which serves
+// every fast trial in the common case; if some replica's trial needs more,
+// the sequence stops here (vslow[1] = k), and the kernels of this and the
+// later attempts leave at once: the attempt's proposal stays in place and
+// its paths stay in vslow[0] for the host to take over from.
+__global__ void k_vol_gate(DevState d, int k) {
+	if(threadIdx.x == 0 && d.vslow[1] < 0 && (d.vslow[0] & ~(int) NEED_PAIRS)) d.vslow[1] = k;
+}
+
+void launch_vol_gate(const DevState &d, int k, hipStream_t s) {
+	hipLaunchKernelGGL(k_vol_gate, dim3(1), dim3(64), 0, s, d, k);
+}
+
+// The end of a run-ahead sequence: where it stopped (attempt + 1, 0 if it
+// ran through) with the host's sequence number into the pinned word; the
+// stop is cleared for the attempts that follow.
+__global__ void k_vol_report(DevState d, int *host_word, unsigned seq) {
+	if(threadIdx.x == 0) {
+		const int stop = d.vslow[1];
+		d.vslow[1] = -1;
+		__hip_atomic_store(host_word, (int) ((seq << 8) | (unsigned) (stop + 1)), __ATOMIC_RELAXED,
+		                   __HIP_MEMORY_SCOPE_SYSTEM);
+	}
+}
+
+void launch_vol_report(const DevState &d, int *host_word, unsigned seq, hipStream_t s) {
+	hipLaunchKernelGGL(k_vol_report, dim3(1), dim3(64), 0, s, d, host_word, seq);
+}
+
 void launch_vol_propose(const DevState &d, uint64_t sweep, int k, hipStream_t s) {
 	hipLaunchKernelGGL(k_vol_propose, dim3((d.R + 63) / 64), dim3(64), 0, s, d, sweep, k);
 }
 
 __global__ void k_vol_decide(DevState d, uint64_t sweep, int k) {
 	int r = blockIdx.x * blockDim.x + threadIdx.x;
-	if(r >= d.R) return;
+	if(r >= d.R || d.vslow[1] >= 0) return;
 	Rep &rp = d.reps[r];
 	if(!rp.trial_active) return;
 	unsigned long long eb = 0, ov = 0, db = 0;
@@ -2280,10 +2314,11 @@ void launch_vol_decide(const DevState &d, uint64_t sweep, int k, hipStream_t s) 
 }
 
 // The accepted fast trials: every position component times its trial scale,
-// as the trial rebuild writes it (x * 1 is x).  Clears the slow-trial flag
-// for the next attempt.
+// as the trial rebuild writes it (x * 1 is x).  Clears the trial paths for
+// the next attempt.
 __global__ void k_vol_apply(DevState d) {
 	const int j = blockIdx.x * blockDim.x + threadIdx.x;
+	if(d.vslow[1] >= 0) return;  // a run-ahead sequence of attempts stopped
 	if(j == 0) *d.vslow = 0;
 	if(j >= d.Ntot) return;
 	const Rep &rp = d.reps[d.prep[j]];

@@ -1,9 +1,12 @@
 """Box trials evaluated on the current cell lists (no trial rebuild while
 the lists' cells, rescaled with the trial, stay at least 1 + delta wide),
-with their energy taken from the sweep's close-pair list (the default), pair
-by pair from the cell records (PP_NPT_RECORD_TRIAL=1) or per particle
-(PP_NPT_PARTICLE_TRIAL=1), against the form that regroups every trial into
-the spare buffer (PP_NPT_FULL_TRIAL=1): the same trajectories bit for bit --
+with their energy taken from the sweep's close-pair list (the default: the
+step's attempts enqueued ahead on that list, the host taking over from the
+first attempt that needs more), from the same list with the host reading
+every attempt's paths first (PP_NPT_STEPWISE=1), pair by pair from the cell
+records (PP_NPT_RECORD_TRIAL=1) or per particle (PP_NPT_PARTICLE_TRIAL=1),
+against the form that regroups every trial into the spare buffer
+(PP_NPT_FULL_TRIAL=1): the same trajectories bit for bit --
 positions,
 orientations, boxes, volumes and move counters -- in NPT (MC.c:450-513) and
 with box-shape moves (MC.c:515-585), for one system and for a batch that
@@ -26,6 +29,7 @@ def _run(pp, monkeypatch, form, make, steps, lx=None):
     monkeypatch.setenv("PP_NPT_FULL_TRIAL", "1" if form == "full" else "0")
     monkeypatch.setenv("PP_NPT_PARTICLE_TRIAL", "1" if form == "particles" else "0")
     monkeypatch.setenv("PP_NPT_RECORD_TRIAL", "1" if form == "records" else "0")
+    monkeypatch.setenv("PP_NPT_STEPWISE", "1" if form == "stepwise" else "0")
     with make() as s:
         if lx:
             s.set_lx_move(*lx)
@@ -84,6 +88,7 @@ def test_box_trials_on_current_lists_keep_the_trajectory(pp, oracle, monkeypatch
     _same_runs(full, fast)
     _same_runs(full, _run(pp, monkeypatch, "records", make, steps, lx))
     _same_runs(full, _run(pp, monkeypatch, "particles", make, steps, lx))
+    _same_runs(full, _run(pp, monkeypatch, "stepwise", make, steps, lx))
     c = fast[1]
     assert c["accepted"][:, 1].sum() > 0 and c["accepted"][:, 1].sum() < c["tries"][:, 1].sum()
     for k, (rr, OO, bb) in enumerate(fast[0]):
