@@ -97,7 +97,7 @@ def test_check_energy_gram_schmidt_bit_exact(pp, oracle):
 
 @pytest.mark.parametrize("tpc,block,launch,trials", [(32, 64, 1, 0), (32, 128, 2, 0), (32, 64, 2, 0), (64, 64, 2, 0),
                                                      (64, 128, 1, 0), (64, 256, 2, 0), (16, 64, 1, 0), (16, 64, 1, 16),
-                                                     (32, 128, 1, 16)])
+                                                     (16, 64, 1, 4), (32, 128, 1, 16)])
 def test_trajectory_independent_of_launch_shape(pp, oracle, tpc, block, launch, trials):
     """Lanes per cell (16, 32 or 64: 4, 2 or 1 cells per wavefront),
     workgroup size and launch form (one launch per colour pass, or one
@@ -273,6 +273,35 @@ def test_small_cell_pass_defers_full_cells(pp, oracle, monkeypatch, trials):
     b, ov, tot, anyo = ref[2]
     ob, oov, otot = oracle.energy(rr, OO, bb)
     assert not anyo and np.array_equal(b, ob) and tot == otot == ref[1]["bonds"]
+
+
+def test_small_cell_pass_in_a_batch(pp, oracle, monkeypatch):
+    """The small-cell pass of several replicas (replica constants per team,
+    orientations in LDS): four dilute replicas of uneven N and T, 16 trials
+    per cell, with the deferral threshold at 8 and at 1 member, give the
+    trajectories of 32-lane teams bit for bit."""
+    starts = [oracle.generate(n, rho, seed=60 + k) for k, (n, rho) in enumerate([(800, 0.1), (1500, 0.15), (600, 0.05),
+                                                                                  (1200, 0.2)])]
+    prm = [dict(T=0.12), dict(T=0.15), dict(T=0.1), dict(T=0.2)]
+
+    def run(tpc, cap):
+        if cap:
+            monkeypatch.setenv("PP_SMALL_CELL_CAP", str(cap))
+        else:
+            monkeypatch.delenv("PP_SMALL_CELL_CAP", raising=False)
+        with pp.System.batch([x[0] for x in starts], [x[1] for x in starts], [x[2] for x in starts], prm, seed=12,
+                             trials_per_cell=16, threads_per_cell=tpc) as s:
+            s.do_ensemble(25)
+            return [s.state(k) for k in range(s.R)], s.counters_all()
+
+    ref = run(32, 0)
+    for tpc, cap in ((16, 0), (16, 1)):
+        got = run(tpc, cap)
+        for a, b in zip(ref[0], got[0]):
+            for x, y in zip(a, b):
+                assert np.array_equal(x, y), (tpc, cap)
+        for key in ("tries", "accepted", "bonds"):
+            assert np.array_equal(ref[1][key], got[1][key]), (key, tpc, cap)
 
 
 @pytest.mark.parametrize("trials_per_cell", [1, 3])
