@@ -946,17 +946,33 @@ __device__ __forceinline__ void pass_cell(const DevState &d, uint64_t sweep, int
 						});
 					}
 		};
-		auto each_static = [&](auto &&f) {
-#pragma unroll
-			for(int u = 0; u < CPL; u++)
-				if(cqr[u] >= 0) f(cxr[u], cyr[u], czr[u], [&](double *Oq) { reg_o(u, Oq); });
-			each_far(f);
-		};
 		team_sync();
-		// 5. KEEP: every member's bonds with the static neighbours
+		// 5. KEEP: every member's bonds with the static neighbours, and with
+		// each other
 		unsigned mbits = 0;
 		if constexpr(KEEP) {
-			// two members at a time (independent chains), one reduction for both
+			// each member lane's own member in its first register slot (the
+			// trials refresh it as members move)
+			const bool mem_lane = lane < cnt;
+			if(mem_lane) {
+				const double *wl = own + lane * OWN_W;
+				cxr[0] = wl[0];
+				cyr[0] = wl[1];
+				czr[0] = wl[2];
+				if constexpr(REG_ORI)
+#pragma unroll
+					for(int c = 0; c < 9; c++) orv[0][c] = wl[3 + c];
+			}
+			// Two members at a time (independent chains), one reduction for
+			// both.  Each lane tests its register slots against member mm: a
+			// static neighbour adds to the member's kept count; another
+			// member sets a bit of the lane's mbits (bit mm of lane j iff
+			// members j and mm are bonded: the pair decision is symmetric).
+			auto slot_for = [&](int u, int mm) { return u == 0 && mem_lane ? lane != mm : cqr[u] >= 0; };
+			auto add = [&](int u, int b, int &bs, int &bm) {
+				if(u == 0 && mem_lane) bm = b;
+				else bs += b;
+			};
 			for(int m = 0; m < cnt; m += 2) {
 				const bool two = m + 1 < cnt;
 				const double *w0 = own + m * OWN_W, *w1 = own + (two ? m + 1 : m) * OWN_W;
@@ -973,22 +989,53 @@ __device__ __forceinline__ void pass_cell(const DevState &d, uint64_t sweep, int
 					P0[c] = w0[NCOMP + c];
 					P1[c] = w1[NCOMP + c];
 				}
-				int b0 = 0, b1 = 0;
+				int b0 = 0, b1 = 0, mb0 = 0, mb1 = 0;
 				bool band0 = tr.c2 < 0., band1 = band0;
-				if(!band0)
-					each_static([&](double qx, double qy, double qz, auto load_o) {
+				if(!band0) {
+#pragma unroll
+					for(int u = 0; u < CPL; u++) {
+						auto lo = [&](double *Oq) { reg_o(u, Oq); };
+						if(slot_for(u, m)) {
+							int b = 0;
+							bonds_new(t0, P0, cxr[u], cyr[u], czr[u], lo, b, band0);
+							add(u, b, b0, mb0);
+						}
+						if(two && slot_for(u, m + 1)) {
+							int b = 0;
+							bonds_new(t1, P1, cxr[u], cyr[u], czr[u], lo, b, band1);
+							add(u, b, b1, mb1);
+						}
+					}
+					each_far([&](double qx, double qy, double qz, auto load_o) {
 						bonds_new(t0, P0, qx, qy, qz, load_o, b0, band0);
 						if(two) bonds_new(t1, P1, qx, qy, qz, load_o, b1, band1);
 					});
+				}
 				if(__ballot(band0) != 0ull && team_any<T>(band0) && band0) {
 					b0 = 0;
-					each_static([&](double qx, double qy, double qz, auto load_o) {
+					mb0 = 0;
+#pragma unroll
+					for(int u = 0; u < CPL; u++)
+						if(slot_for(u, m)) {
+							int b = 0;
+							bonds_new_exact(t0, P0, cxr[u], cyr[u], czr[u], [&](double *Oq) { reg_o(u, Oq); }, b);
+							add(u, b, b0, mb0);
+						}
+					each_far([&](double qx, double qy, double qz, auto load_o) {
 						bonds_new_exact(t0, P0, qx, qy, qz, load_o, b0);
 					});
 				}
 				if(__ballot(band1) != 0ull && team_any<T>(band1) && band1) {
 					b1 = 0;
-					each_static([&](double qx, double qy, double qz, auto load_o) {
+					mb1 = 0;
+#pragma unroll
+					for(int u = 0; u < CPL; u++)
+						if(slot_for(u, m + 1)) {
+							int b = 0;
+							bonds_new_exact(t1, P1, cxr[u], cyr[u], czr[u], [&](double *Oq) { reg_o(u, Oq); }, b);
+							add(u, b, b1, mb1);
+						}
+					each_far([&](double qx, double qy, double qz, auto load_o) {
 						bonds_new_exact(t1, P1, qx, qy, qz, load_o, b1);
 					});
 				}
@@ -998,35 +1045,7 @@ __device__ __forceinline__ void pass_cell(const DevState &d, uint64_t sweep, int
 					sbond[m] = b & 0xffff;
 					if(two) sbond[m + 1] = b >> 16;
 				}
-			}
-			// and the members' bonds among themselves: lane m < cnt keeps bit
-			// j of mbits set iff members m and j are bonded (exact form; the
-			// trials keep it current, so a trial tests only the new position)
-			if(lane < cnt && cnt > 1) {
-				const double *wm = own + lane * OWN_W;
-				Trial tm = tr;
-				tm.nx = wm[0];
-				tm.ny = wm[1];
-				tm.nz = wm[2];
-				double Pm[12];
-#pragma unroll
-				for(int c = 0; c < 12; c++) Pm[c] = wm[NCOMP + c];
-				for(int j = 0; j < cnt; j++) {
-					if(j == lane) continue;
-					const double *wj = own + j * OWN_W;
-					auto load_j = [&](double *Oq) {
-#pragma unroll
-						for(int c = 0; c < 9; c++) Oq[c] = wj[3 + c];
-					};
-					int b = 0;
-					bool band = tr.c2 < 0.;
-					if(!band) bonds_new(tm, Pm, wj[0], wj[1], wj[2], load_j, b, band);
-					if(band) {
-						b = 0;
-						bonds_new_exact(tm, Pm, wj[0], wj[1], wj[2], load_j, b);
-					}
-					mbits |= (unsigned) b << j;
-				}
+				if(mem_lane) mbits |= ((unsigned) mb0 << m) | (two ? (unsigned) mb1 << (m + 1) : 0u);
 			}
 			team_sync();
 			// The members' fractional coordinates inside the cell, u = (cell
