@@ -91,6 +91,8 @@ __device__ __forceinline__ int replica_of_cell(const DevState &d, int c) {
 // box trial whose grid the current lists cannot serve).
 __device__ __forceinline__ bool trial_rebuilds(const Rep &rp) { return rp.trial_active && !rp.trial_fast; }
 
+// ONE: a context of one replica, whose table then loads to scalar registers.
+template <bool ONE>
 __global__ void k_count(DevState d, int mode, uint64_t sweep, long long trace_pos) {
 	int j = blockIdx.x * blockDim.x + threadIdx.x;
 	// reset the scan's look-back state for the scan that follows
@@ -106,7 +108,7 @@ __global__ void k_count(DevState d, int mode, uint64_t sweep, long long trace_po
 		for(int c = j; c < d.C_total; c += gridDim.x * blockDim.x)
 			if(!trial_rebuilds(d.reps[replica_of_cell(d, c)])) d.count[c] = d.start[c + 1] - d.start[c];
 	if(j >= d.Ntot) return;
-	const int r = d.prep[j];
+	const int r = ONE ? 0 : d.prep[j];
 	Rep &rp = d.reps[r];
 	const bool tr = (mode == REB_TRIAL) && trial_rebuilds(rp);
 	if(mode == REB_TRIAL && !tr) return;
@@ -375,13 +377,15 @@ __global__ void k_pass_prefix(DevState d) {
 // slots [0, 26), the exclusive prefix of their populations [32, 59), the
 // cell's first slot, population, local index, replica and buffer selector
 // [59, 64).  A pass finds all its data from this one load.
+template <bool ONE>
 __device__ __forceinline__ void build_record(const DevState &d, int i) {
 	int lo = 0, hi = d.R;
-	while(hi - lo > 1) {
-		int mid = (lo + hi) >> 1;
-		if(d.reps[mid].cell_off <= i) lo = mid;
-		else hi = mid;
-	}
+	if constexpr(!ONE)
+		while(hi - lo > 1) {
+			int mid = (lo + hi) >> 1;
+			if(d.reps[mid].cell_off <= i) lo = mid;
+			else hi = mid;
+		}
 	const Rep &rp = d.reps[lo];
 	const int local = i - rp.cell_off;
 	const int col = local / rp.act_cap, kk = local - col * rp.act_cap;
@@ -423,11 +427,12 @@ __device__ __forceinline__ void build_record(const DevState &d, int i) {
 // source is the previous buffer (sel ^ 1); a trial writes into sel ^ 1.
 // With a new grid shift the workers also write the cell records of the
 // sweep's passes.
+template <bool ONE>
 __global__ void k_scatter(DevState d, int mode, uint64_t sweep) {
 	int j = blockIdx.x * blockDim.x + threadIdx.x;
-	if(mode == REB_SHIFT && j < d.C_total) build_record(d, j);
+	if(mode == REB_SHIFT && j < d.C_total) build_record<ONE>(d, j);
 	if(j >= d.Ntot) return;
-	const int r = d.prep[j];
+	const int r = ONE ? 0 : d.prep[j];
 	const Rep &rp = d.reps[r];
 	const bool tr = (mode == REB_TRIAL) && trial_rebuilds(rp);
 	if(mode == REB_TRIAL && !tr) return;
@@ -452,7 +457,8 @@ void launch_rebuild(const DevState &d, int mode, uint64_t sweep, long long trace
 	unsigned gp = (unsigned) ((d.Ntot + B - 1) / B);
 	if(gp == 0) gp = 1;
 	const int flip = mode != REB_TRIAL;
-	hipLaunchKernelGGL(k_count, dim3(gp), dim3(B), 0, s, d, mode, sweep, trace_pos);
+	if(d.R == 1) hipLaunchKernelGGL(k_count<true>, dim3(gp), dim3(B), 0, s, d, mode, sweep, trace_pos);
+	else hipLaunchKernelGGL(k_count<false>, dim3(gp), dim3(B), 0, s, d, mode, sweep, trace_pos);
 	const int lists = mode == REB_SHIFT ? 1 : 0;
 	if(d.scan_tile == SCAN_TILE_SMALL) {
 		// at most SCAN_SMALL_MAX_TILES tiles: one pass with a look-back
@@ -471,7 +477,8 @@ void launch_rebuild(const DevState &d, int mode, uint64_t sweep, long long trace
 	if(mode == REB_SHIFT) hipLaunchKernelGGL(k_pass_prefix, dim3(1), dim3(64), 0, s, d);
 	unsigned gs = gp;
 	if(mode == REB_SHIFT) gs = std::max(gp, (unsigned) ((d.C_total + B - 1) / B));
-	hipLaunchKernelGGL(k_scatter, dim3(gs), dim3(B), 0, s, d, mode, sweep);
+	if(d.R == 1) hipLaunchKernelGGL(k_scatter<true>, dim3(gs), dim3(B), 0, s, d, mode, sweep);
+	else hipLaunchKernelGGL(k_scatter<false>, dim3(gs), dim3(B), 0, s, d, mode, sweep);
 }
 
 // ------------------------------------------------------------------------
