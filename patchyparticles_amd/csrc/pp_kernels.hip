@@ -356,8 +356,7 @@ __global__ __launch_bounds__(256) void k_scan(DevState d, int flip, int lists, i
 // pfx[k][r] = first entry of replica r in the list of pass k (one thread per
 // pass; the kernel boundary makes every workgroup's counts visible).  Also
 // resets the ticket queue of the dependency-driven sweep.
-__global__ void k_pass_prefix(DevState d) {
-	const int t = threadIdx.x;
+__device__ __forceinline__ void pass_prefix(const DevState &d, int t) {
 	if(t == 0 && d.ddq) *d.ddq = 0;  // the dependency-driven sweep's tickets
 	if(t < 8) {
 		if(d.ovf_n) d.ovf_n[t] = 0;  // the small-cell pass's deferred cells
@@ -370,6 +369,8 @@ __global__ void k_pass_prefix(DevState d) {
 		}
 	}
 }
+
+__global__ void k_pass_prefix(DevState d) { pass_prefix(d, threadIdx.x); }
 
 // Record of entry i of the cell lists (replica r's list of colour col at
 // clist[cell_off + col * act_cap + k]), stored at the entry's position in the
@@ -393,7 +394,7 @@ __device__ __forceinline__ void build_record(const DevState &d, int i) {
 	const int cloc = d.clist[i];
 	int k = 0;
 	while(k < 7 && (int) ((rp.col_perm >> (4 * k)) & 0xFu) != col) k++;
-	const int gi = d.pfx[k * (d.R + 1) + lo] + kk;
+	const int gi = (ONE ? 0 : d.pfx[k * (d.R + 1) + lo]) + kk;
 	const int n0 = rp.g.n[0], n1 = rp.g.n[1], n2 = rp.g.n[2];
 	const int ix = cloc / (n1 * n2), iy = (cloc / n2) % n1, iz = cloc % n2;
 	const int *st = d.start + rp.cell_off;
@@ -427,9 +428,14 @@ __device__ __forceinline__ void build_record(const DevState &d, int i) {
 // source is the previous buffer (sel ^ 1); a trial writes into sel ^ 1.
 // With a new grid shift the workers also write the cell records of the
 // sweep's passes.
+// ONE: one replica, whose lists start at entry 0 of every pass; the first
+// workgroup also writes the pass prefixes (k_pass_prefix's work), which the
+// records then do not need.
 template <bool ONE>
 __global__ void k_scatter(DevState d, int mode, uint64_t sweep) {
 	int j = blockIdx.x * blockDim.x + threadIdx.x;
+	if constexpr(ONE)
+		if(mode == REB_SHIFT && blockIdx.x == 0) pass_prefix(d, threadIdx.x);
 	if(mode == REB_SHIFT && j < d.C_total) build_record<ONE>(d, j);
 	if(j >= d.Ntot) return;
 	const int r = ONE ? 0 : d.prep[j];
@@ -474,7 +480,7 @@ void launch_rebuild(const DevState &d, int mode, uint64_t sweep, long long trace
 		hipLaunchKernelGGL(k_tile_scan, dim3(1), dim3(1024), 0, s, d);
 		hipLaunchKernelGGL(k_scan<SCAN_TILE_LARGE / 256>, dim3(d.ntiles), dim3(256), 0, s, d, flip, lists, 1);
 	}
-	if(mode == REB_SHIFT) hipLaunchKernelGGL(k_pass_prefix, dim3(1), dim3(64), 0, s, d);
+	if(mode == REB_SHIFT && d.R != 1) hipLaunchKernelGGL(k_pass_prefix, dim3(1), dim3(64), 0, s, d);
 	unsigned gs = gp;
 	if(mode == REB_SHIFT) gs = std::max(gp, (unsigned) ((d.C_total + B - 1) / B));
 	if(d.R == 1) hipLaunchKernelGGL(k_scatter<true>, dim3(gs), dim3(B), 0, s, d, mode, sweep);
