@@ -436,26 +436,41 @@ __global__ void k_scatter(DevState d, int mode, uint64_t sweep) {
 	int j = blockIdx.x * blockDim.x + threadIdx.x;
 	if constexpr(ONE)
 		if(mode == REB_SHIFT && blockIdx.x == 0) pass_prefix(d, threadIdx.x);
-	if(mode == REB_SHIFT && j < d.C_total) build_record<ONE>(d, j);
-	if(j >= d.Ntot) return;
-	const int r = ONE ? 0 : d.prep[j];
-	const Rep &rp = d.reps[r];
-	const bool tr = (mode == REB_TRIAL) && trial_rebuilds(rp);
-	if(mode == REB_TRIAL && !tr) return;
-	// the buffer this rebuild switches to holds the live particles first
-	if(mode != REB_TRIAL && j == rp.p_off) d.reps[r].np_hw = rp.np;
-	const int sb = tr ? rp.sel : (rp.sel ^ 1);
-	const double *F = pick_buf(d.f, sb);
-	double *G = pick_buf(d.f, sb ^ 1);
+	// the particle's loads first, so that they overlap the record's chain of
+	// loads; its stores after the record's
+	bool mv = false, hw = false;
+	int r = 0, dst = 0, pid = 0;
+	double v[NCOMP];
+	double *G = nullptr;
+	int *PG = nullptr;
 	const int S = d.stride;
-	const int dst = d.start[d.cellid[j]] + d.slot[j];
+	if(j < d.Ntot) {
+		r = ONE ? 0 : d.prep[j];
+		const Rep &rp = d.reps[r];
+		const bool tr = (mode == REB_TRIAL) && trial_rebuilds(rp);
+		if(mode != REB_TRIAL || tr) {
+			mv = true;
+			// the buffer this rebuild switches to holds the live particles first
+			hw = mode != REB_TRIAL && j == rp.p_off;
+			const int sb = tr ? rp.sel : (rp.sel ^ 1);
+			const double *F = pick_buf(d.f, sb);
+			G = pick_buf(d.f, sb ^ 1);
+			PG = pick_buf(d.pid, sb ^ 1);
+			dst = d.start[d.cellid[j]] + d.slot[j];
 #pragma unroll
-	for(int c = 0; c < NCOMP; c++) {
-		double v = F[c * S + j];
-		if(tr && c < 3) v *= rp.trial_s[c];
-		G[c * S + dst] = v;
+			for(int c = 0; c < NCOMP; c++) {
+				v[c] = F[c * S + j];
+				if(tr && c < 3) v[c] *= rp.trial_s[c];
+			}
+			pid = pick_buf(d.pid, sb)[j];
+		}
 	}
-	pick_buf(d.pid, sb ^ 1)[dst] = pick_buf(d.pid, sb)[j];
+	if(mode == REB_SHIFT && j < d.C_total) build_record<ONE>(d, j);
+	if(!mv) return;
+#pragma unroll
+	for(int c = 0; c < NCOMP; c++) G[c * S + dst] = v[c];
+	PG[dst] = pid;
+	if(hw) d.reps[r].np_hw = d.reps[r].np;
 }
 
 void launch_rebuild(const DevState &d, int mode, uint64_t sweep, long long trace_pos, hipStream_t s) {
