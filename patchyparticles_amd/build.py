@@ -8,6 +8,11 @@ hipcc flags:
                                  where the reference's C source rounds
   -Xclang -target-feature -Xclang -packed-fp32-ops
                                  keep the compiler from emitting packed FP32 ops
+  -mllvm -amdgpu-sched-strategy=max-ilp   (pp_kernels.hip only)
+                                 schedule for instruction-level parallelism: the
+                                 colour pass's wavefronts run mostly alone on
+                                 their SIMDs, bound by each trial's dependent
+                                 chain (C2 +0.8 %, C5 +1.9 % measured)
 """
 from __future__ import annotations
 
@@ -24,6 +29,7 @@ LIB = os.path.join(PKG, "libpp_hip.so")
 BIN = os.path.join(PKG, "bin")
 
 HIP_SOURCES = ["pp_kernels.hip", "pp_learn.hip", "pp_api.cpp", "pp_net.cpp", "pp_host.cpp", "pp_noalloc.cpp"]
+EXTRA_FLAGS = {"pp_kernels.hip": ["-mllvm", "-amdgpu-sched-strategy=max-ilp"]}
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",
     "-O3",
@@ -55,7 +61,7 @@ def build_lib(verbose=True):
     for src in HIP_SOURCES:
         obj = os.path.join(obj_dir, os.path.splitext(src)[0] + ".o")
         lang = ["-x", "hip"] if src.endswith(".cpp") else []
-        _run([hipcc(), *HIPCC_FLAGS, *lang, "-c", os.path.join(CSRC, src), "-o", obj])
+        _run([hipcc(), *HIPCC_FLAGS, *EXTRA_FLAGS.get(src, []), *lang, "-c", os.path.join(CSRC, src), "-o", obj])
         objs.append(obj)
     tmp = LIB + ".tmp"
     _run([hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", tmp])
