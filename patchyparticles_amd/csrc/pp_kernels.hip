@@ -208,7 +208,8 @@ __global__ __launch_bounds__(1024) void k_tile_scan(DevState d) {
 // pre = 1: the tiles' exclusive prefixes are in d.tile_sum (k_tile_sums,
 // k_tile_scan); pre = 0: a decoupled look-back finds them.
 // PER cells per lane: tiles of 256 * PER cells.
-template <int PER>
+// ONE: a context of one replica (its table in scalar registers).
+template <int PER, bool ONE>
 __global__ __launch_bounds__(256) void k_scan(DevState d, int flip, int lists, int pre) {
 	__shared__ int wsum[4];
 	__shared__ int sh_tile, sh_excl, sh_r0;
@@ -219,7 +220,7 @@ __global__ __launch_bounds__(256) void k_scan(DevState d, int flip, int lists, i
 	__syncthreads();
 	const int tile = sh_tile;
 	const int base = tile * (256 * PER) + t * PER;
-	if(t == 0 && lists) {
+	if(t == 0 && lists && !ONE) {
 		// replica of the tile's first cell
 		int lo = 0, hi = d.R;
 		while(hi - lo > 1) {
@@ -301,18 +302,19 @@ __global__ __launch_bounds__(256) void k_scan(DevState d, int flip, int lists, i
 			if(lists && v[e] > 0) {
 				// the next passes visit only non-empty cells: list them by
 				// colour, first counted per workgroup in LDS
-				int lo = sh_r0, hi = d.R;
-				while(hi - lo > 1) {
-					int mid = (lo + hi) >> 1;
-					if(d.reps[mid].cell_off <= c) lo = mid;
-					else hi = mid;
-				}
-				const Rep &rp = d.reps[lo];
+				int lo = ONE ? 0 : sh_r0, hi = d.R;
+				if constexpr(!ONE)
+					while(hi - lo > 1) {
+						int mid = (lo + hi) >> 1;
+						if(d.reps[mid].cell_off <= c) lo = mid;
+						else hi = mid;
+					}
+				const Rep &rp = d.reps[ONE ? 0 : lo];
 				const int cloc = c - rp.cell_off;
 				const int n1 = rp.g.n[1], n2 = rp.g.n[2];
 				const int ix = cloc / (n1 * n2), iy = (cloc / n2) % n1, iz = cloc % n2;
 				const int col = ((ix & 1) << 2) | ((iy & 1) << 1) | (iz & 1);
-				const int key = (lo - sh_r0) * 8 + col;
+				const int key = ONE ? col : (lo - sh_r0) * 8 + col;
 				if(cloc >= rp.g.n[0] * n1 * n2) {
 					// the parking cell: free slots, no particles to move
 				}
@@ -331,15 +333,15 @@ __global__ __launch_bounds__(256) void k_scan(DevState d, int flip, int lists, i
 	}
 	if(lists) {
 		__syncthreads();
-		for(int k = t; k < SCAN_KEYS; k += 256) {
+		for(int k = t; k < (ONE ? 8 : SCAN_KEYS); k += 256) {
 			const int n = lcnt[k];
-			if(n > 0) lbase[k] = atomicAdd(&d.ccount[(sh_r0 + k / 8) * 8 + k % 8], n);
+			if(n > 0) lbase[k] = atomicAdd(&d.ccount[((ONE ? 0 : sh_r0) + k / 8) * 8 + k % 8], n);
 		}
 		__syncthreads();
 #pragma unroll
 		for(int e = 0; e < PER; e++)
 			if(lkey[e] >= 0) {
-				const int r = sh_r0 + lkey[e] / 8;
+				const int r = ONE ? 0 : sh_r0 + lkey[e] / 8;
 				const Rep &rp = d.reps[r];
 				const int cloc = base + e - rp.cell_off;
 				d.clist[rp.cell_off + (lkey[e] & 7) * rp.act_cap + lbase[lkey[e]] + lpos[e]] = cloc;
@@ -483,7 +485,8 @@ void launch_rebuild(const DevState &d, int mode, uint64_t sweep, long long trace
 	const int lists = mode == REB_SHIFT ? 1 : 0;
 	if(d.scan_tile == SCAN_TILE_SMALL) {
 		// at most SCAN_SMALL_MAX_TILES tiles: one pass with a look-back
-		hipLaunchKernelGGL(k_scan<SCAN_TILE_SMALL / 256>, dim3(d.ntiles), dim3(256), 0, s, d, flip, lists, 0);
+		if(d.R == 1) hipLaunchKernelGGL((k_scan<SCAN_TILE_SMALL / 256, true>), dim3(d.ntiles), dim3(256), 0, s, d, flip, lists, 0);
+		else hipLaunchKernelGGL((k_scan<SCAN_TILE_SMALL / 256, false>), dim3(d.ntiles), dim3(256), 0, s, d, flip, lists, 0);
 	}
 	else {
 		// large tiles are taken only beyond SCAN_SMALL_MAX_TILES small ones,
@@ -493,7 +496,7 @@ void launch_rebuild(const DevState &d, int mode, uint64_t sweep, long long trace
 		              "grids of large tiles always exceed the look-back's tile limit");
 		hipLaunchKernelGGL(k_tile_sums<SCAN_TILE_LARGE / 256>, dim3(d.ntiles), dim3(256), 0, s, d);
 		hipLaunchKernelGGL(k_tile_scan, dim3(1), dim3(1024), 0, s, d);
-		hipLaunchKernelGGL(k_scan<SCAN_TILE_LARGE / 256>, dim3(d.ntiles), dim3(256), 0, s, d, flip, lists, 1);
+		hipLaunchKernelGGL((k_scan<SCAN_TILE_LARGE / 256, false>), dim3(d.ntiles), dim3(256), 0, s, d, flip, lists, 1);
 	}
 	if(mode == REB_SHIFT && d.R != 1) hipLaunchKernelGGL(k_pass_prefix, dim3(1), dim3(64), 0, s, d);
 	unsigned gs = gp;
