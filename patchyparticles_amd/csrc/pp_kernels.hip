@@ -160,6 +160,46 @@ __global__ void k_count(DevState d, int mode, uint64_t sweep, long long trace_po
 // (replica, colour) keys of the non-empty-cell lists counted per scan workgroup
 constexpr int SCAN_KEYS = 512;
 
+// A lane's PER consecutive cells of a tile: whole vectors where all PER
+// are in the grid (base is a multiple of PER, the arrays 16-byte aligned).
+template <int PER>
+__device__ __forceinline__ void load_cells(const int *p, int base, int n, int v[PER]) {
+	static_assert(PER == 2 || PER % 4 == 0, "2 or a multiple of 4 cells per lane");
+	if(base + PER <= n) {
+		if constexpr(PER == 2) {
+			const int2 q = *(const int2 *) (p + base);
+			v[0] = q.x;
+			v[1] = q.y;
+		}
+		else
+#pragma unroll
+			for(int e = 0; e < PER; e += 4) {
+				const int4 q = *(const int4 *) (p + base + e);
+				v[e] = q.x;
+				v[e + 1] = q.y;
+				v[e + 2] = q.z;
+				v[e + 3] = q.w;
+			}
+	}
+	else
+#pragma unroll
+		for(int e = 0; e < PER; e++) v[e] = base + e < n ? p[base + e] : 0;
+}
+
+template <int PER>
+__device__ __forceinline__ void store_cells(int *p, int base, int n, const int v[PER]) {
+	if(base + PER <= n) {
+		if constexpr(PER == 2) *(int2 *) (p + base) = make_int2(v[0], v[1]);
+		else
+#pragma unroll
+			for(int e = 0; e < PER; e += 4) *(int4 *) (p + base + e) = make_int4(v[e], v[e + 1], v[e + 2], v[e + 3]);
+	}
+	else
+#pragma unroll
+		for(int e = 0; e < PER; e++)
+			if(base + e < n) p[base + e] = v[e];
+}
+
 // Exclusive scan of the cell counts in one pass (decoupled look-back):
 // workgroups take tiles in dispatch order from a ticket, publish their tile
 // sum, then read their predecessors' published values, 64 at a time, back to
@@ -172,10 +212,11 @@ template <int PER>
 __global__ __launch_bounds__(256) void k_tile_sums(DevState d) {
 	__shared__ int part[4];
 	const int t = threadIdx.x, base = blockIdx.x * (256 * PER) + t * PER;
+	int v[PER];
+	load_cells<PER>(d.count, base, d.C_total, v);
 	int tot = 0;
 #pragma unroll
-	for(int e = 0; e < PER; e++)
-		if(base + e < d.C_total) tot += d.count[base + e];
+	for(int e = 0; e < PER; e++) tot += v[e];
 	tot = team_sum<64>(tot);
 	if((t & 63) == 0) part[t >> 6] = tot;
 	__syncthreads();
@@ -231,13 +272,10 @@ __global__ __launch_bounds__(256) void k_scan(DevState d, int flip, int lists, i
 		sh_r0 = lo;
 	}
 	int v[PER];
+	load_cells<PER>(d.count, base, d.C_total, v);
 	int tot = 0;
 #pragma unroll
-	for(int e = 0; e < PER; e++) {
-		int c = base + e;
-		v[e] = c < d.C_total ? d.count[c] : 0;
-		tot += v[e];
-	}
+	for(int e = 0; e < PER; e++) tot += v[e];
 	int lane = t & 63, wid = t >> 6;
 	const int inc = wave_incl_scan(tot);
 	if(lane == 63) wsum[wid] = inc;
@@ -289,7 +327,19 @@ __global__ __launch_bounds__(256) void k_scan(DevState d, int flip, int lists, i
 	__syncthreads();
 	int woff = 0;
 	for(int w = 0; w < wid; w++) woff += wsum[w];
-	int run = sh_excl + woff + inc - tot;
+	// the cells' first slots; their counts cleared for the next rebuild
+	{
+		int first[PER], zero[PER];
+		int run = sh_excl + woff + inc - tot;
+#pragma unroll
+		for(int e = 0; e < PER; e++) {
+			first[e] = run;
+			zero[e] = 0;
+			run += v[e];
+		}
+		store_cells<PER>(d.start, base, d.C_total, first);
+		store_cells<PER>(d.count, base, d.C_total, zero);
+	}
 	int lkey[PER], lpos[PER];
 #pragma unroll
 	for(int e = 0; e < PER; e++) {
@@ -297,8 +347,6 @@ __global__ __launch_bounds__(256) void k_scan(DevState d, int flip, int lists, i
 		lkey[e] = -1;
 		lpos[e] = 0;
 		if(c < d.C_total) {
-			d.start[c] = run;
-			d.count[c] = 0;
 			if(lists && v[e] > 0) {
 				// the next passes visit only non-empty cells: list them by
 				// colour, first counted per workgroup in LDS
@@ -329,7 +377,6 @@ __global__ __launch_bounds__(256) void k_scan(DevState d, int flip, int lists, i
 				}
 			}
 		}
-		run += v[e];
 	}
 	if(lists) {
 		__syncthreads();
